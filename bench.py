@@ -1,0 +1,203 @@
+"""Benchmark: PB solves/hour on one MI355X.
+
+A step is one complete PB solve through libtabi_hip.so (tabi_solve): tree build,
+interaction lists, leaf-block LU factorisation, right-hand side, GMRES to
+tol 1e-4 from a zero start (with the reference's MAXL=10/NRMAX=10 left
+preconditioning and stopping rule), and the solvation energy.
+
+Besides the JSON line's throughput, the run measures the dominant per-matvec
+kernel live with HIP events on the library's stream (tabi_ctx_kernel_times) and
+reports its algorithmic FLOP rate against the FP64 peak, and times the CPU oracle
+(a line-for-line C restatement of the reference, bit-identical to it on the
+golden cases) on a bounded sample of the same workload on one host core.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from math import comb
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tabi-pb_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from tabipb import driver, hip, meshio  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "PB solves/hour at ~500k faces, ΔG err<1e-3 vs FP64; GMRES iters learned vs 0"
+
+
+def w_far(p):
+    """Algorithmic flops per accepted (target, cluster) pair."""
+    return 20 * comb(p + 5, 3) + 64 * comb(p + 3, 3) + 40
+
+
+def w_mom(p):
+    return 9 * comb(p + 3, 3)
+
+
+W_NEAR = 62
+
+
+def workload(name):
+    """Returns (driver.Problem, description dict)."""
+    usr = meshio.UsrData(fname="mol", den="10", maxparnode=500, order=3, mac=0.8)
+    if name.startswith("sphere"):
+        sub = {"sphere20k": 5, "sphere82k": 6, "sphere328k": 7}[name]
+        v, nv, f = meshio.icosphere(sub, 2.0)
+        xyz, nrm, area = driver.face_geometry(v, nv, f)
+        prob = driver.Problem(usr, xyz, nrm, area, np.zeros((1, 3)), np.ones(1))
+        desc = dict(workload=f"Born sphere a=2A q=+1, icosphere {len(f)} faces", faces=len(f),
+                    atoms=1)
+        return prob, desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(prob, iters, budget_s=12.0):
+    """Times the oracle on one core on a bounded sample of the same problem and
+    extrapolates the serial reference time of one solve."""
+    import oracle as O
+    xyz, usr = prob.xyz, prob.usr
+    n = xyz.shape[0]
+    prm = O.params(prob.eps, prob.kappa, usr.mac, usr.order, usr.maxparnode)
+    t0 = time.perf_counter()
+    txyz, order, nodes = O.build_tree(xyz, usr.maxparnode)
+    t_tree = time.perf_counter() - t0
+    tn, ta = prob.nrm[order], prob.area[order]
+    x = np.random.default_rng(0).standard_normal(2 * n)
+    rng = np.random.default_rng(1)
+    # matvec: targets sampled uniformly over the surface
+    k = max(64, min(n, 2000))
+    targets = np.sort(rng.choice(n, size=k, replace=False))
+    t0 = time.perf_counter()
+    O.treecode_matvec_targets(prm, txyz, tn, ta, nodes, x, targets)
+    t_mv_s = time.perf_counter() - t0
+    per_target = t_mv_s / k
+    # preconditioner: a sample of leaves, factor + solve as the reference does
+    ib, ie = O.leaves(nodes)
+    sel = rng.permutation(len(ib))
+    tl, done, rows = 0.0, 0, 0
+    for l in sel:
+        t0 = time.perf_counter()
+        O.precond(prm, txyz, tn, ta, ib[l:l + 1], ie[l:l + 1], x)
+        tl += time.perf_counter() - t0
+        done += 1
+        rows += int(ie[l] - ib[l] + 1)
+        if tl > budget_s / 2:
+            break
+    t_ms = tl / done * len(ib)
+    t_mv = per_target * n
+    nq = prob.q.shape[0]
+    # right-hand side and energy: O(n * nq) pair sums, timed on a row sample
+    t0 = time.perf_counter()
+    O.rhs(prob.xyz[:min(n, 4000)], prob.nrm[:min(n, 4000)], prob.qpos, prob.q, usr.epsp)
+    t_rhs = (time.perf_counter() - t0) * n / min(n, 4000)
+    t_solve = t_tree + t_rhs * 2.5 + iters * t_mv + (iters + 2) * t_ms
+    sample = (f"oracle (C restatement of the reference, gcc -O2, 1 thread): {k} of {n} "
+              f"matvec targets ({t_mv_s:.1f}s), {done} of {len(ib)} leaf factor+solves "
+              f"({tl:.1f}s), rhs rows {min(n, 4000)}; per-solve model "
+              f"iters*matvec+(iters+2)*msolve+tree+rhs+energy with the GPU's iters={iters}")
+    return dict(value=3600.0 / t_solve, unit="solves/hour", cores=1, kind="port",
+                sample=sample, t_matvec_s=t_mv, t_msolve_s=t_ms, t_solve_s=t_solve)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sphere328k")
+    ap.add_argument("--matvecs", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the solve would shard as one all-gather of x per matvec).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "no GPU visible"
+    prob, desc = workload(a.workload)
+    prm = prob.params()
+    p = prm.order
+    print(f"[bench] workload {desc['workload']} n={prob.xyz.shape[0]}", file=sys.stderr, flush=True)
+
+    for _ in range(a.warmup):
+        res = driver.solve(prob, tol=1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = driver.solve(prob, tol=1e-4)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    print(f"[bench] solve {dt * 1e3:.1f} ms iter={res['iter']} dG={res['soleng']:.6f} "
+          f"phases={res['times_ms']}", file=sys.stderr, flush=True)
+
+    # dominant kernel: per-matvec kernels timed with events on the library stream
+    S = hip.Surface(prm, prob.xyz, prob.nrm, prob.area)
+    st = S.stats()
+    n = S.n
+    dx = torch.from_numpy(np.random.default_rng(0).standard_normal(2 * n)).cuda()
+    dy = torch.empty_like(dx)
+    torch.cuda.synchronize()
+    S.matvec_dev(dx.data_ptr(), dy.data_ptr())
+    S.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(a.matvecs):
+        S.matvec_dev(dx.data_ptr(), dy.data_ptr())
+    kt = S.kernel_times()
+    S.profile(False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(a.matvecs):
+        S.matvec_dev(dx.data_ptr(), dy.data_ptr())
+    torch.cuda.synchronize()
+    mv_ms = (time.perf_counter() - t1) / a.matvecs * 1e3
+    flops = {"near": W_NEAR * st["p_near"], "far": w_far(p) * st["p_far"],
+             "moments": w_mom(p) * st["p_mom"]}
+    avg = {k: kt[k][0] / max(kt[k][1], 1) for k in ("near", "far", "moments", "reduce")}
+    dom = max(("near", "far"), key=lambda k: avg[k])
+    achieved = flops[dom] / (avg[dom] * 1e-3) / 1e12
+    mv_flops = sum(flops.values()) + 4 * n
+    print(f"[bench] matvec {mv_ms:.3f} ms ({mv_flops / (mv_ms * 1e-3) / 1e12:.2f} TFLOP/s alg); "
+          f"kernel avg ms {avg}; stats {st}", file=sys.stderr, flush=True)
+    ms_t = 0.0
+    S.profile(True)
+    for _ in range(a.matvecs):
+        S.msolve_dev(dy.data_ptr(), dx.data_ptr())
+    ms_t = S.kernel_times()["msolve"]
+    S.profile(False)
+    msolve_ms = ms_t[0] / max(ms_t[1], 1)
+    S.close()
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(prob, res["iter"])
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+
+    value = 3600.0 / dt
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/hour", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": dict(desc, leaf=prm.maxparnode, order=p, theta=prm.theta, tol=1e-4,
+                       gmres_iter=res["iter"], soleng_kcal=res["soleng"],
+                       matvec_ms=mv_ms, msolve_ms=msolve_ms,
+                       matvecs_per_s=1e3 / mv_ms, phases_ms=res["times_ms"]),
+        "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "alg_flops_per_launch": flops[dom], "avg_launch_ms": avg[dom]},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
+                                                  ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

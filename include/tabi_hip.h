@@ -1,0 +1,137 @@
+/* tabi_hip.h -- C ABI of the MI355X (gfx950) treecode-accelerated boundary-integral
+ * Poisson-Boltzmann solver (libtabi_hip.so).
+ *
+ * The library replaces the treecode matvec, the leaf-block preconditioner and
+ * the set-up work of gengwh/TABI-PB behind the two callbacks its SLATEC DGMRES
+ * calls.  All vectors are contiguous float64 in TREE ORDER, laid out as
+ * [phi(1..n); dphi/dn(1..n)] with n = number of faces (the reference's
+ * N = 2*nface, src/main.f90:84).  Points are interleaved x,y,z per face (the
+ * memory layout of the Fortran arrays tr_xyz(3,n) / tr_q(3,n)).
+ *
+ * Every function returns 0 on success and a nonzero status on failure; the
+ * message is available from tabi_last_error().  The library holds one default
+ * solver context (the drop-in path) plus any number of explicit handles.
+ * Calls on one context are not re-entrant (the reference's matvec mutates
+ * global state too, src/treecode3d_pb.f:1169-1182).
+ */
+#ifndef TABI_HIP_H
+#define TABI_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Solver parameters.  eps = eps_w/eps_p (src/main.f90:65); kappa as formed at
+ * src/main.f90:62-63; order = Taylor order p, maxparnode = leaf size, theta = MAC
+ * (usrdata.in lines 6-8, src/main.f90:45-47). */
+typedef struct {
+    double eps;
+    double kappa;
+    double theta;
+    int32_t order;       /* 1..8 */
+    int32_t maxparnode;  /* >= 1 */
+    int32_t precision;   /* 0: FP64 throughout; 1: mixed (FP32 near/far field with
+                            compensated per-target sums, FP64 elsewhere) */
+    int32_t flags;       /* bit 0: preconditioner with matvec-consistent L2/L3
+                            placement instead of the reference's (tc.f:1481-1488) */
+} tabi_params;
+
+typedef struct tabi_ctx tabi_ctx;
+
+/* ---- default-context drop-in API (what the Fortran driver binds) ---------- */
+
+/* Set-up for one surface (replaces treecode_initialization, src/main.f90:329-464,
+ * and SETUP/CREATE_TREE, src/treecode3d_pb.f:54-356).  Inputs in ORIGINAL face
+ * order: centroids xyz[3n], unit normals nrm[3n], areas area[n] (as form_matrix
+ * computes them, src/main.f90:569-594).  Builds the tree, the per-target
+ * interaction lists and the factored leaf blocks on the GPU.  orderarr_out[n]
+ * receives the 1-based tree permutation (tree position i holds original face
+ * orderarr(i), src/treecode3d_pb.f:30); nleaf_out the leaf count. */
+int tabi_init(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+              const double *area, int32_t *orderarr_out, int32_t *nleaf_out);
+
+/* y = A x, the MATVEC callback (MATVECmain, src/main.f90:467-488, which calls
+ * TREE_COMPP_PB, src/treecode3d_pb.f:1130-1189).  Host pointers, n2 = 2n.
+ * Reproduces the shortcut y = x when sum|x| < 1e-10 (src/main.f90:478-479). */
+int tabi_matvec(int32_t n2, const double *x, double *y);
+
+/* z = M^-1 r, the MSOLVE callback (MSOLVEdric, src/main.f90:511-529, which calls
+ * LEAFMATVECpara, src/treecode3d_pb.f:1393-1542).  Host pointers. */
+int tabi_msolve(int32_t n2, const double *r, double *z);
+
+/* Right-hand side in tree order (form_matrix, src/main.f90:597-607) for nq point
+ * charges q at qpos[3nq], interior dielectric eps_p. */
+int tabi_rhs(int32_t nq, const double *qpos, const double *q, double eps_p, double *b);
+
+/* Solvation and Coulomb energies in kcal/mol from a tree-order solution x
+ * (src/main.f90:144-157, potential_molecule src/main.f90:614-657). */
+int tabi_energy(int32_t nq, const double *qpos, const double *q, double eps_p,
+                const double *x, double *soleng, double *couleng);
+
+/* GPU-resident restarted GMRES with the reference's settings and stopping rule
+ * (DGMRES with ITOL=2, JPRE=-1, JSCAL=0, src/dgmres_dep.f:279-852; MAXL/NRMAX
+ * from src/main.f90:93-97).  b and x (initial guess in, solution out) are host
+ * pointers in tree order; iter/err/ierr as DGMRES returns them. */
+int tabi_gmres(const double *b, double *x, double tol, int32_t maxl, int32_t nrmax,
+               int32_t *iter, double *err, int32_t *ierr);
+
+/* Releases the default context. */
+void tabi_finalize(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *tabi_last_error(void);
+
+/* Counters of the default context: {nface, nnode, nleaf, P_near, P_far, P_mom,
+ * n_far_items, n_near_items, n_tasks, lu_bytes}. */
+int tabi_stats(int64_t *out10);
+
+/* ---- explicit-handle API (batched and device-resident use) ---------------- */
+
+int tabi_ctx_create(const tabi_params *prm, int32_t n, const double *xyz,
+                    const double *nrm, const double *area, int32_t *orderarr_out,
+                    tabi_ctx **out);
+void tabi_ctx_destroy(tabi_ctx *c);
+int tabi_ctx_stats(tabi_ctx *c, int64_t *out10);
+
+/* device-pointer matvec / preconditioner on the context's stream */
+int tabi_ctx_matvec_dev(tabi_ctx *c, const double *d_x, double *d_y);
+int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z);
+/* host-pointer forms */
+int tabi_ctx_matvec(tabi_ctx *c, const double *x, double *y);
+int tabi_ctx_msolve(tabi_ctx *c, const double *r, double *z);
+int tabi_ctx_rhs(tabi_ctx *c, int32_t nq, const double *qpos, const double *q,
+                 double eps_p, double *b);
+int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q,
+                    double eps_p, const double *x, double *soleng, double *couleng);
+int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
+                   int32_t nrmax, int32_t *iter, double *err, int32_t *ierr);
+
+/* Per-kernel device timing on the context's stream.  With profiling enabled the
+ * library brackets each launch of the timed kernels with HIP events; times
+ * accumulate until reset.  Slots: 0 moments, 1 near field, 2 far field,
+ * 3 reduction, 4 preconditioner solve, 5 list build, 6 LU factorisation,
+ * 7 right-hand side + energy.  out_ms[8] gets the summed milliseconds,
+ * out_launches[8] the launch counts. */
+int tabi_ctx_profile(tabi_ctx *c, int32_t enable);
+int tabi_ctx_kernel_times(tabi_ctx *c, double *out_ms, int64_t *out_launches);
+
+/* One full PB solve for a surface already in HBM-resident form: builds the tree,
+ * interaction lists and leaf factors, forms the right-hand side, runs GMRES from
+ * a zero start to tol, and evaluates the energies.  Host inputs in original face
+ * order; x_out[2n] (may be NULL) receives the tree-order solution, orderarr_out
+ * (may be NULL) the 1-based permutation. */
+int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+               const double *area, int32_t nq, const double *qpos, const double *q,
+               double eps_p, double tol, double *x_out, int32_t *orderarr_out,
+               int32_t *iter, double *err, int32_t *ierr, double *soleng,
+               double *couleng);
+
+/* Per-phase device times (ms) of the last tabi_solve on this thread:
+ * {tree, lists, precond_factor, rhs, gmres, energy, total}. */
+int tabi_last_solve_times(double *out7);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -117,6 +117,18 @@ def treecode_matvec(prm, xyz, nrm, area, nodes, x):
     return y, cnt
 
 
+def treecode_matvec_targets(prm, xyz, nrm, area, nodes, x, targets):
+    xyz, nrm, area, x = _d(xyz), _d(nrm), _d(area), _d(x)
+    targets = _i(targets)
+    n = xyz.shape[0]
+    y = np.zeros(2 * len(targets))
+    cnt = np.zeros(3, dtype=np.int64)
+    lib().or_treecode_matvec_targets(C.byref(prm), C.c_int32(n), _p(xyz), _p(nrm), _p(area),
+                                     _p(nodes), _p(x), C.c_int32(len(targets)), _p(targets),
+                                     _p(y), _p(cnt))
+    return y.reshape(-1, 2), cnt
+
+
 def direct_matvec(prm, xyz, nrm, area, x):
     xyz, nrm, area, x = _d(xyz), _d(nrm), _d(area), _d(x)
     n = xyz.shape[0]

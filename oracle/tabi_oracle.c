@@ -644,9 +644,30 @@ static int32_t count_nodes(const or_node *nodes, int32_t id)
     return k;
 }
 
+static void treecode_targets(const or_params *prm, int32_t n, const double *xyz,
+                             const double *nrm, const double *area, const or_node *nodes,
+                             const double *x, int32_t nt, const int32_t *targets, double *y,
+                             int64_t *counts);
+
 void or_treecode_matvec(const or_params *prm, int32_t n, const double *xyz,
                         const double *nrm, const double *area, const or_node *nodes,
                         const double *x, double *y, int64_t *counts)
+{
+    treecode_targets(prm, n, xyz, nrm, area, nodes, x, -1, NULL, y, counts);
+}
+
+void or_treecode_matvec_targets(const or_params *prm, int32_t n, const double *xyz,
+                                const double *nrm, const double *area, const or_node *nodes,
+                                const double *x, int32_t nt, const int32_t *targets,
+                                double *y, int64_t *counts)
+{
+    treecode_targets(prm, n, xyz, nrm, area, nodes, x, nt, targets, y, counts);
+}
+
+static void treecode_targets(const or_params *prm, int32_t n, const double *xyz,
+                             const double *nrm, const double *area, const or_node *nodes,
+                             const double *x, int32_t nt, const int32_t *targets, double *y,
+                             int64_t *counts)
 {
     mv_ctx c;
     memset(&c, 0, sizeof(c));
@@ -682,7 +703,9 @@ void or_treecode_matvec(const or_params *prm, int32_t n, const double *xyz,
             for (int e = 0; e < 3; ++e) q[7 + 3 * d + e] = -(nv[d] * a * phi);
     }
     double pre1 = 0.5 * (1.0 + prm->eps), pre2 = 0.5 * (1.0 + 1.0 / prm->eps);
-    for (int32_t i = 0; i < n; ++i) {
+    const int32_t ntarg = nt < 0 ? n : nt;
+    for (int32_t it = 0; it < ntarg; ++it) {
+        const int32_t i = nt < 0 ? it : targets[it];
         c.tx = xyz[3 * i]; c.ty = xyz[3 * i + 1]; c.tz = xyz[3 * i + 2];
         const double *nv = nrm + 3 * (int64_t)i;
         c.tq[0] = 1.0;
@@ -693,8 +716,13 @@ void or_treecode_matvec(const or_params *prm, int32_t n, const double *xyz,
         c.self = i;
         double peng[2];
         compp_tree(&c, 0, peng);
-        y[i] = pre1 * x[i] - peng[0];
-        y[n + i] = pre2 * x[n + i] - peng[1];
+        if (nt < 0) {
+            y[i] = pre1 * x[i] - peng[0];
+            y[n + i] = pre2 * x[n + i] - peng[1];
+        } else {
+            y[2 * it] = pre1 * x[i] - peng[0];
+            y[2 * it + 1] = pre2 * x[n + i] - peng[1];
+        }
     }
     if (counts) { counts[0] = c.npair_near; counts[1] = c.npair_far; counts[2] = c.nmom; }
     for (int32_t i = 0; i < nn; ++i) free(c.ms[i]);

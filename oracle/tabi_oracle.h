@@ -72,6 +72,13 @@ void or_treecode_matvec(const or_params *prm, int32_t n, const double *xyz,
                         const double *nrm, const double *area, const or_node *nodes,
                         const double *x, double *y, int64_t *counts);
 
+/* the same for a subset of targets (tree positions targets[0..nt-1]);
+ * y receives {(Ax)_i, (Ax)_{n+i}} interleaved per listed target */
+void or_treecode_matvec_targets(const or_params *prm, int32_t n, const double *xyz,
+                                const double *nrm, const double *area, const or_node *nodes,
+                                const double *x, int32_t nt, const int32_t *targets,
+                                double *y, int64_t *counts);
+
 /* O(n^2) direct-sum matvec with the same kernel and self exclusion */
 void or_direct_matvec(const or_params *prm, int32_t n, const double *xyz,
                       const double *nrm, const double *area, const double *x, double *y);

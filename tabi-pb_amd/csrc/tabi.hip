@@ -1,0 +1,1831 @@
+// MI355X (gfx950) treecode-accelerated boundary-integral Poisson-Boltzmann solver.
+//
+// Device data live in tree order (faces sorted into leaf order by the tree
+// build), so every target group of 64 consecutive faces -- one wavefront -- is a
+// spatially compact patch, and every leaf is a contiguous source range.
+//
+// Per matvec (y = A x, TREE_COMPP_PB, src/treecode3d_pb.f:1130-1189):
+//   k_mom_raw / k_mom_final  cluster moments of the 4 distinct source-charge sets
+//                            (area*dphi/dn and n_d*area*phi), folded with the
+//                            derivative factors into 8 contraction vectors
+//   k_far<P>                 particle-cluster Taylor far field, one wave per task
+//   k_near                   direct near field, one wave per task, sources read
+//                            wave-uniformly through the scalar cache
+//   k_reduce                 y = pre*x - (ordered sum of the task partials)
+// Per preconditioner solve (LEAFMATVECpara, src/treecode3d_pb.f:1393-1542):
+//   k_lu_solve               blocked forward/back substitution with the leaf
+//                            factors computed once at set-up
+// Set-up: k_lists (per-target MAC traversal with lane masks), k_lu_assemble,
+//   k_lu_panel / k_lu_swap_trsm / k_lu_gemm (batched right-looking LU with
+//   partial pivoting over all leaf blocks at once).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tabi_hip.h"
+#include "tabi_internal.h"
+#include "taylor.h"
+
+using tabi::HostTree;
+using u64 = unsigned long long;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279;
+constexpr double kOneOver4Pi = 0.25 / kPi;
+constexpr int kWave = 64;
+constexpr int kMaxOrder = 6;
+constexpr int kStack = 512;      // per-wave traversal stack (entries)
+constexpr int kLuNB = 32;        // LU panel width
+
+thread_local std::string g_err;
+thread_local double g_times[7] = {0, 0, 0, 0, 0, 0, 0};
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw Error(std::string(#expr) + " failed: " + hipGetErrorString(e_));     \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) count = 1;
+        HIPCHK(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct NearTask {
+    int32_t g, s0, s1, pad;
+    u64 mask;
+};
+struct FarTask {
+    int32_t g, i0, i1, pad;
+};
+
+}  // namespace
+
+struct tabi_ctx {
+    tabi_params prm{};
+    int32_t n = 0, ngroups = 0, p = 3, nj = 0, nk = 0;
+    hipStream_t s = nullptr;
+    HostTree tree;
+    // geometry, tree order
+    DBuf<double> tx, ty, tz, tnx, tny, tnz, tarea;
+    DBuf<double> src8;                   // {x,y,z,nx,ny,nz,area,0} per face
+    // nodes
+    DBuf<double4> ngeo;                  // {xmid,ymid,zmid,radius}
+    DBuf<int32_t> nnum, nchild, ncoff, nkids, nibeg;
+    // interaction lists (per target group, CSR)
+    std::vector<int32_t> h_far_off, h_near_off;
+    DBuf<int32_t> far_off, near_off, far_node, near_leaf;
+    DBuf<u64> far_mask, near_mask;
+    DBuf<int32_t> cnt_far, cnt_near, need_mom, err_flag;
+    // moments
+    int32_t nmom = 0, nchunk = 0;
+    DBuf<int32_t> node2mom, chunk_node, chunk_beg, chunk_end, mom_chunk_off, mom_node;
+    DBuf<double> mom_part, mom_w;
+    // tasks
+    int32_t n_near_task = 0, n_far_task = 0;
+    DBuf<NearTask> near_tasks;
+    DBuf<FarTask> far_tasks;
+    DBuf<int32_t> near_task_off, far_task_off;
+    DBuf<double2> part_near, part_far;
+    // preconditioner
+    int32_t nleaf = 0, mmax = 0;
+    std::vector<int64_t> h_blk_off;
+    std::vector<int32_t> h_blk_m;
+    DBuf<int64_t> blk_off;
+    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, leaf_of, lu_info;
+    DBuf<double> lu;
+    // vectors
+    DBuf<double> vx, vy, red;
+    // per-kernel timing (events on stream s)
+    bool prof = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // stats
+    int64_t p_near = 0, p_far = 0, p_mom = 0, n_far_items = 0, n_near_items = 0;
+};
+
+namespace {
+
+// Brackets a launch sequence with events when profiling is on; the elapsed
+// time is added to slot k once the stream reaches the end event.
+struct KTimer {
+    tabi_ctx *c;
+    int k;
+    KTimer(tabi_ctx *c_, int k_) : c(c_), k(k_)
+    {
+        if (c->prof) HIPCHK(hipEventRecord(c->ev[0], c->s));
+    }
+    ~KTimer() noexcept(false)
+    {
+        if (!c->prof) return;
+        HIPCHK(hipEventRecord(c->ev[1], c->s));
+        HIPCHK(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->kt_ms[k] += ms;
+        c->kt_n[k] += 1;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// ---------------------------------------------------------------------------
+// interaction lists: one wave per group of 64 consecutive targets walks the tree
+// depth-first (children in order, as COMPP_TREE does, src/treecode3d_pb.f:359-436)
+// carrying the mask of lanes that have failed the MAC at every ancestor.  At a
+// node the MAC is evaluated per lane exactly as the reference evaluates it
+// (src/treecode3d_pb.f:379-393): accepted lanes get a far item, the remaining
+// lanes get a near item at a leaf or descend.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool mac_accept(double4 g, int numpar, double px, double py,
+                                           double pz, double theta)
+{
+#pragma clang fp contract(off)
+    const double tx = g.x - px, ty = g.y - py, tz = g.z - pz;
+    const double dist = sqrt(tx * tx + ty * ty + tz * tz);
+    return (g.w < dist * theta) && (numpar > 40);
+}
+
+__global__ void __launch_bounds__(256) k_lists(int pass, int n, int ngroups,
+    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
+    const double4 *__restrict__ ngeo, const int32_t *__restrict__ nnum,
+    const int32_t *__restrict__ nchild, const int32_t *__restrict__ ncoff,
+    const int32_t *__restrict__ nkids, double theta, int32_t *__restrict__ cnt_far,
+    int32_t *__restrict__ cnt_near, const int32_t *__restrict__ far_off,
+    const int32_t *__restrict__ near_off, int32_t *__restrict__ far_node,
+    u64 *__restrict__ far_mask, int32_t *__restrict__ near_leaf, u64 *__restrict__ near_mask,
+    int32_t *__restrict__ need_mom, int32_t *__restrict__ err_flag)
+{
+    __shared__ int32_t s_node[4][kStack];
+    __shared__ u64 s_mask[4][kStack];
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int g = blockIdx.x * 4 + w;
+    if (g >= ngroups) return;
+    const int i = g * kWave + lane;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    int sp = 0;
+    const u64 vmask = __ballot(valid);
+    if (lane == 0) { s_node[w][0] = 0; s_mask[w][0] = vmask; }
+    sp = 1;
+    int nf = 0, nn = 0;
+    int fbase = 0, nbase = 0;
+    if (pass == 1) { fbase = far_off[g]; nbase = near_off[g]; }
+    while (sp > 0) {
+        --sp;
+        const int nd = s_node[w][sp];
+        const u64 m = s_mask[w][sp];
+        const double4 geo = ngeo[nd];
+        const int np = nnum[nd];
+        bool acc = false;
+        if ((m >> lane) & 1ull) acc = mac_accept(geo, np, px, py, pz, theta);
+        const u64 accm = __ballot(acc) & m;
+        if (accm) {
+            if (pass == 1 && lane == 0) {
+                far_node[fbase + nf] = nd;
+                far_mask[fbase + nf] = accm;
+                need_mom[nd] = 1;
+            }
+            ++nf;
+        }
+        const u64 rest = m & ~accm;
+        if (rest) {
+            const int nc = nchild[nd];
+            if (nc == 0) {
+                if (pass == 1 && lane == 0) {
+                    near_leaf[nbase + nn] = nd;
+                    near_mask[nbase + nn] = rest;
+                }
+                ++nn;
+            } else {
+                if (sp + nc > kStack) {
+                    if (lane == 0) atomicOr(err_flag, 1);
+                    break;
+                }
+                const int off = ncoff[nd];
+                if (lane < nc) {
+                    s_node[w][sp + nc - 1 - lane] = nkids[off + lane];
+                    s_mask[w][sp + nc - 1 - lane] = rest;
+                }
+                sp += nc;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (pass == 0 && lane == 0) {
+        cnt_far[g] = nf;
+        cnt_near[g] = nn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// moments
+// ---------------------------------------------------------------------------
+template <int P>
+struct MomShape {
+    static constexpr int NK = (P + 1) * (P + 2) * (P + 3) / 6;   // |k| <= P
+    static constexpr int L = P + 2;
+    static constexpr int NJ = (L + 1) * (L + 2) * (L + 3) / 6;   // |j| <= P+2
+};
+
+// inverse of tabi::tidx(): compact index -> exponents (i, j, k)
+__device__ __forceinline__ void tdecode(int id, int &i, int &j, int &k)
+{
+    int d = 0;
+    while ((d + 1) * (d + 2) * (d + 3) / 6 <= id) ++d;
+    int r = id - d * (d + 1) * (d + 2) / 6;
+    int s = 0;
+    while ((s + 1) * (s + 2) / 2 <= r) ++s;
+    k = r - s * (s + 1) / 2;
+    j = s - k;
+    i = d - s;
+}
+
+constexpr int kMomChunk = 512;
+
+// raw moments of one chunk of a node's particles:
+//   M_psi(k) = sum A_j psi_j d^k,  M_d(k) = sum (n_d A_j) phi_j d^k   (COMP_MS,
+// src/treecode3d_pb.f:859-916, with pb_kernel's source charges, src/main.f90:671-692)
+template <int P>
+__global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restrict__ chunk_node,
+    const int32_t *__restrict__ chunk_beg, const int32_t *__restrict__ chunk_end,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, const double *__restrict__ tarea,
+    const double *__restrict__ x, double *__restrict__ part)
+{
+    constexpr int NK = MomShape<P>::NK;
+    constexpr int NPART = 256 / NK;
+    __shared__ double sd[7][kMomChunk];
+    __shared__ double sred[4][256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int nd = chunk_node[c], b = chunk_beg[c], e = chunk_end[c];
+    const double4 geo = ngeo[nd];
+    const int cnt = e - b;
+    for (int q = t; q < cnt; q += 256) {
+        const int i = b + q;
+        const double a = tarea[i], phi = x[i], psi = x[n + i];
+        sd[0][q] = tx[i] - geo.x;
+        sd[1][q] = ty[i] - geo.y;
+        sd[2][q] = tz[i] - geo.z;
+        sd[3][q] = a * psi;
+        sd[4][q] = tnx[i] * a * phi;
+        sd[5][q] = tny[i] * a * phi;
+        sd[6][q] = tnz[i] * a * phi;
+    }
+    __syncthreads();
+    const int k = t % NK, part_id = t / NK;
+    double m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    if (part_id < NPART) {
+        int e1, e2, e3;
+        tdecode(k, e1, e2, e3);
+        for (int q = part_id; q < cnt; q += NPART) {
+            double pxv = 1.0, pyv = 1.0, pzv = 1.0;
+            const double dx = sd[0][q], dy = sd[1][q], dz = sd[2][q];
+            for (int r = 0; r < e1; ++r) pxv *= dx;
+            for (int r = 0; r < e2; ++r) pyv *= dy;
+            for (int r = 0; r < e3; ++r) pzv *= dz;
+            const double mono = pxv * pyv * pzv;
+            m0 += sd[3][q] * mono;
+            m1 += sd[4][q] * mono;
+            m2 += sd[5][q] * mono;
+            m3 += sd[6][q] * mono;
+        }
+    }
+    sred[0][t] = m0; sred[1][t] = m1; sred[2][t] = m2; sred[3][t] = m3;
+    __syncthreads();
+    if (t < NK) {
+        double r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        for (int q = 0; q < NPART; ++q) {
+            r0 += sred[0][q * NK + t]; r1 += sred[1][q * NK + t];
+            r2 += sred[2][q * NK + t]; r3 += sred[3][q * NK + t];
+        }
+        double *o = part + (size_t)c * 4 * NK;
+        o[t] = r0; o[NK + t] = r1; o[2 * NK + t] = r2; o[3 * NK + t] = r3;
+    }
+}
+
+// Sum a node's chunk partials in chunk order and fold the moments with the
+// derivative factors der_cof (src/main.f90:380-399) and the kernel combination
+// of COMPP_TREE_PB (src/treecode3d_pb.f:479-485) into 8 vectors over |j| <= P+2:
+//   far potential   peng1 = sum_j a0_j W[0][j] + ak_j W[1][j]
+//   far normal der. peng2 = sum_d n0_d sum_j a0_j W[2+d][j] + ak_j W[5+d][j]
+// where a0/ak are the Taylor coefficients for kappa = 0 and kappa.
+template <int P>
+__global__ void __launch_bounds__(256) k_mom_final(const int32_t *__restrict__ mom_chunk_off,
+    const double *__restrict__ part, double eps, double *__restrict__ mom_w)
+{
+    constexpr int NK = MomShape<P>::NK;
+    constexpr int NJ = MomShape<P>::NJ;
+    __shared__ double M[4][NK];
+    const int mi = blockIdx.x, t = threadIdx.x;
+    const int c0 = mom_chunk_off[mi], c1 = mom_chunk_off[mi + 1];
+    for (int q = t; q < 4 * NK; q += 256) {
+        double s = 0.0;
+        for (int c = c0; c < c1; ++c) s += part[(size_t)c * 4 * NK + q];
+        M[q / NK][q % NK] = s;
+    }
+    __syncthreads();
+    double *W = mom_w + (size_t)mi * 8 * NJ;
+    for (int j = t; j < NJ; j += 256) {
+        int jv[3];
+        tdecode(j, jv[0], jv[1], jv[2]);
+        const int dj = jv[0] + jv[1] + jv[2];
+        double ha = 0.0, hb = 0.0, hc[3] = {0, 0, 0}, he[3] = {0, 0, 0};
+        if (dj <= P) ha = kOneOver4Pi * M[0][j];
+        if (dj >= 1 && dj - 1 <= P) {
+            for (int d = 0; d < 3; ++d) {
+                if (jv[d] < 1) continue;
+                int kv[3] = {jv[0], jv[1], jv[2]};
+                kv[d] -= 1;
+                const int kid = tabi::tidx(kv[0], kv[1], kv[2]);
+                const double D = (double)(kv[d] + 1) * kOneOver4Pi;
+                hb += D * M[1 + d][kid];
+                hc[d] = D * M[0][kid];
+            }
+        }
+        if (dj >= 2 && dj - 2 <= P) {
+            for (int e = 0; e < 3; ++e) {
+                double s = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    int kv[3] = {jv[0], jv[1], jv[2]};
+                    kv[d] -= 1;
+                    kv[e] -= 1;
+                    if (kv[0] < 0 || kv[1] < 0 || kv[2] < 0) continue;
+                    const int kid = tabi::tidx(kv[0], kv[1], kv[2]);
+                    const double D = (d == e) ? (double)((kv[d] + 1) * (kv[d] + 2)) * kOneOver4Pi
+                                              : (double)((kv[d] + 1) * (kv[e] + 1)) * kOneOver4Pi;
+                    s += D * (-M[1 + d][kid]);
+                }
+                he[e] = s;
+            }
+        }
+        W[0 * NJ + j] = ha - hb;
+        W[1 * NJ + j] = -ha + eps * hb;
+        for (int d = 0; d < 3; ++d) {
+            W[(2 + d) * NJ + j] = -hc[d] - he[d];
+            W[(5 + d) * NJ + j] = hc[d] / eps + he[d];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// far field: one wave per task (a run of far items of one target group)
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(64) k_far(const FarTask *__restrict__ tasks, int ntask, int n,
+    const int32_t *__restrict__ far_node, const u64 *__restrict__ far_mask,
+    const int32_t *__restrict__ node2mom, const double *__restrict__ mom_w,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double kappa, double2 *__restrict__ part)
+{
+    constexpr int L = MomShape<P>::L;
+    constexpr int NJ = MomShape<P>::NJ;
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const FarTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    double acc1 = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+    for (int it = tk.i0; it < tk.i1; ++it) {
+        const int nd = far_node[it];
+        const u64 m = far_mask[it];
+        if (!((m >> lane) & 1ull)) continue;
+        const double4 geo = ngeo[nd];
+        const double *W = mom_w + (size_t)node2mom[nd] * 8 * NJ;
+        const double dx = px - geo.x, dy = py - geo.y, dz = pz - geo.z;
+        {
+            double a[NJ];
+            tabi::taylor<L, false, double>(dx, dy, dz, 0.0, a);
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                acc1 = fma(a[j], W[j], acc1);
+                ax = fma(a[j], W[2 * NJ + j], ax);
+                ay = fma(a[j], W[3 * NJ + j], ay);
+                az = fma(a[j], W[4 * NJ + j], az);
+            }
+        }
+        {
+            double a[NJ];
+            tabi::taylor<L, true, double>(dx, dy, dz, kappa, a);
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                acc1 = fma(a[j], W[NJ + j], acc1);
+                ax = fma(a[j], W[5 * NJ + j], ax);
+                ay = fma(a[j], W[6 * NJ + j], ay);
+                az = fma(a[j], W[7 * NJ + j], az);
+            }
+        }
+    }
+    double p2 = 0.0;
+    if (valid) p2 = tnx[i] * ax + tny[i] * ay + tnz[i] * az;
+    part[(size_t)task * kWave + lane] = make_double2(acc1, p2);
+}
+
+// ---------------------------------------------------------------------------
+// near field: one wave per task (one leaf source range for one target group)
+// pair kernel of COMPP_DIRECT_PB, src/treecode3d_pb.f:965-1001; the self term
+// is excluded (src/treecode3d_pb.f:1169-1172)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks, int ntask, int n,
+    const double *__restrict__ src8, const double *__restrict__ x,
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, const double *__restrict__ tnx,
+    const double *__restrict__ tny, const double *__restrict__ tnz, double kappa, double eps,
+    double2 *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const NearTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool act = (tk.mask >> lane) & 1ull;
+    const bool valid = i < n;
+    const double sx = valid ? tx[i] : 0.0, sy = valid ? ty[i] : 0.0, sz = valid ? tz[i] : 0.0;
+    const double v0x = valid ? tnx[i] : 0.0, v0y = valid ? tny[i] : 0.0,
+                 v0z = valid ? tnz[i] : 0.0;
+    const double kappa2 = kappa * kappa, inv_eps = 1.0 / eps;
+    double p1 = 0.0, p2 = 0.0;
+    if (act) {
+        for (int j = tk.s0; j < tk.s1; ++j) {
+            const double *r = src8 + (size_t)j * 8;
+            const double phi = x[j], psi = x[n + j];
+            const bool use = (j != i);
+            const double dx = r[0] - sx, dy = r[1] - sy, dz = r[2] - sz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            r2 = use ? r2 : 1.0;
+            const double rs = sqrt(r2);
+            const double inv = 1.0 / rs;
+            const double g0 = kOneOver4Pi * inv;
+            const double krs = kappa * rs;
+            const double ek = exp(-krs);
+            const double gk = ek * g0;
+            const double ct = (r[3] * dx + r[4] * dy + r[5] * dz) * inv;
+            const double ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
+            const double tp1 = g0 * inv;
+            const double tp2 = (1.0 + krs) * ek;
+            const double g10 = ct0 * tp1, g20 = tp2 * g10;
+            const double g1 = ct * tp1, g2 = tp2 * g1;
+            const double g3 = (v0x * r[3] + v0y * r[4] + v0z * r[5] - 3.0 * ct0 * ct) * inv * tp1;
+            const double g4 = tp2 * g3 - kappa2 * ct0 * ct * gk;
+            const double l1 = g1 - eps * g2, l2 = g0 - gk, l3 = g4 - g3, l4 = g10 - g20 * inv_eps;
+            const double area = use ? r[6] : 0.0;
+            p1 += (l1 * phi + l2 * psi) * area;
+            p2 += (l3 * phi + l4 * psi) * area;
+        }
+    }
+    part[(size_t)task * kWave + lane] = make_double2(p1, p2);
+}
+
+// y = pre*x - sum of the group's task partials, near tasks then far tasks
+// (TREE_COMPP_PB's combination, src/treecode3d_pb.f:1154-1155,1177-1178)
+__global__ void k_reduce(int n, const int32_t *__restrict__ near_task_off,
+    const int32_t *__restrict__ far_task_off, const double2 *__restrict__ part_near,
+    const double2 *__restrict__ part_far, const double *__restrict__ x, double pre1,
+    double pre2, double *__restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = i / kWave, lane = i % kWave;
+    double p1 = 0.0, p2 = 0.0;
+    for (int t = near_task_off[g]; t < near_task_off[g + 1]; ++t) {
+        const double2 v = part_near[(size_t)t * kWave + lane];
+        p1 += v.x; p2 += v.y;
+    }
+    for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
+        const double2 v = part_far[(size_t)t * kWave + lane];
+        p1 += v.x; p2 += v.y;
+    }
+    y[i] = pre1 * x[i] - p1;
+    y[n + i] = pre2 * x[n + i] - p2;
+}
+
+// ---------------------------------------------------------------------------
+// leaf-block preconditioner
+// ---------------------------------------------------------------------------
+// Assemble every leaf block, column-major with leading dimension m = 2*nl, in
+// the reference's layout (src/treecode3d_pb.f:1439-1528): one workgroup per
+// source face j, threads over the targets i of j's leaf.  flags bit 0 selects
+// the matvec-consistent placement of the L2/L3 couplings.
+__global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__restrict__ leaf_of,
+    const int32_t *__restrict__ blk_ibeg, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, const double *__restrict__ tarea, double kappa,
+    double eps, int flags, double *__restrict__ lu)
+{
+    const int j = blockIdx.x;
+    const int l = leaf_of[j];
+    const int ib = blk_ibeg[l], m = blk_m[l], nl = m / 2;
+    const int jj = j - ib;
+    double *A = lu + blk_off[l];
+    const double pre1 = 0.5 * (1.0 + eps), pre2 = 0.5 * (1.0 + 1.0 / eps);
+    const double rx = tx[j], ry = ty[j], rz = tz[j];
+    const double vx = tnx[j], vy = tny[j], vz = tnz[j], aj = tarea[j];
+    const bool swap23 = flags & 1;
+    for (int ii = threadIdx.x; ii < nl; ii += blockDim.x) {
+        const int i = ib + ii;
+        double e1, e2, e3, e4;
+        if (i == j) {
+            A[(size_t)jj * m + ii] = pre1;
+            A[(size_t)jj * m + ii + nl] = 0.0;
+            A[(size_t)(jj + nl) * m + ii] = 0.0;
+            A[(size_t)(jj + nl) * m + ii + nl] = pre2;
+            continue;
+        }
+        const double dx = rx - tx[i], dy = ry - ty[i], dz = rz - tz[i];
+        const double v0x = tnx[i], v0y = tny[i], v0z = tnz[i];
+        const double rs = sqrt(dx * dx + dy * dy + dz * dz);
+        const double g0 = kOneOver4Pi / rs;
+        const double krs = kappa * rs;
+        const double ek = exp(-krs);
+        const double gk = ek * g0;
+        const double ct = (vx * dx + vy * dy + vz * dz) / rs;
+        const double ct0 = (v0x * dx + v0y * dy + v0z * dz) / rs;
+        const double tp1 = g0 / rs;
+        const double tp2 = (1.0 + krs) * ek;
+        const double g10 = ct0 * tp1, g20 = tp2 * g10;
+        const double g1 = ct * tp1, g2 = tp2 * g1;
+        const double g3 = (v0x * vx + v0y * vy + v0z * vz - 3.0 * ct0 * ct) / rs * tp1;
+        const double g4 = tp2 * g3 - kappa * kappa * ct0 * ct * gk;
+        e1 = -(g1 - eps * g2) * aj;
+        e2 = -(g0 - gk) * aj;
+        e3 = -(g4 - g3) * aj;
+        e4 = -(g10 - g20 / eps) * aj;
+        if (swap23) { const double t = e2; e2 = e3; e3 = t; }
+        A[(size_t)jj * m + ii] = e1;
+        A[(size_t)jj * m + ii + nl] = e2;
+        A[(size_t)(jj + nl) * m + ii] = e3;
+        A[(size_t)(jj + nl) * m + ii + nl] = e4;
+    }
+}
+
+// Panel factorisation of columns k0..k0+nb-1 of every block with m > k0:
+// partial pivoting with the first-maximum rule (IDAMAX), reciprocal scaling and
+// rank-1 updates restricted to the panel (the unblocked step of DGETRF2's
+// semantics, src/dgetrf_dep.f:757-914).
+__global__ void __launch_bounds__(256) k_lu_panel(int k0, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
+    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info)
+{
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int l = blockIdx.x, t = threadIdx.x;
+    const int m = blk_m[l];
+    if (m <= k0) return;
+    const int nbe = min(kLuNB, m - k0);
+    double *A = lu + blk_off[l];
+    int32_t *pv = piv + piv_off[l];
+    for (int c = 0; c < nbe; ++c) {
+        const int col = k0 + c;
+        double *Ac = A + (size_t)col * m;
+        double best = -1.0;
+        int bi = 0x7fffffff;
+        for (int r = col + t; r < m; r += 256) {
+            const double v = fabs(Ac[r]);
+            if (v > best) { best = v; bi = r; }
+        }
+        sv[t] = best; si[t] = bi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) {
+                const double v2 = sv[t + s];
+                const int i2 = si[t + s];
+                if (v2 > sv[t] || (v2 == sv[t] && i2 < si[t])) { sv[t] = v2; si[t] = i2; }
+            }
+            __syncthreads();
+        }
+        const int prow = si[0];
+        __syncthreads();
+        if (t == 0) pv[col] = prow;
+        if (prow != col && t < nbe) {
+            double *cc = A + (size_t)(k0 + t) * m;
+            const double tmp = cc[col];
+            cc[col] = cc[prow];
+            cc[prow] = tmp;
+        }
+        __syncthreads();
+        const double pivot = Ac[col];
+        if (pivot == 0.0) {
+            if (t == 0) atomicCAS(info + l, 0, col + 1);
+            __syncthreads();
+            continue;
+        }
+        const double rinv = 1.0 / pivot;
+        for (int r = col + 1 + t; r < m; r += 256) Ac[r] *= rinv;
+        __syncthreads();
+        const int rows = m - col - 1, cols = nbe - c - 1;
+        for (int e = t; e < rows * cols; e += 256) {
+            const int cc = e / rows, r = col + 1 + e % rows;
+            double *Aj = A + (size_t)(col + 1 + cc) * m;
+            Aj[r] -= Ac[r] * Aj[col];
+        }
+        __syncthreads();
+    }
+}
+
+// Apply the panel's row interchanges to the columns outside the panel and form
+// U12 = L11^-1 A12 for the columns to its right.  One thread per column.
+__global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
+    double *__restrict__ lu, const int32_t *__restrict__ piv)
+{
+    __shared__ double L11[kLuNB][kLuNB + 1];
+    const int l = blockIdx.y;
+    const int m = blk_m[l];
+    if (m <= k0) return;
+    const int nbe = min(kLuNB, m - k0);
+    double *A = lu + blk_off[l];
+    const int32_t *pv = piv + piv_off[l];
+    for (int e = threadIdx.x; e < nbe * nbe; e += blockDim.x) {
+        const int r = e % nbe, c = e / nbe;
+        L11[r][c] = A[(size_t)(k0 + c) * m + k0 + r];
+    }
+    __syncthreads();
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m || (j >= k0 && j < k0 + nbe)) return;
+    double *Aj = A + (size_t)j * m;
+    for (int c = 0; c < nbe; ++c) {
+        const int pr = pv[k0 + c];
+        if (pr != k0 + c) {
+            const double tmp = Aj[k0 + c];
+            Aj[k0 + c] = Aj[pr];
+            Aj[pr] = tmp;
+        }
+    }
+    if (j < k0) return;
+    // U12 column j: forward substitution with unit-lower L11, in place
+    for (int c = 1; c < nbe; ++c) {
+        double s = Aj[k0 + c];
+        for (int c2 = 0; c2 < c; ++c2) s -= L11[c][c2] * Aj[k0 + c2];
+        Aj[k0 + c] = s;
+    }
+}
+
+// Trailing update A22 -= L21 U12 on 64x64 tiles; 256 threads x (4x4) outputs.
+__global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, double *__restrict__ lu)
+{
+    __shared__ double sL[kLuNB][64 + 1];
+    __shared__ double sU[kLuNB][64 + 1];
+    const int l = blockIdx.z;
+    const int m = blk_m[l];
+    if (m <= k0 + kLuNB) return;
+    const int nbe = kLuNB;
+    const int r0 = k0 + nbe + blockIdx.y * 64, c0 = k0 + nbe + blockIdx.x * 64;
+    if (r0 >= m || c0 >= m) return;
+    double *A = lu + blk_off[l];
+    const int t = threadIdx.x;
+    for (int e = t; e < nbe * 64; e += 256) {
+        const int c = e / 64, r = e % 64;
+        sL[c][r] = (r0 + r < m) ? A[(size_t)(k0 + c) * m + r0 + r] : 0.0;
+        const int cc = e / nbe, rr = e % nbe;  // U12 row rr, column c0+cc
+        sU[rr][cc] = (c0 + cc < m) ? A[(size_t)(c0 + cc) * m + k0 + rr] : 0.0;
+    }
+    __syncthreads();
+    const int tr = (t % 16) * 4, tc = (t / 16) * 4;
+    double acc[4][4] = {};
+#pragma unroll 4
+    for (int c = 0; c < nbe; ++c) {
+        double lv[4], uv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { lv[q] = sL[c][tr + q]; uv[q] = sU[c][tc + q]; }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(lv[a], uv[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int col = c0 + tc + b;
+        double *Ac = A + (size_t)col * m;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int row = r0 + tr + a;
+            if (row < m && col < m) Ac[row] -= acc[a][b];
+        }
+    }
+}
+
+// z = M^-1 r for every leaf block: row interchanges, then blocked forward and
+// back substitution with the cached factors.  One workgroup per block.
+constexpr int kSolveB = 64;
+__global__ void __launch_bounds__(256) k_lu_solve(int n, const int32_t *__restrict__ blk_ibeg,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    const int32_t *__restrict__ piv_off, const double *__restrict__ lu,
+    const int32_t *__restrict__ piv, const double *__restrict__ r, double *__restrict__ z)
+{
+    extern __shared__ double zs[];
+    const int l = blockIdx.x, t = threadIdx.x;
+    const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
+    const double *A = lu + blk_off[l];
+    const int32_t *pv = piv + piv_off[l];
+    for (int q = t; q < nl; q += 256) {
+        zs[q] = r[ib + q];
+        zs[nl + q] = r[n + ib + q];
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int k = 0; k < m; ++k) {
+            const int p = pv[k];
+            if (p != k) { const double tmp = zs[k]; zs[k] = zs[p]; zs[p] = tmp; }
+        }
+    }
+    __syncthreads();
+    const int lane = t & 63, w = t >> 6;
+    // forward: unit lower triangular
+    for (int kb = 0; kb < m; kb += kSolveB) {
+        const int bs = min(kSolveB, m - kb);
+        if (w == 0) {
+            double zl = (lane < bs) ? zs[kb + lane] : 0.0;
+            for (int c = 0; c < bs; ++c) {
+                const double zc = __shfl(zl, c);
+                if (lane > c && lane < bs) zl -= A[(size_t)(kb + c) * m + kb + lane] * zc;
+            }
+            if (lane < bs) zs[kb + lane] = zl;
+        }
+        __syncthreads();
+        for (int row = kb + bs + t; row < m; row += 256) {
+            double s = zs[row];
+            for (int c = 0; c < bs; ++c) s -= A[(size_t)(kb + c) * m + row] * zs[kb + c];
+            zs[row] = s;
+        }
+        __syncthreads();
+    }
+    // backward: upper triangular
+    const int nblk = (m + kSolveB - 1) / kSolveB;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int kb = bi * kSolveB;
+        const int bs = min(kSolveB, m - kb);
+        if (w == 0) {
+            double zl = (lane < bs) ? zs[kb + lane] : 0.0;
+            for (int c = bs - 1; c >= 0; --c) {
+                if (lane == c) zl = zl / A[(size_t)(kb + c) * m + kb + c];
+                const double zc = __shfl(zl, c);
+                if (lane < c) zl -= A[(size_t)(kb + c) * m + kb + lane] * zc;
+            }
+            if (lane < bs) zs[kb + lane] = zl;
+        }
+        __syncthreads();
+        for (int row = t; row < kb; row += 256) {
+            double s = zs[row];
+            for (int c = 0; c < bs; ++c) s -= A[(size_t)(kb + c) * m + row] * zs[kb + c];
+            zs[row] = s;
+        }
+        __syncthreads();
+    }
+    for (int q = t; q < nl; q += 256) {
+        z[ib + q] = zs[q];
+        z[n + ib + q] = zs[nl + q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side and energy
+// ---------------------------------------------------------------------------
+// b_i = sum_k q_k/(4 pi eps_p r), b_{n+i} = sum_k q_k cos/(4 pi eps_p r^2)
+// (src/main.f90:597-607), one thread per face, charges staged through LDS
+__global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double *__restrict__ qpos,
+    const double *__restrict__ q, double eps_p, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double *__restrict__ b)
+{
+    __shared__ double sq[4][256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    const double vx = valid ? tnx[i] : 0.0, vy = valid ? tny[i] : 0.0, vz = valid ? tnz[i] : 0.0;
+    double b1 = 0.0, b2 = 0.0;
+    for (int k0 = 0; k0 < nq; k0 += 256) {
+        const int k = k0 + threadIdx.x;
+        __syncthreads();
+        if (k < nq) {
+            sq[0][threadIdx.x] = qpos[3 * k];
+            sq[1][threadIdx.x] = qpos[3 * k + 1];
+            sq[2][threadIdx.x] = qpos[3 * k + 2];
+            sq[3][threadIdx.x] = q[k];
+        }
+        __syncthreads();
+        const int cnt = min(256, nq - k0);
+        for (int c = 0; c < cnt; ++c) {
+            const double dx = sq[0][c] - px, dy = sq[1][c] - py, dz = sq[2][c] - pz;
+            const double rs = sqrt(dx * dx + dy * dy + dz * dz);
+            const double inv = 1.0 / rs;
+            const double ct = (vx * dx + vy * dy + vz * dz) * inv;
+            b1 += sq[3][c] * inv;
+            b2 += sq[3][c] * ct * inv * inv;
+        }
+    }
+    if (valid) {
+        const double s = 1.0 / (4.0 * kPi * eps_p);
+        b[i] = b1 * s;
+        b[n + i] = b2 * s;
+    }
+}
+
+// potential at each charge from the surface solution (potential_molecule,
+// src/main.f90:627-655): one workgroup per charge, deterministic tree reduction
+__global__ void __launch_bounds__(256) k_energy(int n, const double *__restrict__ qpos,
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, const double *__restrict__ tnx,
+    const double *__restrict__ tny, const double *__restrict__ tnz,
+    const double *__restrict__ tarea, const double *__restrict__ x, double eps, double kappa,
+    double *__restrict__ ptl)
+{
+    __shared__ double red[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    const double cx = qpos[3 * k], cy = qpos[3 * k + 1], cz = qpos[3 * k + 2];
+    double s = 0.0;
+    for (int j = t; j < n; j += 256) {
+        const double dx = tx[j] - cx, dy = ty[j] - cy, dz = tz[j] - cz;
+        const double rs = sqrt(dx * dx + dy * dy + dz * dz);
+        const double inv = 1.0 / rs;
+        const double g0 = kOneOver4Pi * inv;
+        const double krs = kappa * rs;
+        const double ek = exp(-krs);
+        const double gk = ek * g0;
+        const double ct = (tnx[j] * dx + tny[j] * dy + tnz[j] * dz) * inv;
+        const double tp1 = g0 * inv;
+        const double tp2 = (1.0 + krs) * ek;
+        const double g1 = ct * tp1, g2 = tp2 * g1;
+        const double h1 = g1 - eps * g2, h2 = g0 - gk;
+        s += tarea[j] * (h1 * x[j] + h2 * x[n + j]);
+    }
+    red[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) ptl[k] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1 for GMRES (deterministic two-pass reductions)
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 512;
+
+__global__ void __launch_bounds__(256) k_dot_part(int64_t n, const double *__restrict__ a,
+    const double *__restrict__ b, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        s += a[i] * b[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_part(int np, const double *__restrict__ part,
+    double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_scal(int64_t n, double a, double *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+
+// out = sum_k c_k v_k (k < nv), vectors at stride ld
+__global__ void k_lincomb(int64_t n, int nv, const double *__restrict__ coef,
+    const double *__restrict__ v, int64_t ld, double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nv; ++k) s += coef[k] * v[(int64_t)k * ld + i];
+    out[i] = s;
+}
+
+__global__ void k_sub(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+    double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+template <int P>
+void launch_moments(tabi_ctx *c)
+{
+    if (c->nmom == 0) return;
+    k_mom_raw<P><<<c->nchunk, 256, 0, c->s>>>(c->n, c->chunk_node.p, c->chunk_beg.p,
+        c->chunk_end.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
+        c->tarea.p, c->vx.p, c->mom_part.p);
+    k_mom_final<P><<<c->nmom, 256, 0, c->s>>>(c->mom_chunk_off.p, c->mom_part.p, c->prm.eps,
+                                              c->mom_w.p);
+}
+
+template <int P>
+void launch_far(tabi_ctx *c)
+{
+    if (c->n_far_task == 0) return;
+    k_far<P><<<c->n_far_task, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n,
+        c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_w.p, c->ngeo.p, c->tx.p, c->ty.p,
+        c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->part_far.p);
+}
+
+void dispatch_moments(tabi_ctx *c)
+{
+    switch (c->p) {
+    case 1: launch_moments<1>(c); break;
+    case 2: launch_moments<2>(c); break;
+    case 3: launch_moments<3>(c); break;
+    case 4: launch_moments<4>(c); break;
+    case 5: launch_moments<5>(c); break;
+    case 6: launch_moments<6>(c); break;
+    default: throw Error("unsupported Taylor order");
+    }
+}
+
+void dispatch_far(tabi_ctx *c)
+{
+    switch (c->p) {
+    case 1: launch_far<1>(c); break;
+    case 2: launch_far<2>(c); break;
+    case 3: launch_far<3>(c); break;
+    case 4: launch_far<4>(c); break;
+    case 5: launch_far<5>(c); break;
+    case 6: launch_far<6>(c); break;
+    default: throw Error("unsupported Taylor order");
+    }
+}
+
+// y = A x on device vectors c->vx -> c->vy
+void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
+{
+    // the moment and field kernels read x from c->vx
+    if (dx != c->vx.p)
+        HIPCHK(hipMemcpyAsync(c->vx.p, dx, sizeof(double) * 2 * c->n, hipMemcpyDeviceToDevice, c->s));
+    { KTimer kt(c, 0); dispatch_moments(c); }
+    if (c->n_near_task) {
+        KTimer kt(c, 1);
+        k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, c->src8.p,
+            c->vx.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
+            c->prm.eps, c->part_near.p);
+    }
+    if (c->n_far_task) { KTimer kt(c, 2); dispatch_far(c); }
+    const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
+    {
+        KTimer kt(c, 3);
+        k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->near_task_off.p, c->far_task_off.p,
+            c->part_near.p, c->part_far.p, c->vx.p, pre1, pre2, dy);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
+{
+    const size_t shm = sizeof(double) * (size_t)c->mmax;
+    KTimer kt(c, 4);
+    k_lu_solve<<<c->nleaf, 256, shm, c->s>>>(c->n, c->blk_ibeg.p, c->blk_m.p, c->blk_off.p,
+        c->piv_off.p, c->lu.p, c->piv.p, dr, dz);
+    HIPCHK(hipGetLastError());
+}
+
+double dev_dot(tabi_ctx *c, int64_t n, const double *a, const double *b)
+{
+    k_dot_part<<<kRedBlocks, 256, 0, c->s>>>(n, a, b, c->red.p);
+    k_sum_part<<<1, 256, 0, c->s>>>(kRedBlocks, c->red.p, c->red.p + kRedBlocks);
+    double h = 0.0;
+    HIPCHK(hipMemcpyAsync(&h, c->red.p + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return h;
+}
+
+void build_lists(tabi_ctx *c)
+{
+    const int G = c->ngroups;
+    const HostTree &T = c->tree;
+    const int nn = T.nnode();
+    c->cnt_far.alloc(G);
+    c->cnt_near.alloc(G);
+    c->need_mom.alloc(nn);
+    c->err_flag.alloc(1);
+    HIPCHK(hipMemsetAsync(c->need_mom.p, 0, sizeof(int32_t) * nn, c->s));
+    HIPCHK(hipMemsetAsync(c->err_flag.p, 0, sizeof(int32_t), c->s));
+    const unsigned nb = nblk(G, 4);
+    k_lists<<<nb, 256, 0, c->s>>>(0, c->n, G, c->tx.p, c->ty.p, c->tz.p, c->ngeo.p, c->nnum.p,
+        c->nchild.p, c->ncoff.p, c->nkids.p, c->prm.theta, c->cnt_far.p, c->cnt_near.p,
+        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->err_flag.p);
+    HIPCHK(hipGetLastError());
+    std::vector<int32_t> cf(G), cn(G);
+    int32_t ef = 0;
+    HIPCHK(hipMemcpyAsync(cf.data(), c->cnt_far.p, sizeof(int32_t) * G, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(cn.data(), c->cnt_near.p, sizeof(int32_t) * G, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(&ef, c->err_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (ef) throw Error("tree traversal stack overflow");
+    c->h_far_off.assign(G + 1, 0);
+    c->h_near_off.assign(G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+        c->h_far_off[g + 1] = c->h_far_off[g] + cf[g];
+        c->h_near_off[g + 1] = c->h_near_off[g] + cn[g];
+    }
+    const int64_t nf = c->h_far_off[G], nnr = c->h_near_off[G];
+    c->n_far_items = nf;
+    c->n_near_items = nnr;
+    c->far_off.upload(c->h_far_off.data(), G + 1, c->s);
+    c->near_off.upload(c->h_near_off.data(), G + 1, c->s);
+    c->far_node.alloc(std::max<int64_t>(nf, 1));
+    c->far_mask.alloc(std::max<int64_t>(nf, 1));
+    c->near_leaf.alloc(std::max<int64_t>(nnr, 1));
+    c->near_mask.alloc(std::max<int64_t>(nnr, 1));
+    k_lists<<<nb, 256, 0, c->s>>>(1, c->n, G, c->tx.p, c->ty.p, c->tz.p, c->ngeo.p, c->nnum.p,
+        c->nchild.p, c->ncoff.p, c->nkids.p, c->prm.theta, nullptr, nullptr, c->far_off.p,
+        c->near_off.p, c->far_node.p, c->far_mask.p, c->near_leaf.p, c->near_mask.p,
+        c->need_mom.p, c->err_flag.p);
+    HIPCHK(hipGetLastError());
+
+    // host: tasks, moment chunks, counters
+    std::vector<int32_t> near_leaf(nnr), need(nn), far_node(nf);
+    std::vector<u64> near_mask(nnr), far_mask(nf);
+    HIPCHK(hipMemcpyAsync(near_leaf.data(), c->near_leaf.p, sizeof(int32_t) * nnr, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(near_mask.data(), c->near_mask.p, sizeof(u64) * nnr, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(far_mask.data(), c->far_mask.p, sizeof(u64) * nf, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(need.data(), c->need_mom.p, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+
+    c->p_far = 0;
+    for (int64_t k = 0; k < nf; ++k) c->p_far += __builtin_popcountll(far_mask[k]);
+    c->p_near = 0;
+    int64_t near_src_total = 0;
+    for (int64_t k = 0; k < nnr; ++k) {
+        const int np = T.numpar[near_leaf[k]];
+        c->p_near += (int64_t)__builtin_popcountll(near_mask[k]) * np;
+        near_src_total += np;
+    }
+    c->p_near -= c->n;  // self pairs are excluded
+
+    // task granularity: enough waves to fill the device several times over
+    const int64_t target = 65536;
+    int S = 64;
+    while (S < 2048 && near_src_total / S > target) S *= 2;
+    int F = 1;
+    while (F < 64 && nf / F > target) F *= 2;
+    std::vector<NearTask> nt;
+    std::vector<FarTask> ft;
+    std::vector<int32_t> nto(G + 1, 0), fto(G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+        for (int32_t k = c->h_near_off[g]; k < c->h_near_off[g + 1]; ++k) {
+            const int nd = near_leaf[k];
+            const int b = T.ibeg[nd], e = T.iend[nd] + 1;
+            for (int s0 = b; s0 < e; s0 += S) nt.push_back({g, s0, std::min(e, s0 + S), 0, near_mask[k]});
+        }
+        nto[g + 1] = (int32_t)nt.size();
+        for (int32_t k = c->h_far_off[g]; k < c->h_far_off[g + 1]; k += F)
+            ft.push_back({g, k, std::min(c->h_far_off[g + 1], k + F), 0});
+        fto[g + 1] = (int32_t)ft.size();
+    }
+    c->n_near_task = (int32_t)nt.size();
+    c->n_far_task = (int32_t)ft.size();
+    c->near_tasks.alloc(std::max<size_t>(nt.size(), 1));
+    c->far_tasks.alloc(std::max<size_t>(ft.size(), 1));
+    if (!nt.empty()) c->near_tasks.upload(nt.data(), nt.size(), c->s);
+    if (!ft.empty()) c->far_tasks.upload(ft.data(), ft.size(), c->s);
+    c->near_task_off.upload(nto.data(), G + 1, c->s);
+    c->far_task_off.upload(fto.data(), G + 1, c->s);
+    c->part_near.alloc((size_t)std::max(c->n_near_task, 1) * kWave);
+    c->part_far.alloc((size_t)std::max(c->n_far_task, 1) * kWave);
+
+    // moment nodes and their particle chunks
+    std::vector<int32_t> n2m(nn, -1), cnode, cbeg, cend, mco(1, 0);
+    int32_t nm = 0;
+    c->p_mom = 0;
+    for (int i = 0; i < nn; ++i) {
+        if (!need[i]) continue;
+        n2m[i] = nm++;
+        c->p_mom += T.numpar[i];
+        for (int b = T.ibeg[i]; b <= T.iend[i]; b += kMomChunk) {
+            cnode.push_back(i);
+            cbeg.push_back(b);
+            cend.push_back(std::min(T.iend[i] + 1, b + kMomChunk));
+        }
+        mco.push_back((int32_t)cnode.size());
+    }
+    c->nmom = nm;
+    c->nchunk = (int32_t)cnode.size();
+    c->node2mom.upload(n2m.data(), nn, c->s);
+    if (nm) {
+        c->chunk_node.upload(cnode.data(), cnode.size(), c->s);
+        c->chunk_beg.upload(cbeg.data(), cbeg.size(), c->s);
+        c->chunk_end.upload(cend.data(), cend.size(), c->s);
+        c->mom_chunk_off.upload(mco.data(), mco.size(), c->s);
+        c->mom_part.alloc((size_t)c->nchunk * 4 * c->nk);
+        c->mom_w.alloc((size_t)nm * 8 * c->nj);
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+}
+
+void build_precond(tabi_ctx *c)
+{
+    const HostTree &T = c->tree;
+    const int nl = (int)T.leaves.size();
+    c->nleaf = nl;
+    std::vector<int32_t> bib(nl), bm(nl), poff(nl), leaf_of(c->n);
+    c->h_blk_off.assign(nl, 0);
+    int64_t off = 0;
+    int32_t pof = 0;
+    c->mmax = 0;
+    for (int l = 0; l < nl; ++l) {
+        const int nd = T.leaves[l];
+        bib[l] = T.ibeg[nd];
+        bm[l] = 2 * T.numpar[nd];
+        c->h_blk_off[l] = off;
+        off += (int64_t)bm[l] * bm[l];
+        poff[l] = pof;
+        pof += bm[l];
+        c->mmax = std::max(c->mmax, bm[l]);
+        for (int i = T.ibeg[nd]; i <= T.iend[nd]; ++i) leaf_of[i] = l;
+    }
+    c->h_blk_m = bm;
+    c->blk_ibeg.upload(bib.data(), nl, c->s);
+    c->blk_m.upload(bm.data(), nl, c->s);
+    c->blk_off.upload(c->h_blk_off.data(), nl, c->s);
+    c->piv_off.upload(poff.data(), nl, c->s);
+    c->leaf_of.upload(leaf_of.data(), c->n, c->s);
+    c->piv.alloc(pof);
+    c->lu.alloc(off);
+    c->lu_info.alloc(nl);
+    HIPCHK(hipMemsetAsync(c->lu_info.p, 0, sizeof(int32_t) * nl, c->s));
+    k_lu_assemble<<<c->n, 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
+        c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
+        c->prm.kappa, c->prm.eps, c->prm.flags, c->lu.p);
+    HIPCHK(hipGetLastError());
+    const int tiles = (c->mmax + 63) / 64;
+    for (int k0 = 0; k0 < c->mmax; k0 += kLuNB) {
+        k_lu_panel<<<nl, 256, 0, c->s>>>(k0, c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p,
+                                         c->piv.p, c->lu_info.p);
+        k_lu_swap_trsm<<<dim3(nblk(c->mmax, 256), nl), 256, 0, c->s>>>(k0, c->blk_m.p,
+            c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
+        const int rem = c->mmax - k0 - kLuNB;
+        if (rem > 0) {
+            const int tt = (rem + 63) / 64;
+            k_lu_gemm<<<dim3(tt, tt, nl), 256, 0, c->s>>>(k0, c->blk_m.p, c->blk_off.p, c->lu.p);
+        }
+        (void)tiles;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<int32_t> info(nl);
+    HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int l = 0; l < nl; ++l)
+        if (info[l]) throw Error("singular leaf block " + std::to_string(l + 1) + " (column " +
+                                 std::to_string(info[l]) + ")");
+}
+
+void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz,
+               const double *nrm, const double *area, int32_t *orderarr_out, double *t_phase)
+{
+    if (!prm) throw Error("null parameters");
+    if (n < 1) throw Error("need at least one face");
+    if (prm->order < 1 || prm->order > kMaxOrder) throw Error("Taylor order must be 1..6");
+    if (prm->maxparnode < 1) throw Error("maxparnode must be >= 1");
+    if (prm->precision != 0) throw Error("only precision 0 (FP64) is available in this build");
+    c->prm = *prm;
+    c->n = n;
+    c->p = prm->order;
+    c->nk = (c->p + 1) * (c->p + 2) * (c->p + 3) / 6;
+    c->nj = (c->p + 3) * (c->p + 4) * (c->p + 5) / 6;
+    c->ngroups = (n + kWave - 1) / kWave;
+    if (!c->s) HIPCHK(hipStreamCreate(&c->s));
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> x(n), y(n), z(n);
+    for (int i = 0; i < n; ++i) { x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2]; }
+    tabi::build_tree_host(n, x.data(), y.data(), z.data(), prm->maxparnode, c->tree);
+    const HostTree &T = c->tree;
+    if (orderarr_out)
+        for (int i = 0; i < n; ++i) orderarr_out[i] = T.order[i] + 1;
+    std::vector<double> nx(n), ny(n), nz(n), ar(n), s8((size_t)n * 8);
+    for (int i = 0; i < n; ++i) {
+        const int o = T.order[i];
+        nx[i] = nrm[3 * o]; ny[i] = nrm[3 * o + 1]; nz[i] = nrm[3 * o + 2]; ar[i] = area[o];
+        double *r = &s8[(size_t)i * 8];
+        r[0] = x[i]; r[1] = y[i]; r[2] = z[i]; r[3] = nx[i]; r[4] = ny[i]; r[5] = nz[i];
+        r[6] = ar[i]; r[7] = 0.0;
+    }
+    const int nn = T.nnode();
+    std::vector<double4> geo(nn);
+    for (int k = 0; k < nn; ++k) geo[k] = make_double4(T.xmid[k], T.ymid[k], T.zmid[k], T.radius[k]);
+    c->tx.upload(x.data(), n, c->s); c->ty.upload(y.data(), n, c->s); c->tz.upload(z.data(), n, c->s);
+    c->tnx.upload(nx.data(), n, c->s); c->tny.upload(ny.data(), n, c->s); c->tnz.upload(nz.data(), n, c->s);
+    c->tarea.upload(ar.data(), n, c->s);
+    c->src8.upload(s8.data(), (size_t)n * 8, c->s);
+    c->ngeo.upload(geo.data(), nn, c->s);
+    c->nnum.upload(T.numpar.data(), nn, c->s);
+    c->nchild.upload(T.nchild.data(), nn, c->s);
+    c->ncoff.upload(T.child_off.data(), nn, c->s);
+    c->nibeg.upload(T.ibeg.data(), nn, c->s);
+    c->nkids.upload(T.children.empty() ? T.nchild.data() : T.children.data(),
+                    std::max<size_t>(T.children.size(), 1), c->s);
+    c->vx.alloc((size_t)2 * n);
+    c->vy.alloc((size_t)2 * n);
+    c->red.alloc(kRedBlocks + 8);
+    HIPCHK(hipStreamSynchronize(c->s));
+    auto t1 = std::chrono::steady_clock::now();
+    build_lists(c);
+    auto t2 = std::chrono::steady_clock::now();
+    build_precond(c);
+    auto t3 = std::chrono::steady_clock::now();
+    if (t_phase) {
+        t_phase[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t_phase[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        t_phase[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    }
+}
+
+void ctx_stats(tabi_ctx *c, int64_t *o)
+{
+    o[0] = c->n;
+    o[1] = c->tree.nnode();
+    o[2] = c->nleaf;
+    o[3] = c->p_near;
+    o[4] = c->p_far;
+    o[5] = c->p_mom;
+    o[6] = c->n_far_items;
+    o[7] = c->n_near_items;
+    o[8] = (int64_t)c->n_near_task + c->n_far_task;
+    o[9] = c->lu.n * (int64_t)sizeof(double);
+}
+
+// GMRES with left preconditioning, ITOL=2, JSCAL=0, KMP=MAXL (DGMRES/DPIGMR,
+// src/dgmres_dep.f:689-852,1652-1863): modified Gram-Schmidt with the
+// conditional re-orthogonalisation of DORTH (1376-1411), Givens QR of the
+// Hessenberg matrix (DHEQR, 1093-1125), residual vector for the stopping test
+// and restarts from the Krylov basis (DRLCAL, 1948-1977), least squares update
+// (DHELS, 925-947).  Vectors stay on the device; only scalars come back.
+void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl, int nrmax,
+               int *iter_out, double *err_out, int *ierr_out)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    if (maxl > N) maxl = (int)N;
+    const int ldh = maxl + 1;
+    DBuf<double> V, r, z, wk, dl, coef;
+    V.alloc((size_t)(maxl + 1) * N);
+    r.alloc(N); z.alloc(N); wk.alloc(N); dl.alloc(N);
+    coef.alloc(maxl + 2);
+    auto vcol = [&](int k) { return V.p + (size_t)k * N; };
+    const unsigned nb = nblk(N, 256);
+    std::vector<double> hes((size_t)ldh * maxl), q(2 * maxl), yls(maxl + 1), cf(maxl + 2);
+    auto nrm2 = [&](const double *v) { return std::sqrt(dev_dot(c, N, v, v)); };
+    // is the vector numerically zero (sum |x| < 1e-10, src/main.f90:478)?
+    auto matvec_guarded = [&](const double *in, double *out) {
+        matvec_dev(c, in, out);
+    };
+    // initial residual; x0 = 0 short-circuits the matvec
+    msolve_dev(c, db, r.p);
+    const double bnrm = nrm2(r.p);
+    // sum|x0| test on the host copy is done by the caller: x0 == 0 -> r = b
+    std::vector<double> hx(N);
+    HIPCHK(hipMemcpyAsync(hx.data(), dxs, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    double sabs = 0.0;
+    for (int64_t i = 0; i < N; ++i) sabs += std::fabs(hx[i]);
+    if (sabs < 1.0e-10) {
+        HIPCHK(hipMemcpyAsync(r.p, dxs, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
+    } else {
+        matvec_guarded(dxs, r.p);
+    }
+    k_sub<<<nb, 256, 0, c->s>>>(N, db, r.p, r.p);
+    int iter = 0, ierr = 0, nrsts = 0;
+    double err = 0.0;
+    for (;;) {
+        if (nrsts > nrmax) { ierr = 1; break; }
+        if (nrsts > 0) HIPCHK(hipMemcpyAsync(r.p, dl.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemsetAsync(z.p, 0, sizeof(double) * N, c->s));
+        int iflag = 0, lgmr = 0;
+        if (nrsts == 0) {
+            HIPCHK(hipMemcpyAsync(wk.p, r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
+            msolve_dev(c, wk.p, r.p);
+        }
+        HIPCHK(hipMemcpyAsync(vcol(0), r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
+        const double r0nrm = nrm2(vcol(0));
+        err = r0nrm / bnrm;
+        bool done_early = err <= tol;
+        if (!done_early) {
+            k_scal<<<nb, 256, 0, c->s>>>(N, 1.0 / r0nrm, vcol(0));
+            std::fill(hes.begin(), hes.end(), 0.0);
+            double prod = 1.0, rho = 0.0, snormw = 0.0;
+            bool converged = false, stalled = false;
+            for (int ll = 1; ll <= maxl; ++ll) {
+                lgmr = ll;
+                matvec_guarded(vcol(ll - 1), wk.p);
+                msolve_dev(c, wk.p, vcol(ll));
+                double *w = vcol(ll);
+                double *hcol = &hes[(size_t)(ll - 1) * ldh];
+                const double vnrm = nrm2(w);
+                for (int i = 0; i < ll; ++i) {
+                    hcol[i] = dev_dot(c, N, vcol(i), w);
+                    k_axpy<<<nb, 256, 0, c->s>>>(N, -hcol[i], vcol(i), w);
+                }
+                snormw = nrm2(w);
+                if (!(vnrm + 0.001 * snormw != vnrm)) {
+                    double sumdsq = 0.0;
+                    for (int i = 0; i < ll; ++i) {
+                        const double tem = -dev_dot(c, N, vcol(i), w);
+                        if (hcol[i] + 0.001 * tem == hcol[i]) continue;
+                        hcol[i] = hcol[i] - tem;
+                        k_axpy<<<nb, 256, 0, c->s>>>(N, tem, vcol(i), w);
+                        sumdsq += tem * tem;
+                    }
+                    if (sumdsq != 0.0) snormw = std::sqrt(std::max(0.0, snormw * snormw - sumdsq));
+                }
+                hcol[ll] = snormw;
+                // DHEQR update of the new column
+                for (int k = 0; k < ll - 1; ++k) {
+                    const double cc = q[2 * k], ss = q[2 * k + 1];
+                    const double t1 = hcol[k], t2 = hcol[k + 1];
+                    hcol[k] = cc * t1 - ss * t2;
+                    hcol[k + 1] = ss * t1 + cc * t2;
+                }
+                {
+                    const double t1 = hcol[ll - 1], t2 = hcol[ll];
+                    double cc, ss;
+                    if (t2 == 0.0) { cc = 1.0; ss = 0.0; }
+                    else if (std::fabs(t2) >= std::fabs(t1)) {
+                        const double t = t1 / t2;
+                        ss = -1.0 / std::sqrt(1.0 + t * t);
+                        cc = -ss * t;
+                    } else {
+                        const double t = t2 / t1;
+                        cc = 1.0 / std::sqrt(1.0 + t * t);
+                        ss = -cc * t;
+                    }
+                    q[2 * (ll - 1)] = cc;
+                    q[2 * (ll - 1) + 1] = ss;
+                    hcol[ll - 1] = cc * t1 - ss * t2;
+                    if (hcol[ll - 1] == 0.0) { stalled = true; break; }
+                }
+                prod = prod * q[2 * ll - 1];
+                rho = std::fabs(prod * r0nrm);
+                // DRLCAL: rl = r0nrm*prod*(S..S v1 + ... + C_ll/snormw v_{ll+1})
+                {
+                    for (int k = 0; k <= ll; ++k) cf[k] = 0.0;
+                    cf[0] = 1.0;
+                    for (int i = 1; i <= ll - 1; ++i) {
+                        const double ss = q[2 * i - 1], cc = q[2 * i - 2];
+                        for (int k = 0; k < i; ++k) cf[k] *= ss;
+                        cf[i] = cc;
+                    }
+                    const double ss = q[2 * ll - 1], cc = q[2 * ll - 2] / snormw;
+                    for (int k = 0; k < ll; ++k) cf[k] *= ss;
+                    cf[ll] = cc;
+                    const double tem = r0nrm * prod;
+                    for (int k = 0; k <= ll; ++k) cf[k] *= tem;
+                    HIPCHK(hipMemcpyAsync(coef.p, cf.data(), sizeof(double) * (ll + 1), hipMemcpyHostToDevice, c->s));
+                    k_lincomb<<<nb, 256, 0, c->s>>>(N, ll + 1, coef.p, V.p, N, dl.p);
+                }
+                err = nrm2(dl.p) / bnrm;
+                if (err <= tol) { converged = true; break; }
+                if (ll == maxl) break;
+                k_scal<<<nb, 256, 0, c->s>>>(N, 1.0 / snormw, w);
+            }
+            if (stalled) {
+                iflag = 2;
+            } else {
+                if (!converged) iflag = (rho < r0nrm) ? 1 : 2;
+                if (iflag != 2) {
+                    // dl already holds the residual for lgmr == maxl (restart vector)
+                    const int nn = lgmr;
+                    std::fill(yls.begin(), yls.end(), 0.0);
+                    yls[0] = r0nrm;
+                    for (int k = 0; k < nn; ++k) {
+                        const double cc = q[2 * k], ss = q[2 * k + 1];
+                        const double t1 = yls[k], t2 = yls[k + 1];
+                        yls[k] = cc * t1 - ss * t2;
+                        yls[k + 1] = ss * t1 + cc * t2;
+                    }
+                    for (int kb = 0; kb < nn; ++kb) {
+                        const int k = nn - 1 - kb;
+                        yls[k] = yls[k] / hes[(size_t)k * ldh + k];
+                        const double t = -yls[k];
+                        for (int i = 0; i < k; ++i) yls[i] += t * hes[(size_t)k * ldh + i];
+                    }
+                    HIPCHK(hipMemcpyAsync(coef.p, yls.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->s));
+                    k_lincomb<<<nb, 256, 0, c->s>>>(N, nn, coef.p, V.p, N, z.p);
+                }
+            }
+        }
+        iter += lgmr;
+        if (iflag != 2) k_axpy<<<nb, 256, 0, c->s>>>(N, 1.0, z.p, dxs);
+        if (iflag == 0) { ierr = 0; break; }
+        if (iflag == 1) { ++nrsts; continue; }
+        ierr = 2;
+        break;
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    *iter_out = iter;
+    *err_out = err;
+    *ierr_out = ierr;
+}
+
+void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p, double *db)
+{
+    DBuf<double> dq, dqp;
+    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
+    dq.upload(q, std::max(nq, 1), c->s);
+    k_rhs<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, nq, dqp.p, dq.p, eps_p, c->tx.p, c->ty.p,
+        c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, db);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+}
+
+void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
+                const double *dxs, double *sol, double *cou)
+{
+    DBuf<double> dqp, dptl;
+    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
+    dptl.alloc(std::max(nq, 1));
+    if (nq > 0)
+        k_energy<<<nq, 256, 0, c->s>>>(c->n, dqp.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p,
+            c->tnz.p, c->tarea.p, dxs, c->prm.eps, c->prm.kappa, dptl.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> ptl(std::max(nq, 1));
+    HIPCHK(hipMemcpyAsync(ptl.data(), dptl.p, sizeof(double) * nq, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    const double para = 332.0716;  // src/main.f90:64
+    double s = 0.0, cc = 0.0;
+    for (int k = 0; k < nq; ++k) {
+        s += q[k] * ptl[k];
+        for (int j = k + 1; j < nq; ++j) {
+            const double dx = qpos[3 * k] - qpos[3 * j], dy = qpos[3 * k + 1] - qpos[3 * j + 1],
+                         dz = qpos[3 * k + 2] - qpos[3 * j + 2];
+            cc += 1.0 / eps_p / std::sqrt(dx * dx + dy * dy + dz * dz) * q[k] * q[j];
+        }
+    }
+    *sol = s * 0.5 * para * 4.0 * kPi;
+    *cou = cc * para;
+}
+
+std::unique_ptr<tabi_ctx> g_default;
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        g_err.clear();
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+tabi_ctx *need_default()
+{
+    if (!g_default) throw Error("tabi_init has not been called");
+    return g_default.get();
+}
+
+void check_n2(tabi_ctx *c, int32_t n2)
+{
+    if (n2 != 2 * c->n) throw Error("vector length " + std::to_string(n2) + " does not match 2*nface = " +
+                                    std::to_string(2 * c->n));
+}
+
+void ctx_matvec_host(tabi_ctx *c, const double *x, double *y)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    double sabs = 0.0;
+    for (int64_t i = 0; i < N; ++i) sabs += std::fabs(x[i]);
+    if (sabs < 1.0e-10) {  // src/main.f90:478-479
+        std::memcpy(y, x, sizeof(double) * N);
+        return;
+    }
+    HIPCHK(hipMemcpyAsync(c->vx.p, x, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
+    matvec_dev(c, c->vx.p, c->vy.p);
+    HIPCHK(hipMemcpyAsync(y, c->vy.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+}
+
+void ctx_msolve_host(tabi_ctx *c, const double *r, double *z)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    DBuf<double> dr;
+    dr.upload(r, N, c->s);
+    msolve_dev(c, dr.p, c->vy.p);
+    HIPCHK(hipMemcpyAsync(z, c->vy.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+}
+
+void ctx_gmres_host(tabi_ctx *c, const double *b, double *x, double tol, int maxl, int nrmax,
+                    int *iter, double *err, int *ierr)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    DBuf<double> db, dxs;
+    db.upload(b, N, c->s);
+    dxs.upload(x, N, c->s);
+    gmres_dev(c, db.p, dxs.p, tol, maxl, nrmax, iter, err, ierr);
+    HIPCHK(hipMemcpyAsync(x, dxs.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tabi_init(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+              const double *area, int32_t *orderarr_out, int32_t *nleaf_out)
+{
+    return guarded([&] {
+        auto c = std::make_unique<tabi_ctx>();
+        ctx_setup(c.get(), prm, n, xyz, nrm, area, orderarr_out, nullptr);
+        if (nleaf_out) *nleaf_out = c->nleaf;
+        if (g_default && g_default->s) (void)hipStreamDestroy(g_default->s);
+        g_default = std::move(c);
+    });
+}
+
+int tabi_matvec(int32_t n2, const double *x, double *y)
+{
+    return guarded([&] {
+        tabi_ctx *c = need_default();
+        check_n2(c, n2);
+        ctx_matvec_host(c, x, y);
+    });
+}
+
+int tabi_msolve(int32_t n2, const double *r, double *z)
+{
+    return guarded([&] {
+        tabi_ctx *c = need_default();
+        check_n2(c, n2);
+        ctx_msolve_host(c, r, z);
+    });
+}
+
+int tabi_rhs(int32_t nq, const double *qpos, const double *q, double eps_p, double *b)
+{
+    return tabi_ctx_rhs(g_default.get(), nq, qpos, q, eps_p, b);
+}
+
+int tabi_energy(int32_t nq, const double *qpos, const double *q, double eps_p, const double *x,
+                double *soleng, double *couleng)
+{
+    return tabi_ctx_energy(g_default.get(), nq, qpos, q, eps_p, x, soleng, couleng);
+}
+
+int tabi_gmres(const double *b, double *x, double tol, int32_t maxl, int32_t nrmax,
+               int32_t *iter, double *err, int32_t *ierr)
+{
+    return tabi_ctx_gmres(g_default.get(), b, x, tol, maxl, nrmax, iter, err, ierr);
+}
+
+void tabi_finalize(void)
+{
+    if (g_default) {
+        hipStream_t s = g_default->s;
+        g_default.reset();
+        if (s) (void)hipStreamDestroy(s);
+    }
+}
+
+const char *tabi_last_error(void) { return g_err.c_str(); }
+
+int tabi_stats(int64_t *out10)
+{
+    return guarded([&] { ctx_stats(need_default(), out10); });
+}
+
+int tabi_ctx_create(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+                    const double *area, int32_t *orderarr_out, tabi_ctx **out)
+{
+    return guarded([&] {
+        auto c = std::make_unique<tabi_ctx>();
+        ctx_setup(c.get(), prm, n, xyz, nrm, area, orderarr_out, nullptr);
+        *out = c.release();
+    });
+}
+
+void tabi_ctx_destroy(tabi_ctx *c)
+{
+    if (!c) return;
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t s = c->s;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+int tabi_ctx_stats(tabi_ctx *c, int64_t *out10)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        ctx_stats(c, out10);
+    });
+}
+
+int tabi_ctx_matvec_dev(tabi_ctx *c, const double *d_x, double *d_y)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        matvec_dev(c, d_x, d_y);
+    });
+}
+
+int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        msolve_dev(c, d_r, d_z);
+    });
+}
+
+int tabi_ctx_matvec(tabi_ctx *c, const double *x, double *y)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        ctx_matvec_host(c, x, y);
+    });
+}
+
+int tabi_ctx_msolve(tabi_ctx *c, const double *r, double *z)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        ctx_msolve_host(c, r, z);
+    });
+}
+
+int tabi_ctx_rhs(tabi_ctx *c, int32_t nq, const double *qpos, const double *q, double eps_p,
+                 double *b)
+{
+    return guarded([&] {
+        if (!c) throw Error("tabi_init has not been called");
+        DBuf<double> db;
+        db.alloc((size_t)2 * c->n);
+        rhs_dev(c, nq, qpos, q, eps_p, db.p);
+        HIPCHK(hipMemcpyAsync(b, db.p, sizeof(double) * 2 * c->n, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    });
+}
+
+int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q, double eps_p,
+                    const double *x, double *soleng, double *couleng)
+{
+    return guarded([&] {
+        if (!c) throw Error("tabi_init has not been called");
+        DBuf<double> dxs;
+        dxs.upload(x, (size_t)2 * c->n, c->s);
+        energy_dev(c, nq, qpos, q, eps_p, dxs.p, soleng, couleng);
+    });
+}
+
+int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
+                   int32_t nrmax, int32_t *iter, double *err, int32_t *ierr)
+{
+    return guarded([&] {
+        if (!c) throw Error("tabi_init has not been called");
+        int it = 0, ie = 0;
+        ctx_gmres_host(c, b, x, tol, maxl, nrmax, &it, err, &ie);
+        *iter = it;
+        *ierr = ie;
+    });
+}
+
+int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+               const double *area, int32_t nq, const double *qpos, const double *q,
+               double eps_p, double tol, double *x_out, int32_t *orderarr_out, int32_t *iter,
+               double *err, int32_t *ierr, double *soleng, double *couleng)
+{
+    return guarded([&] {
+        auto t0 = std::chrono::steady_clock::now();
+        tabi_ctx c;
+        double ph[3] = {0, 0, 0};
+        ctx_setup(&c, prm, n, xyz, nrm, area, orderarr_out, ph);
+        auto t1 = std::chrono::steady_clock::now();
+        const int64_t N = 2 * (int64_t)n;
+        DBuf<double> db, dxs;
+        db.alloc(N);
+        dxs.alloc(N);
+        rhs_dev(&c, nq, qpos, q, eps_p, db.p);
+        HIPCHK(hipMemsetAsync(dxs.p, 0, sizeof(double) * N, c.s));
+        auto t2 = std::chrono::steady_clock::now();
+        int it = 0, ie = 0;
+        gmres_dev(&c, db.p, dxs.p, tol, 10, 10, &it, err, &ie);
+        auto t3 = std::chrono::steady_clock::now();
+        energy_dev(&c, nq, qpos, q, eps_p, dxs.p, soleng, couleng);
+        auto t4 = std::chrono::steady_clock::now();
+        if (x_out) {
+            HIPCHK(hipMemcpyAsync(x_out, dxs.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.s));
+            HIPCHK(hipStreamSynchronize(c.s));
+        }
+        *iter = it;
+        *ierr = ie;
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        g_times[0] = ph[0]; g_times[1] = ph[1]; g_times[2] = ph[2];
+        g_times[3] = ms(t1, t2); g_times[4] = ms(t2, t3); g_times[5] = ms(t3, t4);
+        g_times[6] = ms(t0, t4);
+        hipStream_t s = c.s;
+        c.s = nullptr;
+        (void)hipStreamDestroy(s);
+    });
+}
+
+int tabi_ctx_profile(tabi_ctx *c, int32_t enable)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (enable && !c->ev[0]) {
+            HIPCHK(hipEventCreate(&c->ev[0]));
+            HIPCHK(hipEventCreate(&c->ev[1]));
+        }
+        c->prof = enable != 0;
+        for (int i = 0; i < 8; ++i) { c->kt_ms[i] = 0.0; c->kt_n[i] = 0; }
+    });
+}
+
+int tabi_ctx_kernel_times(tabi_ctx *c, double *out_ms, int64_t *out_launches)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        for (int i = 0; i < 8; ++i) { out_ms[i] = c->kt_ms[i]; out_launches[i] = c->kt_n[i]; }
+    });
+}
+
+int tabi_last_solve_times(double *out7)
+{
+    for (int i = 0; i < 7; ++i) out7[i] = g_times[i];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+"""ctypes binding of libtabi_hip.so (the C ABI declared in include/tabi_hip.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+entry point raises; nothing falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libtabi_hip.so"))
+_LIB = None
+
+# every symbol include/tabi_hip.h declares
+EXPORTS = (
+    "tabi_init", "tabi_matvec", "tabi_msolve", "tabi_rhs", "tabi_energy", "tabi_gmres",
+    "tabi_finalize", "tabi_last_error", "tabi_stats", "tabi_ctx_create", "tabi_ctx_destroy",
+    "tabi_ctx_stats", "tabi_ctx_matvec_dev", "tabi_ctx_msolve_dev", "tabi_ctx_matvec",
+    "tabi_ctx_msolve", "tabi_ctx_rhs", "tabi_ctx_energy", "tabi_ctx_gmres", "tabi_solve",
+    "tabi_last_solve_times", "tabi_ctx_profile", "tabi_ctx_kernel_times",
+)
+
+KERNEL_SLOTS = ("moments", "near", "far", "reduce", "msolve", "lists", "lu_factor", "rhs_energy")
+
+STAT_NAMES = ("nface", "nnode", "nleaf", "p_near", "p_far", "p_mom", "n_far_items",
+              "n_near_items", "n_tasks", "lu_bytes")
+
+
+class TabiError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("eps", C.c_double), ("kappa", C.c_double), ("theta", C.c_double),
+                ("order", C.c_int32), ("maxparnode", C.c_int32), ("precision", C.c_int32),
+                ("flags", C.c_int32)]
+
+
+def load():
+    """Load the library (once).  Raises TabiError if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise TabiError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                        "(make -C tabi-pb_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    lib.tabi_last_error.restype = C.c_char_p
+    lib.tabi_finalize.restype = None
+    lib.tabi_ctx_destroy.restype = None
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _d(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _check(rc):
+    if rc != 0:
+        raise TabiError(load().tabi_last_error().decode())
+
+
+def make_params(eps, kappa, theta, order, maxparnode, precision=0, flags=0):
+    return Params(float(eps), float(kappa), float(theta), int(order), int(maxparnode),
+                  int(precision), int(flags))
+
+
+class Surface:
+    """One surface on the GPU: tree, interaction lists and factored leaf blocks.
+
+    Mirrors the reference's set-up + callback pair (treecode_initialization,
+    MATVECmain, MSOLVEdric; src/main.f90:329-529).  Vectors are tree-ordered
+    [phi; dphi/dn] of length 2n.
+    """
+
+    def __init__(self, prm: Params, xyz, nrm, area):
+        lib = load()
+        xyz = _d(xyz, (-1, 3))
+        nrm = _d(nrm, (-1, 3))
+        area = _d(area)
+        n = xyz.shape[0]
+        if nrm.shape[0] != n or area.shape[0] != n:
+            raise TabiError("xyz, nrm and area disagree in length")
+        self.n = n
+        self.prm = prm
+        self.order = np.zeros(n, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib.tabi_ctx_create(C.byref(prm), C.c_int32(n), _p(xyz), _p(nrm), _p(area),
+                                   _p(self.order), C.byref(h)))
+        self._h = h
+        self.orderarr = self.order.copy()          # 1-based, as the reference's orderarr
+        self.perm = self.order.astype(np.int64) - 1  # tree position -> original index
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tabi_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        out = np.zeros(10, dtype=np.int64)
+        _check(load().tabi_ctx_stats(self._h, _p(out)))
+        return dict(zip(STAT_NAMES, (int(v) for v in out)))
+
+    def to_tree(self, v):
+        """Permute an original-order length-n or length-2n vector into tree order."""
+        v = np.asarray(v)
+        if v.shape[0] == self.n:
+            return v[self.perm]
+        return np.concatenate([v[: self.n][self.perm], v[self.n:][self.perm]])
+
+    def from_tree(self, v):
+        v = np.asarray(v)
+        out = np.empty_like(v)
+        if v.shape[0] == self.n:
+            out[self.perm] = v
+            return out
+        out[: self.n][self.perm] = v[: self.n]
+        out[self.n:][self.perm] = v[self.n:]
+        return out
+
+    def profile(self, enable=True):
+        _check(load().tabi_ctx_profile(self._h, C.c_int32(1 if enable else 0)))
+
+    def kernel_times(self):
+        ms = np.zeros(8)
+        cnt = np.zeros(8, dtype=np.int64)
+        _check(load().tabi_ctx_kernel_times(self._h, _p(ms), _p(cnt)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_SLOTS)}
+
+    def matvec(self, x):
+        x = _d(x)
+        y = np.empty_like(x)
+        _check(load().tabi_ctx_matvec(self._h, _p(x), _p(y)))
+        return y
+
+    def msolve(self, r):
+        r = _d(r)
+        z = np.empty_like(r)
+        _check(load().tabi_ctx_msolve(self._h, _p(r), _p(z)))
+        return z
+
+    def matvec_dev(self, d_x: int, d_y: int):
+        """Device-pointer matvec (pointers as integers, e.g. torch.Tensor.data_ptr())."""
+        _check(load().tabi_ctx_matvec_dev(self._h, C.c_void_p(d_x), C.c_void_p(d_y)))
+
+    def msolve_dev(self, d_r: int, d_z: int):
+        _check(load().tabi_ctx_msolve_dev(self._h, C.c_void_p(d_r), C.c_void_p(d_z)))
+
+    def rhs(self, qpos, q, eps_p):
+        qpos = _d(qpos, (-1, 3))
+        q = _d(q)
+        b = np.zeros(2 * self.n)
+        _check(load().tabi_ctx_rhs(self._h, C.c_int32(q.shape[0]), _p(qpos), _p(q),
+                                   C.c_double(eps_p), _p(b)))
+        return b
+
+    def energy(self, qpos, q, eps_p, x):
+        qpos = _d(qpos, (-1, 3))
+        q = _d(q)
+        x = _d(x)
+        s = C.c_double()
+        cc = C.c_double()
+        _check(load().tabi_ctx_energy(self._h, C.c_int32(q.shape[0]), _p(qpos), _p(q),
+                                      C.c_double(eps_p), _p(x), C.byref(s), C.byref(cc)))
+        return s.value, cc.value
+
+    def gmres(self, b, x0=None, tol=1e-4, maxl=10, nrmax=10):
+        b = _d(b)
+        x = np.zeros_like(b) if x0 is None else _d(x0).copy()
+        it = C.c_int32()
+        err = C.c_double()
+        ierr = C.c_int32()
+        _check(load().tabi_ctx_gmres(self._h, _p(b), _p(x), C.c_double(tol), C.c_int32(maxl),
+                                     C.c_int32(nrmax), C.byref(it), C.byref(err), C.byref(ierr)))
+        return x, it.value, err.value, ierr.value
+
+
+def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4):
+    """One full PB solve on the GPU (tabi_solve).  Returns a dict with the
+    tree-order solution, permutation, GMRES counters, energies and phase times."""
+    lib = load()
+    xyz = _d(xyz, (-1, 3))
+    nrm = _d(nrm, (-1, 3))
+    area = _d(area)
+    qpos = _d(qpos, (-1, 3))
+    q = _d(q)
+    n = xyz.shape[0]
+    x = np.zeros(2 * n)
+    order = np.zeros(n, dtype=np.int32)
+    it = C.c_int32()
+    err = C.c_double()
+    ierr = C.c_int32()
+    sol = C.c_double()
+    cou = C.c_double()
+    _check(lib.tabi_solve(C.byref(prm), C.c_int32(n), _p(xyz), _p(nrm), _p(area),
+                          C.c_int32(q.shape[0]), _p(qpos), _p(q), C.c_double(eps_p),
+                          C.c_double(tol), _p(x), _p(order), C.byref(it), C.byref(err),
+                          C.byref(ierr), C.byref(sol), C.byref(cou)))
+    t = np.zeros(7)
+    lib.tabi_last_solve_times(_p(t))
+    return dict(x=x, orderarr=order, iter=it.value, err=err.value, ierr=ierr.value,
+                soleng=sol.value, couleng=cou.value,
+                times_ms=dict(zip(("tree", "lists", "precond", "rhs", "gmres", "energy",
+                                   "total"), t)))

@@ -34,7 +34,13 @@ def icosphere(subdiv: int, radius: float = 1.0):
          (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
          (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
          (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
-    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    def unit(p):
+        p = np.asarray(p, dtype=np.float64)
+        # element-wise IEEE arithmetic only (no BLAS), so every machine builds
+        # the same bits
+        return p / np.sqrt((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2])
+
+    verts = [unit(p) for p in v]
     faces = list(f)
     for _ in range(subdiv):
         cache = {}
@@ -43,8 +49,7 @@ def icosphere(subdiv: int, radius: float = 1.0):
         def mid(a, b):
             key = (a, b) if a < b else (b, a)
             if key not in cache:
-                m = verts[a] + verts[b]
-                verts.append(m / np.linalg.norm(m))
+                verts.append(unit(verts[a] + verts[b]))
                 cache[key] = len(verts) - 1
             return cache[key]
 

@@ -1,0 +1,210 @@
+"""Pins the CPU oracle against the reference program's own outputs (golden
+fixtures written by tests/golden/gen_golden.py from the flang-built reference)
+and against what the mathematics fixes: Taylor coefficients against
+arbitrary-precision derivatives, the treecode at theta = 0 against the O(n^2)
+direct sum, leaf solves against a dense solver, the partition against its
+specification, and the Born sphere against the analytic energy."""
+import itertools
+
+import mpmath
+import numpy as np
+import pytest
+
+import cases
+from conftest import load_golden
+
+FAST_CASES = ["sph3_l100_p3", "sph4_off_p6", "sph3_k0_p2", "sph3_direct"]
+
+
+def oracle_problem(O, name):
+    v, nv, f, atoms, usr = cases.build(name)
+    f2, _ = O.cull_faces(v, f)
+    xyz, nrm, area = O.face_geometry(v, nv, f2)
+    kappa = O.kappa_of(usr.bulk_strength, usr.epsw)
+    prm = O.params(usr.epsw / usr.epsp, kappa, usr.mac, usr.order, usr.maxparnode)
+    return dict(v=v, nv=nv, f=f2, xyz=xyz, nrm=nrm, area=area, atoms=atoms, usr=usr,
+                kappa=kappa, prm=prm)
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_fixture_inputs_unchanged(name):
+    g = load_golden(name)
+    v, nv, f, atoms, _ = cases.build(name)
+    assert str(g["input_sha256"]) == cases.input_hash(v, nv, f, atoms)
+
+
+@pytest.mark.parametrize("name", FAST_CASES + ["sph5_born"])
+def test_oracle_tree_and_rhs_match_reference(oracle_lib, name):
+    O = oracle_lib
+    g = load_golden(name)
+    P = oracle_problem(O, name)
+    assert P["kappa"] == float(g["kappa"])
+    assert P["xyz"].shape[0] == int(g["n"])
+    txyz, order, nodes = O.build_tree(P["xyz"], P["usr"].maxparnode)
+    np.testing.assert_array_equal(order + 1, g["orderarr"])
+    ib, ie = O.leaves(nodes)
+    np.testing.assert_array_equal(ib + 1, g["leaf_beg"])
+    np.testing.assert_array_equal(ie + 1, g["leaf_end"])
+    b = O.rhs(P["xyz"], P["nrm"], P["atoms"][:, :3].astype(np.float32).astype(np.float64),
+              P["atoms"][:, 3].astype(np.float32).astype(np.float64), P["usr"].epsp)
+    n = P["xyz"].shape[0]
+    bt = np.concatenate([b[order], b[n + order]])
+    s = int(g["stride"])
+    np.testing.assert_array_equal(bt[::s], g["bvct"])
+
+
+@pytest.mark.parametrize("name", FAST_CASES)
+def test_oracle_matvec_msolve_gmres_match_reference(oracle_lib, name):
+    O = oracle_lib
+    g = load_golden(name)
+    P = oracle_problem(O, name)
+    txyz, order, nodes = O.build_tree(P["xyz"], P["usr"].maxparnode)
+    tn, ta = P["nrm"][order], P["area"][order]
+    x = g["x"]
+    y, _ = O.treecode_matvec(P["prm"], txyz, tn, ta, nodes, x)
+    np.testing.assert_allclose(y, g["y"], rtol=0, atol=1e-14 * np.abs(g["y"]).max())
+    ib, ie = O.leaves(nodes)
+    z, bad = O.precond(P["prm"], txyz, tn, ta, ib, ie, x)
+    assert bad == 0
+    np.testing.assert_allclose(z, g["z"], rtol=0, atol=1e-13 * np.abs(g["z"]).max())
+    xs, it, err, ierr, _ = O.gmres(P["prm"], txyz, tn, ta, nodes, g["bvct"])
+    assert (it, ierr) == (int(g["iter"]), int(g["ierr"]))
+    assert err == pytest.approx(float(g["err"]), rel=1e-10)
+    qp = P["atoms"][:, :3].astype(np.float32).astype(np.float64)
+    q = P["atoms"][:, 3].astype(np.float32).astype(np.float64)
+    sol, cou = O.energy(txyz, tn, ta, xs, qp, q, P["usr"].epsw / P["usr"].epsp, P["kappa"],
+                        P["usr"].epsp)
+    assert sol == pytest.approx(float(g["soleng"]), rel=1e-12)
+    assert cou == pytest.approx(float(g["couleng"]), rel=1e-12, abs=1e-12)
+
+
+def test_oracle_born_matvec_matches_reference(oracle_lib):
+    O = oracle_lib
+    g = load_golden("sph5_born")
+    P = oracle_problem(O, "sph5_born")
+    txyz, order, nodes = O.build_tree(P["xyz"], P["usr"].maxparnode)
+    x = cases.harness_x(2 * P["xyz"].shape[0])
+    np.testing.assert_allclose(x[::8], g["x"], rtol=1e-15, atol=1e-15)
+    y, cnt = O.treecode_matvec(P["prm"], txyz, P["nrm"][order], P["area"][order], nodes, x)
+    np.testing.assert_allclose(y[::8], g["y"], rtol=0, atol=1e-12 * np.abs(g["y"]).max())
+    assert np.linalg.norm(y) == pytest.approx(float(g["y_norm"]), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# mathematics
+# ---------------------------------------------------------------------------
+def _mp_coeffs(d, kappa, lim):
+    """a_k = (1/k!) d^k/dy^k exp(-kappa|x-y|)/|x-y| at y = 0, x = d (mpmath)."""
+    mpmath.mp.dps = 40
+    x0 = [mpmath.mpf(v) for v in d]
+    out = {}
+    for i, j, k in itertools.product(range(lim + 1), repeat=3):
+        if i + j + k > lim:
+            continue
+        f = lambda a, b, c: mpmath.exp(-kappa * mpmath.sqrt((x0[0] - a) ** 2 + (x0[1] - b) ** 2
+                                       + (x0[2] - c) ** 2)) / mpmath.sqrt(
+            (x0[0] - a) ** 2 + (x0[1] - b) ** 2 + (x0[2] - c) ** 2)
+        dv = mpmath.diff(f, (0, 0, 0), (i, j, k))
+        out[(i, j, k)] = float(dv / (mpmath.factorial(i) * mpmath.factorial(j) * mpmath.factorial(k)))
+    return out
+
+
+@pytest.mark.parametrize("kappa", [0.0, 0.12572533685666004, 0.9])
+@pytest.mark.parametrize("d", [(1.3, -0.4, 0.7), (-2.0, 0.5, 3.1)])
+def test_taylor_coefficients_against_mpmath(oracle_lib, d, kappa):
+    lim = 5
+    got = oracle_lib.taylor_coeffs(d, kappa, lim)
+    ref = _mp_coeffs(d, kappa, lim)
+    scale = max(abs(v) for v in ref.values())
+    for (i, j, k), v in ref.items():
+        assert got[i, j, k] == pytest.approx(v, rel=1e-10, abs=1e-13 * scale), (i, j, k)
+
+
+def test_treecode_at_theta0_is_direct_sum(oracle_lib):
+    O = oracle_lib
+    P = oracle_problem(O, "sph3_direct")
+    txyz, order, nodes = O.build_tree(P["xyz"], P["usr"].maxparnode)
+    tn, ta = P["nrm"][order], P["area"][order]
+    x = np.random.default_rng(3).standard_normal(2 * txyz.shape[0])
+    y, cnt = O.treecode_matvec(P["prm"], txyz, tn, ta, nodes, x)
+    yd = O.direct_matvec(P["prm"], txyz, tn, ta, x)
+    n = txyz.shape[0]
+    assert cnt[1] == 0 and cnt[0] == n * (n - 1)
+    np.testing.assert_allclose(y, yd, rtol=0, atol=1e-13 * np.abs(yd).max())
+
+
+def test_treecode_far_field_approximates_direct_sum(oracle_lib):
+    O = oracle_lib
+    P = oracle_problem(O, "sph4_off_p6")
+    txyz, order, nodes = O.build_tree(P["xyz"], P["usr"].maxparnode)
+    tn, ta = P["nrm"][order], P["area"][order]
+    x = np.random.default_rng(4).standard_normal(2 * txyz.shape[0])
+    y, cnt = O.treecode_matvec(P["prm"], txyz, tn, ta, nodes, x)
+    yd = O.direct_matvec(P["prm"], txyz, tn, ta, x)
+    assert cnt[1] > 0
+    # p = 6, theta = 0.5: far-field truncation error well below 1e-4
+    assert np.linalg.norm(y - yd) / np.linalg.norm(yd) < 1e-4
+
+
+def test_leaf_solve_inverts_the_reference_block(oracle_lib):
+    O = oracle_lib
+    P = oracle_problem(O, "sph3_l100_p3")
+    txyz, order, nodes = O.build_tree(P["xyz"], P["usr"].maxparnode)
+    tn, ta = P["nrm"][order], P["area"][order]
+    ib, ie = O.leaves(nodes)
+    n = txyz.shape[0]
+    r = np.random.default_rng(5).standard_normal(2 * n)
+    z, bad = O.precond(P["prm"], txyz, tn, ta, ib, ie, r)
+    assert bad == 0
+    for l in range(len(ib)):
+        blk = O.leaf_block(P["prm"], txyz, tn, ta, int(ib[l]), int(ie[l]))
+        m = ie[l] - ib[l] + 1
+        rl = np.concatenate([r[ib[l]:ie[l] + 1], r[n + ib[l]:n + ie[l] + 1]])
+        zl = np.linalg.solve(blk, rl)
+        got = np.concatenate([z[ib[l]:ie[l] + 1], z[n + ib[l]:n + ie[l] + 1]])
+        np.testing.assert_allclose(got, zl, rtol=1e-11, atol=1e-13)
+        # reference layout: phi-row / psi-column coupling is -L3*A (tc.f:1481-1488)
+        assert blk[0, 0] == pytest.approx(0.5 * (1 + P["usr"].epsw / P["usr"].epsp))
+        assert blk[m, m] == pytest.approx(0.5 * (1 + P["usr"].epsp / P["usr"].epsw))
+
+
+def test_tree_permutation_is_a_bijection_with_contiguous_leaves(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(11)
+    xyz = rng.standard_normal((3000, 3)) * [3.0, 1.0, 0.4]
+    txyz, order, nodes = O.build_tree(xyz, 37)
+    assert sorted(order.tolist()) == list(range(3000))
+    np.testing.assert_array_equal(txyz, xyz[order])
+    ib, ie = O.leaves(nodes)
+    # depth-first leaf order is not memory order (children are created x-low,
+    # x-high, then y-split halves), but the leaf ranges tile 0..n-1
+    srt = np.argsort(ib)
+    ib, ie = ib[srt], ie[srt]
+    assert ib[0] == 0 and ie[-1] == 2999
+    assert np.all(ib[1:] == ie[:-1] + 1)
+    assert np.all(ie - ib + 1 <= 37)
+    # every node box holds its particles; children partition the parent range
+    for nd in nodes:
+        sl = txyz[nd["ibeg"]:nd["iend"] + 1]
+        assert np.all(sl[:, 0] >= nd["xmin"]) and np.all(sl[:, 0] <= nd["xmax"])
+        assert np.all(sl[:, 2] >= nd["zmin"]) and np.all(sl[:, 2] <= nd["zmax"])
+        if nd["nchild"]:
+            ch = nodes[nd["child"][:nd["nchild"]]]
+            ch = ch[np.argsort(ch["ibeg"])]
+            assert ch["ibeg"][0] == nd["ibeg"] and ch["iend"][-1] == nd["iend"]
+            assert np.all(ch["ibeg"][1:] == ch["iend"][:-1] + 1)
+
+
+def test_born_sphere_energy_converges_to_analytic(oracle_lib):
+    """Kirkwood/Born: dG = 1/2 * 332.0716 q^2/a * (1/(eps_w (1+kappa a)) - 1/eps_p)."""
+    kappa = oracle_lib.kappa_of(0.15, 80.0)
+    exact = 0.5 * 332.0716 * (1.0 / (80.0 * (1 + kappa * 2.0)) - 1.0) / 2.0
+    assert exact == pytest.approx(-82.188683, abs=1e-5)
+    e3 = float(load_golden("sph3_l100_p3")["soleng"])
+    e5 = float(load_golden("sph5_born")["soleng"])
+    assert abs(e5 - exact) < abs(e3 - exact)
+    assert abs(e5 - exact) / abs(exact) < 0.01
+
+
+def test_kappa_uses_single_precision_literal(oracle_lib):
+    assert oracle_lib.kappa_of(0.15, 80.0) == 0.12572533685666004
