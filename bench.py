@@ -55,6 +55,21 @@ def workload(name):
         desc = dict(workload=f"Born sphere a=2A q=+1, icosphere {len(f)} faces", faces=len(f),
                     atoms=1)
         return prob, desc
+    if name.startswith("mol"):
+        from tabipb import surface
+        natom = int(name[3:].split("_")[0])
+        h = 0.6 if natom >= 4000 else 0.5
+        xyz, q, r = surface.synth_molecule(natom, seed=1)
+        v, nv, f = surface.ses_mesh(xyz, r, probe=1.4, h=h)
+        f, _ = driver.cull_faces(v, f)
+        fxyz, nrm, area = driver.face_geometry(v, nv, f)
+        # charges and positions as the driver reads them from a PQR (real*4)
+        qpos = xyz.astype(np.float32).astype(np.float64)
+        qq = q.astype(np.float32).astype(np.float64)
+        prob = driver.Problem(usr, fxyz, nrm, area, qpos, qq)
+        desc = dict(workload=f"synthetic {natom}-atom molecule (seed 1), SES probe 1.4 A "
+                             f"grid {h} A, {len(f)} faces", faces=len(f), atoms=natom)
+        return prob, desc
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -111,7 +126,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sphere328k")
+    ap.add_argument("--workload", default="mol8000")
     ap.add_argument("--matvecs", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
