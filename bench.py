@@ -160,17 +160,18 @@ def main():
     dy = torch.empty_like(dx)
     torch.cuda.synchronize()
     S.matvec_dev(dx.data_ptr(), dy.data_ptr())
+    S.sync()
     S.profile(True)
     t0 = time.perf_counter()
     for _ in range(a.matvecs):
         S.matvec_dev(dx.data_ptr(), dy.data_ptr())
     kt = S.kernel_times()
     S.profile(False)
-    torch.cuda.synchronize()
+    S.sync()
     t1 = time.perf_counter()
     for _ in range(a.matvecs):
         S.matvec_dev(dx.data_ptr(), dy.data_ptr())
-    torch.cuda.synchronize()
+    S.sync()
     mv_ms = (time.perf_counter() - t1) / a.matvecs * 1e3
     flops = {"near": W_NEAR * st["p_near"], "far": w_far(p) * st["p_far"],
              "moments": w_mom(p) * st["p_mom"]}

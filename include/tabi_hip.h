@@ -94,9 +94,12 @@ int tabi_ctx_create(const tabi_params *prm, int32_t n, const double *xyz,
 void tabi_ctx_destroy(tabi_ctx *c);
 int tabi_ctx_stats(tabi_ctx *c, int64_t *out10);
 
-/* device-pointer matvec / preconditioner on the context's stream */
+/* device-pointer matvec / preconditioner, enqueued on the context's own stream
+ * and returning before completion; the caller must have finished writing the
+ * input on its streams, and must call tabi_ctx_sync before reading the output */
 int tabi_ctx_matvec_dev(tabi_ctx *c, const double *d_x, double *d_y);
 int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z);
+int tabi_ctx_sync(tabi_ctx *c);
 /* host-pointer forms */
 int tabi_ctx_matvec(tabi_ctx *c, const double *x, double *y);
 int tabi_ctx_msolve(tabi_ctx *c, const double *r, double *z);

@@ -1710,6 +1710,14 @@ int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z)
     });
 }
 
+int tabi_ctx_sync(tabi_ctx *c)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        HIPCHK(hipStreamSynchronize(c->s));
+    });
+}
+
 int tabi_ctx_matvec(tabi_ctx *c, const double *x, double *y)
 {
     return guarded([&] {

@@ -20,7 +20,7 @@ EXPORTS = (
     "tabi_finalize", "tabi_last_error", "tabi_stats", "tabi_ctx_create", "tabi_ctx_destroy",
     "tabi_ctx_stats", "tabi_ctx_matvec_dev", "tabi_ctx_msolve_dev", "tabi_ctx_matvec",
     "tabi_ctx_msolve", "tabi_ctx_rhs", "tabi_ctx_energy", "tabi_ctx_gmres", "tabi_solve",
-    "tabi_last_solve_times", "tabi_ctx_profile", "tabi_ctx_kernel_times",
+    "tabi_last_solve_times", "tabi_ctx_profile", "tabi_ctx_kernel_times", "tabi_ctx_sync",
 )
 
 KERNEL_SLOTS = ("moments", "near", "far", "reduce", "msolve", "lists", "lu_factor", "rhs_energy")
@@ -162,6 +162,10 @@ class Surface:
 
     def msolve_dev(self, d_r: int, d_z: int):
         _check(load().tabi_ctx_msolve_dev(self._h, C.c_void_p(d_r), C.c_void_p(d_z)))
+
+    def sync(self):
+        """Wait for work enqueued by the device-pointer entry points."""
+        _check(load().tabi_ctx_sync(self._h))
 
     def rhs(self, qpos, q, eps_p):
         qpos = _d(qpos, (-1, 3))

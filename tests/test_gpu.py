@@ -147,7 +147,7 @@ def test_device_pointer_entry_points_agree_with_host_ones():
     torch.cuda.synchronize()
     S.matvec_dev(dx.data_ptr(), dy.data_ptr())
     S.msolve_dev(dy.data_ptr(), dx.data_ptr())
-    torch.cuda.synchronize()
+    S.sync()
     y = S.matvec(x)
     z = S.msolve(y)
     np.testing.assert_array_equal(dx.cpu().numpy(), z)
