@@ -110,6 +110,13 @@ int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q
 int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
                    int32_t nrmax, int32_t *iter, double *err, int32_t *ierr);
 
+/* Device-memory plumbing for callers that hold no other GPU runtime (so the
+ * device-pointer entry points can be used from C, Fortran or ctypes). */
+int tabi_device_alloc(int64_t bytes, void **out);
+int tabi_device_free(void *p);
+int tabi_copy_to_device(void *dst, const void *src, int64_t bytes);
+int tabi_copy_to_host(void *dst, const void *src, int64_t bytes);
+
 /* Per-kernel device timing on the context's stream.  With profiling enabled the
  * library brackets each launch of the timed kernels with HIP events; times
  * accumulate until reset.  Slots: 0 moments, 1 near field, 2 far field,

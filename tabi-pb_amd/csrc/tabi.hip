@@ -1710,6 +1710,29 @@ int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z)
     });
 }
 
+int tabi_device_alloc(int64_t bytes, void **out)
+{
+    return guarded([&] {
+        if (bytes <= 0) throw Error("allocation size must be positive");
+        HIPCHK(hipMalloc(out, (size_t)bytes));
+    });
+}
+
+int tabi_device_free(void *p)
+{
+    return guarded([&] { HIPCHK(hipFree(p)); });
+}
+
+int tabi_copy_to_device(void *dst, const void *src, int64_t bytes)
+{
+    return guarded([&] { HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice)); });
+}
+
+int tabi_copy_to_host(void *dst, const void *src, int64_t bytes)
+{
+    return guarded([&] { HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost)); });
+}
+
 int tabi_ctx_sync(tabi_ctx *c)
 {
     return guarded([&] {

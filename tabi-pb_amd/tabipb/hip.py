@@ -21,6 +21,7 @@ EXPORTS = (
     "tabi_ctx_stats", "tabi_ctx_matvec_dev", "tabi_ctx_msolve_dev", "tabi_ctx_matvec",
     "tabi_ctx_msolve", "tabi_ctx_rhs", "tabi_ctx_energy", "tabi_ctx_gmres", "tabi_solve",
     "tabi_last_solve_times", "tabi_ctx_profile", "tabi_ctx_kernel_times", "tabi_ctx_sync",
+    "tabi_device_alloc", "tabi_device_free", "tabi_copy_to_device", "tabi_copy_to_host",
 )
 
 KERNEL_SLOTS = ("moments", "near", "far", "reduce", "msolve", "lists", "lu_factor", "rhs_energy")
@@ -69,6 +70,45 @@ def _d(a, shape=None):
 def _check(rc):
     if rc != 0:
         raise TabiError(load().tabi_last_error().decode())
+
+
+class DeviceArray:
+    """A float64 device buffer owned through the library (no other GPU runtime
+    is involved).  ``ptr`` is the device address for the *_dev entry points."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = C.c_void_p()
+        _check(load().tabi_device_alloc(C.c_int64(8 * self.n), C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = _d(a).ravel()
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = _d(a).ravel()
+        assert a.size == self.n
+        _check(load().tabi_copy_to_device(C.c_void_p(self.ptr), _p(a), C.c_int64(8 * self.n)))
+
+    def get(self):
+        out = np.empty(self.n)
+        _check(load().tabi_copy_to_host(_p(out), C.c_void_p(self.ptr), C.c_int64(8 * self.n)))
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().tabi_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def make_params(eps, kappa, theta, order, maxparnode, precision=0, flags=0):
@@ -157,7 +197,7 @@ class Surface:
         return z
 
     def matvec_dev(self, d_x: int, d_y: int):
-        """Device-pointer matvec (pointers as integers, e.g. torch.Tensor.data_ptr())."""
+        """Device-pointer matvec (addresses as integers, e.g. DeviceArray.ptr)."""
         _check(load().tabi_ctx_matvec_dev(self._h, C.c_void_p(d_x), C.c_void_p(d_y)))
 
     def msolve_dev(self, d_r: int, d_z: int):

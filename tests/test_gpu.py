@@ -139,18 +139,18 @@ def test_zero_vector_shortcut():
 
 
 def test_device_pointer_entry_points_agree_with_host_ones():
-    import torch
     P, S = surface("sph4_off_p6")
     x = np.random.default_rng(2).standard_normal(2 * S.n)
-    dx = torch.from_numpy(x).cuda()
-    dy = torch.empty_like(dx)
-    torch.cuda.synchronize()
-    S.matvec_dev(dx.data_ptr(), dy.data_ptr())
-    S.msolve_dev(dy.data_ptr(), dx.data_ptr())
+    dx = hip.DeviceArray.from_host(x)
+    dy = hip.DeviceArray(2 * S.n)
+    S.matvec_dev(dx.ptr, dy.ptr)
+    S.msolve_dev(dy.ptr, dx.ptr)
     S.sync()
     y = S.matvec(x)
     z = S.msolve(y)
-    np.testing.assert_array_equal(dx.cpu().numpy(), z)
+    np.testing.assert_array_equal(dx.get(), z)
+    dx.free()
+    dy.free()
 
 
 def test_errors_are_reported():
