@@ -184,14 +184,20 @@ class Surface:
         _check(load().tabi_ctx_kernel_times(self._h, _p(ms), _p(cnt)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_SLOTS)}
 
+    def _len(self, v):
+        if v.shape != (2 * self.n,):
+            raise TabiError(f"vector length {v.size} does not match 2*nface = {2 * self.n}")
+
     def matvec(self, x):
         x = _d(x)
+        self._len(x)
         y = np.empty_like(x)
         _check(load().tabi_ctx_matvec(self._h, _p(x), _p(y)))
         return y
 
     def msolve(self, r):
         r = _d(r)
+        self._len(r)
         z = np.empty_like(r)
         _check(load().tabi_ctx_msolve(self._h, _p(r), _p(z)))
         return z
@@ -234,6 +240,38 @@ class Surface:
         _check(load().tabi_ctx_gmres(self._h, _p(b), _p(x), C.c_double(tol), C.c_int32(maxl),
                                      C.c_int32(nrmax), C.byref(it), C.byref(err), C.byref(ierr)))
         return x, it.value, err.value, ierr.value
+
+
+# ---- default-context drop-in API (what a Fortran or C driver binds) --------
+def dropin_init(prm: Params, xyz, nrm, area):
+    """tabi_init: returns (orderarr 1-based, nleaf)."""
+    xyz, nrm, area = _d(xyz, (-1, 3)), _d(nrm, (-1, 3)), _d(area)
+    n = xyz.shape[0]
+    order = np.zeros(n, dtype=np.int32)
+    nl = C.c_int32()
+    _check(load().tabi_init(C.byref(prm), C.c_int32(n), _p(xyz), _p(nrm), _p(area),
+                            _p(order), C.byref(nl)))
+    return order, nl.value
+
+
+def dropin_matvec(x):
+    """tabi_matvec(n2 = len(x), x, y): the MATVEC callback."""
+    x = _d(x)
+    y = np.empty_like(x)
+    _check(load().tabi_matvec(C.c_int32(x.size), _p(x), _p(y)))
+    return y
+
+
+def dropin_msolve(r):
+    """tabi_msolve(n2 = len(r), r, z): the MSOLVE callback."""
+    r = _d(r)
+    z = np.empty_like(r)
+    _check(load().tabi_msolve(C.c_int32(r.size), _p(r), _p(z)))
+    return z
+
+
+def dropin_finalize():
+    load().tabi_finalize()
 
 
 def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4):

@@ -153,12 +153,27 @@ def test_device_pointer_entry_points_agree_with_host_ones():
     dy.free()
 
 
-def test_errors_are_reported():
+def test_dropin_api_and_errors():
+    """The default-context API the Fortran shims bind (tabi_init/tabi_matvec/
+    tabi_msolve): same results as the handle API, and a status plus message
+    for bad calls instead of a crash."""
     P, S = surface("sph3_l100_p3")
+    order, nleaf = hip.dropin_init(P.params(), P.xyz, P.nrm, P.area)
+    np.testing.assert_array_equal(order, S.orderarr)
+    x = np.random.default_rng(9).standard_normal(2 * S.n)
+    np.testing.assert_array_equal(hip.dropin_matvec(x), S.matvec(x))
+    np.testing.assert_array_equal(hip.dropin_msolve(x), S.msolve(x))
+    with pytest.raises(hip.TabiError, match="does not match"):
+        hip.dropin_matvec(np.zeros(2 * S.n + 2))
+    with pytest.raises(hip.TabiError, match="does not match"):
+        hip.dropin_msolve(np.zeros(2 * S.n - 2))
+    with pytest.raises(hip.TabiError, match="order"):
+        hip.Surface(P.params(order=9), P.xyz, P.nrm, P.area)
     with pytest.raises(hip.TabiError):
         S.matvec(np.zeros(2 * S.n + 2))
-    with pytest.raises(hip.TabiError):
-        hip.Surface(P.params(order=9), P.xyz, P.nrm, P.area)
+    hip.dropin_finalize()
+    with pytest.raises(hip.TabiError, match="tabi_init"):
+        hip.dropin_matvec(x)
 
 
 def test_matvec_linearity_and_gmres_residual_at_larger_size():
