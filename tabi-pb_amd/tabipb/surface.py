@@ -368,9 +368,32 @@ def vertex_normals(verts, faces):
     return vn / nn[:, None]
 
 
-def ses_mesh(xyz, radius, probe=1.4, h=0.5):
-    """Solvent-excluded surface triangulation: (verts, normals, faces 0-based)."""
+def outer_component(v, faces):
+    """Keep the connected component of largest enclosed volume (the outer
+    surface; internal cavity surfaces are dropped, as MSMS does by default)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    g = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(len(v), len(v)))
+    nc, lab = connected_components(g, directed=False)
+    if nc == 1:
+        return v, faces
+    fl = lab[faces[:, 0]]
+    six_vol = np.einsum("ij,ij->i", v[faces[:, 0]], np.cross(v[faces[:, 1]], v[faces[:, 2]]))
+    vol = np.bincount(fl, weights=six_vol, minlength=nc)
+    keep = faces[fl == np.argmax(vol)]
+    used = np.unique(keep)
+    remap = -np.ones(len(v), dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    return v[used], remap[keep]
+
+
+def ses_mesh(xyz, radius, probe=1.4, h=0.5, cavities=False):
+    """Solvent-excluded surface triangulation: (verts, normals, faces 0-based).
+    With cavities=False only the outer surface is kept."""
     f, lo, h = ses_field(xyz, radius, probe, h)
     v, fc = marching_tets(f, lo, h)
     v, fc = clean_mesh(v, fc, h)
+    if not cavities:
+        v, fc = outer_component(v, fc)
     return v, vertex_normals(v, fc), fc
