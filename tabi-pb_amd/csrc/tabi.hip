@@ -106,7 +106,7 @@ struct tabi_ctx {
     HostTree tree;
     // geometry, tree order
     DBuf<double> tx, ty, tz, tnx, tny, tnz, tarea;
-    DBuf<double> src8;                   // {x,y,z,nx,ny,nz,area,0} per face
+    DBuf<double> src8;                   // {x,y,z,nx,ny,nz,A*x_phi,A*x_psi} per face
     // nodes
     DBuf<double4> ngeo;                  // {xmid,ymid,zmid,radius}
     DBuf<int32_t> nnum, nchild, ncoff, nkids, nibeg;
@@ -130,7 +130,7 @@ struct tabi_ctx {
     std::vector<int64_t> h_blk_off;
     std::vector<int32_t> h_blk_m;
     DBuf<int64_t> blk_off;
-    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, leaf_of, lu_info;
+    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info;
     DBuf<double> lu;
     // vectors
     DBuf<double> vx, vy, red;
@@ -474,12 +474,65 @@ __global__ void __launch_bounds__(64) k_far(const FarTask *__restrict__ tasks, i
 // pair kernel of COMPP_DIRECT_PB, src/treecode3d_pb.f:965-1001; the self term
 // is excluded (src/treecode3d_pb.f:1169-1172)
 // ---------------------------------------------------------------------------
+// 1/sqrt(r2) to full double precision: hardware estimate + two Newton steps
+__device__ __forceinline__ double rsq_nr(double r2)
+{
+    double y = __builtin_amdgcn_rsq(r2);
+    const double hx = 0.5 * r2;
+    double t = hx * y;
+    y = y * fma(-t, y, 1.5);
+    t = hx * y;
+    y = y * fma(-t, y, 1.5);
+    return y;
+}
+
+// per-matvec source weights in the source records: {.., A*phi, A*dphi/dn}
+__global__ void k_src_weights(int n, const double *__restrict__ tarea,
+                              const double *__restrict__ x, double *__restrict__ src8)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double a = tarea[j];
+    src8[(size_t)j * 8 + 6] = a * x[j];
+    src8[(size_t)j * 8 + 7] = a * x[n + j];
+}
+
+// L1..L4 of one (target, source) pair; (dx,dy,dz) = source - target
+struct PairL {
+    double l1, l2, l3, l4;
+};
+
+__device__ __forceinline__ PairL pair_l(double dx, double dy, double dz, double r2,
+                                        double vx, double vy, double vz, double v0x,
+                                        double v0y, double v0z, double kappa, double kappa2,
+                                        double eps, double inv_eps)
+{
+    const double inv = rsq_nr(r2);
+    const double rs = r2 * inv;
+    const double krs = kappa * rs;
+    const double ek = exp(-krs);
+    const double g0 = kOneOver4Pi * inv;
+    const double gk = ek * g0;
+    const double ct = (vx * dx + vy * dy + vz * dz) * inv;
+    const double ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
+    const double tp1 = g0 * inv;
+    const double tp2 = fma(krs, ek, ek);                 // (1 + kappa r) e^{-kappa r}
+    const double g1 = ct * tp1;
+    const double g10 = ct0 * tp1;
+    const double g3 = (v0x * vx + v0y * vy + v0z * vz - 3.0 * ct0 * ct) * inv * tp1;
+    PairL o;
+    o.l1 = g1 * fma(-eps, tp2, 1.0);                     // G1 - eps G2
+    o.l2 = g0 - gk;                                      // G0 - Gk
+    o.l3 = fma(g3, tp2 - 1.0, -kappa2 * ct0 * ct * gk);   // G4 - G3
+    o.l4 = g10 * fma(-inv_eps, tp2, 1.0);                // G10 - G20/eps
+    return o;
+}
+
 __global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks, int ntask, int n,
-    const double *__restrict__ src8, const double *__restrict__ x,
-    const double *__restrict__ tx, const double *__restrict__ ty,
-    const double *__restrict__ tz, const double *__restrict__ tnx,
-    const double *__restrict__ tny, const double *__restrict__ tnz, double kappa, double eps,
-    double2 *__restrict__ part)
+    const double *__restrict__ src8, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double kappa, double eps, double2 *__restrict__ part)
 {
     const int task = blockIdx.x;
     if (task >= ntask) return;
@@ -494,31 +547,18 @@ __global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks,
     const double kappa2 = kappa * kappa, inv_eps = 1.0 / eps;
     double p1 = 0.0, p2 = 0.0;
     if (act) {
+#pragma unroll 2
         for (int j = tk.s0; j < tk.s1; ++j) {
             const double *r = src8 + (size_t)j * 8;
-            const double phi = x[j], psi = x[n + j];
             const bool use = (j != i);
             const double dx = r[0] - sx, dy = r[1] - sy, dz = r[2] - sz;
             double r2 = dx * dx + dy * dy + dz * dz;
             r2 = use ? r2 : 1.0;
-            const double rs = sqrt(r2);
-            const double inv = 1.0 / rs;
-            const double g0 = kOneOver4Pi * inv;
-            const double krs = kappa * rs;
-            const double ek = exp(-krs);
-            const double gk = ek * g0;
-            const double ct = (r[3] * dx + r[4] * dy + r[5] * dz) * inv;
-            const double ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
-            const double tp1 = g0 * inv;
-            const double tp2 = (1.0 + krs) * ek;
-            const double g10 = ct0 * tp1, g20 = tp2 * g10;
-            const double g1 = ct * tp1, g2 = tp2 * g1;
-            const double g3 = (v0x * r[3] + v0y * r[4] + v0z * r[5] - 3.0 * ct0 * ct) * inv * tp1;
-            const double g4 = tp2 * g3 - kappa2 * ct0 * ct * gk;
-            const double l1 = g1 - eps * g2, l2 = g0 - gk, l3 = g4 - g3, l4 = g10 - g20 * inv_eps;
-            const double area = use ? r[6] : 0.0;
-            p1 += (l1 * phi + l2 * psi) * area;
-            p2 += (l3 * phi + l4 * psi) * area;
+            const PairL L = pair_l(dx, dy, dz, r2, r[3], r[4], r[5], v0x, v0y, v0z, kappa,
+                                   kappa2, eps, inv_eps);
+            const double w1 = use ? r[6] : 0.0, w2 = use ? r[7] : 0.0;
+            p1 = fma(L.l1, w1, fma(L.l2, w2, p1));
+            p2 = fma(L.l3, w1, fma(L.l4, w2, p2));
         }
     }
     part[(size_t)task * kWave + lane] = make_double2(p1, p2);
@@ -755,40 +795,60 @@ __global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int32_t *__restri
     }
 }
 
-// z = M^-1 r for every leaf block: row interchanges, then blocked forward and
-// back substitution with the cached factors.  One workgroup per block.
+// Composite row permutation of each leaf's pivot sequence: (P r)[k] = r[perm[k]]
+// (the interchanges of DGETRS/DLASWP applied in order, src/dgetrf_dep.f:346-448).
+// One thread per leaf; runs once per surface.
+__global__ void k_lu_perm(int nleaf, const int32_t *__restrict__ blk_m,
+                          const int32_t *__restrict__ piv_off, const int32_t *__restrict__ piv,
+                          int32_t *__restrict__ perm)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf) return;
+    const int m = blk_m[l];
+    const int32_t *pv = piv + piv_off[l];
+    int32_t *pm = perm + piv_off[l];
+    for (int k = 0; k < m; ++k) pm[k] = k;
+    for (int k = 0; k < m; ++k) {
+        const int p = pv[k];
+        if (p != k) { const int32_t t = pm[k]; pm[k] = pm[p]; pm[p] = t; }
+    }
+}
+
+// z = M^-1 r for every leaf block with the cached factors: permuted gather, then
+// blocked forward (unit lower) and back (upper) substitution.  Each 64x64
+// diagonal tile is staged in LDS and solved by one wave; the rows outside the
+// tile are updated by all four waves with coalesced column reads.
 constexpr int kSolveB = 64;
 __global__ void __launch_bounds__(256) k_lu_solve(int n, const int32_t *__restrict__ blk_ibeg,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     const int32_t *__restrict__ piv_off, const double *__restrict__ lu,
-    const int32_t *__restrict__ piv, const double *__restrict__ r, double *__restrict__ z)
+    const int32_t *__restrict__ perm, const double *__restrict__ r, double *__restrict__ z)
 {
     extern __shared__ double zs[];
+    __shared__ double tile[kSolveB][kSolveB + 1];
     const int l = blockIdx.x, t = threadIdx.x;
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
     const double *A = lu + blk_off[l];
-    const int32_t *pv = piv + piv_off[l];
-    for (int q = t; q < nl; q += 256) {
-        zs[q] = r[ib + q];
-        zs[nl + q] = r[n + ib + q];
-    }
-    __syncthreads();
-    if (t == 0) {
-        for (int k = 0; k < m; ++k) {
-            const int p = pv[k];
-            if (p != k) { const double tmp = zs[k]; zs[k] = zs[p]; zs[p] = tmp; }
-        }
+    const int32_t *pm = perm + piv_off[l];
+    for (int k = t; k < m; k += 256) {
+        const int src = pm[k];
+        zs[k] = src < nl ? r[ib + src] : r[n + ib + src - nl];
     }
     __syncthreads();
     const int lane = t & 63, w = t >> 6;
     // forward: unit lower triangular
     for (int kb = 0; kb < m; kb += kSolveB) {
         const int bs = min(kSolveB, m - kb);
+        for (int e = t; e < bs * kSolveB; e += 256) {
+            const int c = e / kSolveB, rr = e % kSolveB;
+            if (rr < bs) tile[rr][c] = A[(size_t)(kb + c) * m + kb + rr];
+        }
+        __syncthreads();
         if (w == 0) {
             double zl = (lane < bs) ? zs[kb + lane] : 0.0;
-            for (int c = 0; c < bs; ++c) {
+            for (int c = 0; c < bs - 1; ++c) {
                 const double zc = __shfl(zl, c);
-                if (lane > c && lane < bs) zl -= A[(size_t)(kb + c) * m + kb + lane] * zc;
+                if (lane > c && lane < bs) zl -= tile[lane][c] * zc;
             }
             if (lane < bs) zs[kb + lane] = zl;
         }
@@ -805,12 +865,17 @@ __global__ void __launch_bounds__(256) k_lu_solve(int n, const int32_t *__restri
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int kb = bi * kSolveB;
         const int bs = min(kSolveB, m - kb);
+        for (int e = t; e < bs * kSolveB; e += 256) {
+            const int c = e / kSolveB, rr = e % kSolveB;
+            if (rr < bs && c < bs) tile[rr][c] = A[(size_t)(kb + c) * m + kb + rr];
+        }
+        __syncthreads();
         if (w == 0) {
             double zl = (lane < bs) ? zs[kb + lane] : 0.0;
             for (int c = bs - 1; c >= 0; --c) {
-                if (lane == c) zl = zl / A[(size_t)(kb + c) * m + kb + c];
+                if (lane == c) zl = zl / tile[c][c];
                 const double zc = __shfl(zl, c);
-                if (lane < c) zl -= A[(size_t)(kb + c) * m + kb + lane] * zc;
+                if (lane < c) zl -= tile[lane][c] * zc;
             }
             if (lane < bs) zs[kb + lane] = zl;
         }
@@ -872,41 +937,77 @@ __global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double *__rest
     }
 }
 
-// potential at each charge from the surface solution (potential_molecule,
-// src/main.f90:627-655): one workgroup per charge, deterministic tree reduction
-__global__ void __launch_bounds__(256) k_energy(int n, const double *__restrict__ qpos,
-    const double *__restrict__ tx, const double *__restrict__ ty,
-    const double *__restrict__ tz, const double *__restrict__ tnx,
-    const double *__restrict__ tny, const double *__restrict__ tnz,
-    const double *__restrict__ tarea, const double *__restrict__ x, double eps, double kappa,
-    double *__restrict__ ptl)
+// Potential at each charge from the surface solution (potential_molecule,
+// src/main.f90:627-655): tasks of (64 charges, a run of kEnergyChunk faces), one
+// wave each, faces read wave-uniformly from the source records (which hold
+// A*phi, A*dphi/dn of the solution); partials are summed in chunk order.
+constexpr int kEnergyChunk = 2048;
+__global__ void __launch_bounds__(64) k_energy(int n, int nq, int nchunk,
+    const double *__restrict__ qpos, const double *__restrict__ src8, double eps, double kappa,
+    double *__restrict__ part)
 {
-    __shared__ double red[256];
-    const int k = blockIdx.x, t = threadIdx.x;
-    const double cx = qpos[3 * k], cy = qpos[3 * k + 1], cz = qpos[3 * k + 2];
+    const int task = blockIdx.x;
+    const int g = task / nchunk, ch = task % nchunk;
+    const int lane = lane_id();
+    const int k = g * kWave + lane;
+    const bool valid = k < nq;
+    const double cx = valid ? qpos[3 * k] : 0.0, cy = valid ? qpos[3 * k + 1] : 0.0,
+                 cz = valid ? qpos[3 * k + 2] : 0.0;
+    const int f0 = ch * kEnergyChunk, f1 = min(n, f0 + kEnergyChunk);
     double s = 0.0;
-    for (int j = t; j < n; j += 256) {
-        const double dx = tx[j] - cx, dy = ty[j] - cy, dz = tz[j] - cz;
-        const double rs = sqrt(dx * dx + dy * dy + dz * dz);
-        const double inv = 1.0 / rs;
-        const double g0 = kOneOver4Pi * inv;
+#pragma unroll 2
+    for (int j = f0; j < f1; ++j) {
+        const double *r = src8 + (size_t)j * 8;
+        const double dx = r[0] - cx, dy = r[1] - cy, dz = r[2] - cz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double inv = rsq_nr(r2);
+        const double rs = r2 * inv;
         const double krs = kappa * rs;
         const double ek = exp(-krs);
-        const double gk = ek * g0;
-        const double ct = (tnx[j] * dx + tny[j] * dy + tnz[j] * dz) * inv;
-        const double tp1 = g0 * inv;
-        const double tp2 = (1.0 + krs) * ek;
-        const double g1 = ct * tp1, g2 = tp2 * g1;
-        const double h1 = g1 - eps * g2, h2 = g0 - gk;
-        s += tarea[j] * (h1 * x[j] + h2 * x[n + j]);
+        const double g0 = kOneOver4Pi * inv;
+        const double ct = (r[3] * dx + r[4] * dy + r[5] * dz) * inv;
+        const double g1 = ct * g0 * inv;
+        const double h1 = g1 * fma(-eps, fma(krs, ek, ek), 1.0);   // G1 - eps G2
+        const double h2 = g0 - ek * g0;                            // G0 - Gk
+        s = fma(h1, r[6], fma(h2, r[7], s));
     }
-    red[t] = s;
+    part[(size_t)task * kWave + lane] = s;
+}
+
+__global__ void k_energy_reduce(int nq, int nchunk, const double *__restrict__ part,
+                                double *__restrict__ ptl)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const int g = k / kWave, lane = k % kWave;
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) s += part[((size_t)g * nchunk + ch) * kWave + lane];
+    ptl[k] = s;
+}
+
+// Coulomb energy sum_{i<j} q_i q_j / (eps_p r_ij) (src/main.f90:149-153): one
+// thread per charge i over j > i, block partial sums
+__global__ void __launch_bounds__(256) k_coulomb(int nq, const double *__restrict__ qpos,
+    const double *__restrict__ q, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < nq) {
+        const double xi = qpos[3 * i], yi = qpos[3 * i + 1], zi = qpos[3 * i + 2];
+        for (int j = i + 1; j < nq; ++j) {
+            const double dx = xi - qpos[3 * j], dy = yi - qpos[3 * j + 1], dz = zi - qpos[3 * j + 2];
+            s += q[j] * rsq_nr(dx * dx + dy * dy + dz * dz);
+        }
+        s *= q[i];
+    }
+    red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) red[t] += red[t + w];
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (t == 0) ptl[k] = red[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1035,9 +1136,10 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     { KTimer kt(c, 0); dispatch_moments(c); }
     if (c->n_near_task) {
         KTimer kt(c, 1);
+        k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p, c->src8.p);
         k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, c->src8.p,
-            c->vx.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
-            c->prm.eps, c->part_near.p);
+            c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->prm.eps,
+            c->part_near.p);
     }
     if (c->n_far_task) { KTimer kt(c, 2); dispatch_far(c); }
     const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
@@ -1054,7 +1156,7 @@ void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
     const size_t shm = sizeof(double) * (size_t)c->mmax;
     KTimer kt(c, 4);
     k_lu_solve<<<c->nleaf, 256, shm, c->s>>>(c->n, c->blk_ibeg.p, c->blk_m.p, c->blk_off.p,
-        c->piv_off.p, c->lu.p, c->piv.p, dr, dz);
+        c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
     HIPCHK(hipGetLastError());
 }
 
@@ -1240,6 +1342,8 @@ void build_precond(tabi_ctx *c)
         }
         (void)tiles;
     }
+    c->perm.alloc(pof);
+    k_lu_perm<<<nblk(nl, 64), 64, 0, c->s>>>(nl, c->blk_m.p, c->piv_off.p, c->piv.p, c->perm.p);
     HIPCHK(hipGetLastError());
     std::vector<int32_t> info(nl);
     HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
@@ -1277,7 +1381,7 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
         nx[i] = nrm[3 * o]; ny[i] = nrm[3 * o + 1]; nz[i] = nrm[3 * o + 2]; ar[i] = area[o];
         double *r = &s8[(size_t)i * 8];
         r[0] = x[i]; r[1] = y[i]; r[2] = z[i]; r[3] = nx[i]; r[4] = ny[i]; r[5] = nz[i];
-        r[6] = ar[i]; r[7] = 0.0;
+        r[6] = 0.0; r[7] = 0.0;  // source weights, set per matvec
     }
     const int nn = T.nnode();
     std::vector<double4> geo(nn);
@@ -1508,28 +1612,33 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
 void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
                 const double *dxs, double *sol, double *cou)
 {
-    DBuf<double> dqp, dptl;
-    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
-    dptl.alloc(std::max(nq, 1));
-    if (nq > 0)
-        k_energy<<<nq, 256, 0, c->s>>>(c->n, dqp.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p,
-            c->tnz.p, c->tarea.p, dxs, c->prm.eps, c->prm.kappa, dptl.p);
-    HIPCHK(hipGetLastError());
-    std::vector<double> ptl(std::max(nq, 1));
-    HIPCHK(hipMemcpyAsync(ptl.data(), dptl.p, sizeof(double) * nq, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    KTimer kt(c, 7);
     const double para = 332.0716;  // src/main.f90:64
+    if (nq <= 0) { *sol = 0.0; *cou = 0.0; return; }
+    DBuf<double> dqp, dq, dptl, part, cpart;
+    dqp.upload(qpos, (size_t)3 * nq, c->s);
+    dq.upload(q, nq, c->s);
+    dptl.alloc(nq);
+    const int ngq = (nq + kWave - 1) / kWave;
+    const int nch = (c->n + kEnergyChunk - 1) / kEnergyChunk;
+    part.alloc((size_t)ngq * nch * kWave);
+    k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, dxs, c->src8.p);
+    k_energy<<<ngq * nch, 64, 0, c->s>>>(c->n, nq, nch, dqp.p, c->src8.p, c->prm.eps,
+                                         c->prm.kappa, part.p);
+    k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, nch, part.p, dptl.p);
+    const int ncb = (nq + 255) / 256;
+    cpart.alloc(ncb);
+    k_coulomb<<<ncb, 256, 0, c->s>>>(nq, dqp.p, dq.p, cpart.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> ptl(nq), cp(ncb);
+    HIPCHK(hipMemcpyAsync(ptl.data(), dptl.p, sizeof(double) * nq, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(cp.data(), cpart.p, sizeof(double) * ncb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
     double s = 0.0, cc = 0.0;
-    for (int k = 0; k < nq; ++k) {
-        s += q[k] * ptl[k];
-        for (int j = k + 1; j < nq; ++j) {
-            const double dx = qpos[3 * k] - qpos[3 * j], dy = qpos[3 * k + 1] - qpos[3 * j + 1],
-                         dz = qpos[3 * k + 2] - qpos[3 * j + 2];
-            cc += 1.0 / eps_p / std::sqrt(dx * dx + dy * dy + dz * dz) * q[k] * q[j];
-        }
-    }
+    for (int k = 0; k < nq; ++k) s += q[k] * ptl[k];
+    for (int b = 0; b < ncb; ++b) cc += cp[b];
     *sol = s * 0.5 * para * 4.0 * kPi;
-    *cou = cc * para;
+    *cou = cc / eps_p * para;
 }
 
 std::unique_ptr<tabi_ctx> g_default;
