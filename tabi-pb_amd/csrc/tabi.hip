@@ -815,79 +815,97 @@ __global__ void k_lu_perm(int nleaf, const int32_t *__restrict__ blk_m,
 }
 
 // z = M^-1 r for every leaf block with the cached factors: permuted gather, then
-// blocked forward (unit lower) and back (upper) substitution.  Each 64x64
-// diagonal tile is staged in LDS and solved by one wave; the rows outside the
-// tile are updated by all four waves with coalesced column reads.
+// blocked forward (unit lower) and back (upper) substitution.  One wavefront per
+// leaf: each 64x64 diagonal tile is held row-per-lane in registers and solved
+// with shuffles; the rows outside the tile are updated 64 at a time, each lane
+// issuing the tile's 64 column loads back to back so the factor stream keeps
+// many loads in flight.  z lives in LDS.
 constexpr int kSolveB = 64;
-__global__ void __launch_bounds__(256) k_lu_solve(int n, const int32_t *__restrict__ blk_ibeg,
+constexpr int kHalf = 32;
+
+// s - sum_{c < bs} A[c*m] * zt[c] for one row, loads issued 32 at a time
+__device__ __forceinline__ double gemv_row(const double *__restrict__ Ac, int m, int bs,
+                                           const double *zt, double s)
+{
+    double s1 = 0.0;
+#pragma unroll 1
+    for (int h = 0; h < kSolveB; h += kHalf) {
+        double a[kHalf];
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? Ac[(size_t)(h + c) * m] : 0.0;
+#pragma unroll
+        for (int c = 0; c < kHalf; c += 2) {
+            s = fma(-a[c], (h + c < bs) ? zt[h + c] : 0.0, s);
+            s1 = fma(-a[c + 1], (h + c + 1 < bs) ? zt[h + c + 1] : 0.0, s1);
+        }
+    }
+    return s + s1;
+}
+
+__global__ void __launch_bounds__(64) k_lu_solve(int n, const int32_t *__restrict__ blk_ibeg,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     const int32_t *__restrict__ piv_off, const double *__restrict__ lu,
     const int32_t *__restrict__ perm, const double *__restrict__ r, double *__restrict__ z)
 {
     extern __shared__ double zs[];
-    __shared__ double tile[kSolveB][kSolveB + 1];
-    const int l = blockIdx.x, t = threadIdx.x;
+    const int l = blockIdx.x, lane = threadIdx.x;
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
     const double *A = lu + blk_off[l];
     const int32_t *pm = perm + piv_off[l];
-    for (int k = t; k < m; k += 256) {
+    for (int k = lane; k < m; k += kWave) {
         const int src = pm[k];
         zs[k] = src < nl ? r[ib + src] : r[n + ib + src - nl];
     }
     __syncthreads();
-    const int lane = t & 63, w = t >> 6;
-    // forward: unit lower triangular
-    for (int kb = 0; kb < m; kb += kSolveB) {
-        const int bs = min(kSolveB, m - kb);
-        for (int e = t; e < bs * kSolveB; e += 256) {
-            const int c = e / kSolveB, rr = e % kSolveB;
-            if (rr < bs) tile[rr][c] = A[(size_t)(kb + c) * m + kb + rr];
+    const int nb = (m + kSolveB - 1) / kSolveB;
+    // forward substitution, L unit lower
+    for (int bi = 0; bi < nb; ++bi) {
+        const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
+        const bool in = lane < bs;
+        double zl = in ? zs[kb + lane] : 0.0;
+#pragma unroll 1
+        for (int h = 0; h < kSolveB; h += kHalf) {
+            double tr[kHalf];
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c)
+                tr[c] = (in && h + c < lane) ? A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c) zl -= tr[c] * __shfl(zl, h + c);
         }
+        if (in) zs[kb + lane] = zl;
         __syncthreads();
-        if (w == 0) {
-            double zl = (lane < bs) ? zs[kb + lane] : 0.0;
-            for (int c = 0; c < bs - 1; ++c) {
-                const double zc = __shfl(zl, c);
-                if (lane > c && lane < bs) zl -= tile[lane][c] * zc;
-            }
-            if (lane < bs) zs[kb + lane] = zl;
-        }
-        __syncthreads();
-        for (int row = kb + bs + t; row < m; row += 256) {
-            double s = zs[row];
-            for (int c = 0; c < bs; ++c) s -= A[(size_t)(kb + c) * m + row] * zs[kb + c];
-            zs[row] = s;
-        }
-        __syncthreads();
-    }
-    // backward: upper triangular
-    const int nblk = (m + kSolveB - 1) / kSolveB;
-    for (int bi = nblk - 1; bi >= 0; --bi) {
-        const int kb = bi * kSolveB;
-        const int bs = min(kSolveB, m - kb);
-        for (int e = t; e < bs * kSolveB; e += 256) {
-            const int c = e / kSolveB, rr = e % kSolveB;
-            if (rr < bs && c < bs) tile[rr][c] = A[(size_t)(kb + c) * m + kb + rr];
-        }
-        __syncthreads();
-        if (w == 0) {
-            double zl = (lane < bs) ? zs[kb + lane] : 0.0;
-            for (int c = bs - 1; c >= 0; --c) {
-                if (lane == c) zl = zl / tile[c][c];
-                const double zc = __shfl(zl, c);
-                if (lane < c) zl -= tile[lane][c] * zc;
-            }
-            if (lane < bs) zs[kb + lane] = zl;
-        }
-        __syncthreads();
-        for (int row = t; row < kb; row += 256) {
-            double s = zs[row];
-            for (int c = 0; c < bs; ++c) s -= A[(size_t)(kb + c) * m + row] * zs[kb + c];
-            zs[row] = s;
+        for (int r0 = kb + bs; r0 < m; r0 += kWave) {
+            const int row = r0 + lane;
+            if (row < m) zs[row] = gemv_row(A + (size_t)kb * m + row, m, bs, zs + kb, zs[row]);
         }
         __syncthreads();
     }
-    for (int q = t; q < nl; q += 256) {
+    // back substitution, U upper with diagonal
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
+        const bool in = lane < bs;
+        double zl = in ? zs[kb + lane] : 0.0;
+#pragma unroll 1
+        for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
+            double tr[kHalf];
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c)
+                tr[c] = (in && h + c >= lane && h + c < bs) ? A[(size_t)(kb + h + c) * m + kb + lane]
+                                                           : 0.0;
+#pragma unroll
+            for (int c = kHalf - 1; c >= 0; --c) {
+                if (lane == h + c && in) zl = zl / tr[c];
+                const double zc = __shfl(zl, h + c);
+                if (lane < h + c) zl -= tr[c] * zc;
+            }
+        }
+        if (in) zs[kb + lane] = zl;
+        __syncthreads();
+        for (int r0 = 0; r0 < kb; r0 += kWave)
+            zs[r0 + lane] = gemv_row(A + (size_t)kb * m + r0 + lane, m, bs, zs + kb, zs[r0 + lane]);
+        __syncthreads();
+    }
+    for (int q = lane; q < nl; q += kWave) {
         z[ib + q] = zs[q];
         z[n + ib + q] = zs[nl + q];
     }
@@ -1155,7 +1173,7 @@ void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
 {
     const size_t shm = sizeof(double) * (size_t)c->mmax;
     KTimer kt(c, 4);
-    k_lu_solve<<<c->nleaf, 256, shm, c->s>>>(c->n, c->blk_ibeg.p, c->blk_m.p, c->blk_off.p,
+    k_lu_solve<<<c->nleaf, kWave, shm, c->s>>>(c->n, c->blk_ibeg.p, c->blk_m.p, c->blk_off.p,
         c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
     HIPCHK(hipGetLastError());
 }
