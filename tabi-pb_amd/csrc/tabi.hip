@@ -90,7 +90,7 @@ struct DBuf {
 };
 
 struct NearTask {
-    int32_t g, s0, s1, pad;
+    int32_t g, s0, s1, leaf;  // target group, source range [s0,s1), leaf node
     u64 mask;
 };
 struct FarTask {
@@ -107,6 +107,8 @@ struct tabi_ctx {
     // geometry, tree order
     DBuf<double> tx, ty, tz, tnx, tny, tnz, tarea;
     DBuf<double> src8;                   // {x,y,z,nx,ny,nz,A*x_phi,A*x_psi} per face
+    DBuf<float> src32;                   // mixed mode: same, float, leaf-local coordinates
+    DBuf<float> mom_w32;                 // mixed mode: contraction vectors in float
     // nodes
     DBuf<double4> ngeo;                  // {xmid,ymid,zmid,radius}
     DBuf<int32_t> nnum, nchild, ncoff, nkids, nibeg;
@@ -562,6 +564,168 @@ __global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks,
         }
     }
     part[(size_t)task * kWave + lane] = make_double2(p1, p2);
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision (precision = 1): FP32 pair arithmetic and Taylor recurrences,
+// Kahan-compensated per-target sums inside a task, FP64 across tasks
+// ---------------------------------------------------------------------------
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// geometry part of the FP32 source records: coordinates relative to the
+// centre of the source's leaf box (so nearby pairs keep their digits)
+__global__ void k_src32_geom(int n, const int32_t *__restrict__ face_leafnode,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, float *__restrict__ src32)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double4 g = ngeo[face_leafnode[j]];
+    float *r = src32 + (size_t)j * 8;
+    r[0] = (float)(tx[j] - g.x);
+    r[1] = (float)(ty[j] - g.y);
+    r[2] = (float)(tz[j] - g.z);
+    r[3] = (float)tnx[j];
+    r[4] = (float)tny[j];
+    r[5] = (float)tnz[j];
+}
+
+__global__ void k_src32_weights(int n, const double *__restrict__ tarea,
+                                const double *__restrict__ x, float *__restrict__ src32)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double a = tarea[j];
+    src32[(size_t)j * 8 + 6] = (float)(a * x[j]);
+    src32[(size_t)j * 8 + 7] = (float)(a * x[n + j]);
+}
+
+__global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ tasks, int ntask,
+    int n, const float *__restrict__ src32, const double4 *__restrict__ ngeo,
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, const double *__restrict__ tnx,
+    const double *__restrict__ tny, const double *__restrict__ tnz, float kappa, float eps,
+    double2 *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const NearTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool act = (tk.mask >> lane) & 1ull;
+    const bool valid = i < n;
+    const double4 lg = ngeo[tk.leaf];
+    const float sx = valid ? (float)(tx[i] - lg.x) : 0.f, sy = valid ? (float)(ty[i] - lg.y) : 0.f,
+                sz = valid ? (float)(tz[i] - lg.z) : 0.f;
+    const float v0x = valid ? (float)tnx[i] : 0.f, v0y = valid ? (float)tny[i] : 0.f,
+                v0z = valid ? (float)tnz[i] : 0.f;
+    const float kappa2 = kappa * kappa, inv_eps = 1.f / eps;
+    const float c4 = (float)kOneOver4Pi;
+    Kahan p1, p2;
+    if (act) {
+#pragma unroll 4
+        for (int j = tk.s0; j < tk.s1; ++j) {
+            const float *r = src32 + (size_t)j * 8;
+            const bool use = (j != i);
+            const float dx = r[0] - sx, dy = r[1] - sy, dz = r[2] - sz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            r2 = use ? r2 : 1.f;
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float rs = r2 * inv;
+            const float krs = kappa * rs;
+            const float ek = __expf(-krs);
+            const float g0 = c4 * inv;
+            const float gk = ek * g0;
+            const float ct = (r[3] * dx + r[4] * dy + r[5] * dz) * inv;
+            const float ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
+            const float tp1 = g0 * inv;
+            const float tp2 = fmaf(krs, ek, ek);
+            const float g1 = ct * tp1, g10 = ct0 * tp1;
+            const float g3 = (v0x * r[3] + v0y * r[4] + v0z * r[5] - 3.f * ct0 * ct) * inv * tp1;
+            const float l1 = g1 * fmaf(-eps, tp2, 1.f);
+            const float l2 = g0 - gk;
+            const float l3 = fmaf(g3, tp2 - 1.f, -kappa2 * ct0 * ct * gk);
+            const float l4 = g10 * fmaf(-inv_eps, tp2, 1.f);
+            const float w1 = use ? r[6] : 0.f, w2 = use ? r[7] : 0.f;
+            p1.add(fmaf(l1, w1, l2 * w2));
+            p2.add(fmaf(l3, w1, l4 * w2));
+        }
+    }
+    part[(size_t)task * kWave + lane] = make_double2((double)p1.s, (double)p2.s);
+}
+
+template <int P>
+__global__ void __launch_bounds__(64) k_far32(const FarTask *__restrict__ tasks, int ntask, int n,
+    const int32_t *__restrict__ far_node, const u64 *__restrict__ far_mask,
+    const int32_t *__restrict__ node2mom, const float *__restrict__ mom_w,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, float kappa, double2 *__restrict__ part)
+{
+    constexpr int L = MomShape<P>::L;
+    constexpr int NJ = MomShape<P>::NJ;
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const FarTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    const float nx = valid ? (float)tnx[i] : 0.f, ny = valid ? (float)tny[i] : 0.f,
+                nz = valid ? (float)tnz[i] : 0.f;
+    Kahan s1, s2;
+    for (int it = tk.i0; it < tk.i1; ++it) {
+        const int nd = far_node[it];
+        const u64 m = far_mask[it];
+        if (!((m >> lane) & 1ull)) continue;
+        const double4 geo = ngeo[nd];
+        const float *W = mom_w + (size_t)node2mom[nd] * 8 * NJ;
+        const float dx = (float)(px - geo.x), dy = (float)(py - geo.y), dz = (float)(pz - geo.z);
+        float acc1 = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+        {
+            float a[NJ];
+            tabi::taylor<L, false, float>(dx, dy, dz, 0.f, a);
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                acc1 = fmaf(a[j], W[j], acc1);
+                ax = fmaf(a[j], W[2 * NJ + j], ax);
+                ay = fmaf(a[j], W[3 * NJ + j], ay);
+                az = fmaf(a[j], W[4 * NJ + j], az);
+            }
+        }
+        {
+            float a[NJ];
+            tabi::taylor<L, true, float>(dx, dy, dz, kappa, a);
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                acc1 = fmaf(a[j], W[NJ + j], acc1);
+                ax = fmaf(a[j], W[5 * NJ + j], ax);
+                ay = fmaf(a[j], W[6 * NJ + j], ay);
+                az = fmaf(a[j], W[7 * NJ + j], az);
+            }
+        }
+        s1.add(acc1);
+        s2.add(nx * ax + ny * ay + nz * az);
+    }
+    part[(size_t)task * kWave + lane] = make_double2((double)s1.s, (double)s2.s);
+}
+
+__global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 // y = pre*x - sum of the group's task partials, near tasks then far tasks
@@ -1114,6 +1278,14 @@ template <int P>
 void launch_far(tabi_ctx *c)
 {
     if (c->n_far_task == 0) return;
+    if (c->prm.precision == 1) {
+        const int64_t nw = (int64_t)c->nmom * 8 * c->nj;
+        k_to_float<<<nblk(nw, 256), 256, 0, c->s>>>(nw, c->mom_w.p, c->mom_w32.p);
+        k_far32<P><<<c->n_far_task, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n,
+            c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_w32.p, c->ngeo.p, c->tx.p,
+            c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, (float)c->prm.kappa, c->part_far.p);
+        return;
+    }
     k_far<P><<<c->n_far_task, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n,
         c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_w.p, c->ngeo.p, c->tx.p, c->ty.p,
         c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->part_far.p);
@@ -1154,10 +1326,18 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     { KTimer kt(c, 0); dispatch_moments(c); }
     if (c->n_near_task) {
         KTimer kt(c, 1);
-        k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p, c->src8.p);
-        k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, c->src8.p,
-            c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->prm.eps,
-            c->part_near.p);
+        if (c->prm.precision == 1) {
+            k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p,
+                                                               c->src32.p);
+            k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n,
+                c->src32.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
+                (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p);
+        } else {
+            k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p, c->src8.p);
+            k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n,
+                c->src8.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
+                c->prm.eps, c->part_near.p);
+        }
     }
     if (c->n_far_task) { KTimer kt(c, 2); dispatch_far(c); }
     const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
@@ -1265,7 +1445,7 @@ void build_lists(tabi_ctx *c)
         for (int32_t k = c->h_near_off[g]; k < c->h_near_off[g + 1]; ++k) {
             const int nd = near_leaf[k];
             const int b = T.ibeg[nd], e = T.iend[nd] + 1;
-            for (int s0 = b; s0 < e; s0 += S) nt.push_back({g, s0, std::min(e, s0 + S), 0, near_mask[k]});
+            for (int s0 = b; s0 < e; s0 += S) nt.push_back({g, s0, std::min(e, s0 + S), nd, near_mask[k]});
         }
         nto[g + 1] = (int32_t)nt.size();
         for (int32_t k = c->h_far_off[g]; k < c->h_far_off[g + 1]; k += F)
@@ -1308,6 +1488,19 @@ void build_lists(tabi_ctx *c)
         c->mom_chunk_off.upload(mco.data(), mco.size(), c->s);
         c->mom_part.alloc((size_t)c->nchunk * 4 * c->nk);
         c->mom_w.alloc((size_t)nm * 8 * c->nj);
+        if (c->prm.precision == 1) c->mom_w32.alloc((size_t)nm * 8 * c->nj);
+    }
+    if (c->prm.precision == 1) {
+        std::vector<int32_t> fl(c->n);
+        for (int32_t nd : T.leaves)
+            for (int i = T.ibeg[nd]; i <= T.iend[nd]; ++i) fl[i] = nd;
+        DBuf<int32_t> dfl;
+        dfl.upload(fl.data(), c->n, c->s);
+        c->src32.alloc((size_t)c->n * 8);
+        k_src32_geom<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, dfl.p, c->ngeo.p, c->tx.p, c->ty.p,
+            c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->src32.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->s));
     }
     HIPCHK(hipStreamSynchronize(c->s));
 }
@@ -1378,7 +1571,7 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     if (n < 1) throw Error("need at least one face");
     if (prm->order < 1 || prm->order > kMaxOrder) throw Error("Taylor order must be 1..6");
     if (prm->maxparnode < 1) throw Error("maxparnode must be >= 1");
-    if (prm->precision != 0) throw Error("only precision 0 (FP64) is available in this build");
+    if (prm->precision != 0 && prm->precision != 1) throw Error("precision must be 0 or 1");
     c->prm = *prm;
     c->n = n;
     c->p = prm->order;

@@ -195,3 +195,46 @@ def test_matvec_linearity_and_gmres_residual_at_larger_size():
     assert ierr == 0
     res = S.msolve(rhs - S.matvec(x))
     assert np.linalg.norm(res) / np.linalg.norm(S.msolve(rhs)) <= 1e-4 * (1 + 1e-6)
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (precision=1): FP32 near/far field with compensated sums,
+# FP64 GMRES, leaf LU, self term and energy.  Tolerances: matvec rel. L2 <= 1e-5
+# against the FP64 reference output, solvation energy rel. <= 1e-4 (the
+# headline bound is 1e-3), GMRES converges to the same tolerance.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["sph3_l100_p3", "sph4_off_p6", "sph3_k0_p2", "sph5_born"])
+def test_mixed_precision_matvec_against_reference(name):
+    g = load_golden(name)
+    P = problem(name)
+    S = hip.Surface(P.params(precision=1), P.xyz, P.nrm, P.area)
+    np.testing.assert_array_equal(S.orderarr, g["orderarr"])
+    s = int(g["stride"])
+    x = g["x"] if s == 1 else cases.harness_x(2 * S.n)
+    y = S.matvec(x)
+    assert rel_l2(y[::s], g["y"]) <= 1e-5
+    z = S.msolve(x)
+    assert rel_l2(z[::s], g["z"]) <= RTOL   # the preconditioner stays FP64
+    b = S.rhs(P.qpos, P.q, P.usr.epsp)
+    xs, it, err, ierr = S.gmres(b, tol=1e-4)
+    assert ierr == 0 and err <= 1e-4
+    sol, _ = S.energy(P.qpos, P.q, P.usr.epsp, xs)
+    assert sol == pytest.approx(float(g["soleng"]), rel=1e-4)
+    S.close()
+
+
+def test_mixed_precision_full_solve_on_molecule():
+    """A 200-atom synthetic molecule: the mixed-precision solve's energy within
+    1e-4 of the FP64 solve, both converged."""
+    from tabipb import surface, meshio
+    xyz, q, r = surface.synth_molecule(200, seed=3)
+    v, nv, f = surface.ses_mesh(xyz, r, probe=1.4, h=0.5)
+    f, _ = driver.cull_faces(v, f)
+    fx, fn, fa = driver.face_geometry(v, nv, f)
+    usr = meshio.UsrData(maxparnode=300, order=3, mac=0.8)
+    P = driver.Problem(usr, fx, fn, fa, xyz.astype(np.float32).astype(np.float64),
+                       q.astype(np.float32).astype(np.float64))
+    r64 = driver.solve(P, precision=0)
+    r32 = driver.solve(P, precision=1)
+    assert r64["ierr"] == 0 and r32["ierr"] == 0
+    assert abs(r32["soleng"] - r64["soleng"]) / abs(r64["soleng"]) < 1e-4
