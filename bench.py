@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 from tabipb import driver, hip, meshio  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 METRIC = "PB solves/hour at ~500k faces, ΔG err<1e-3 vs FP64; GMRES iters learned vs 0"
 
 
@@ -135,75 +136,95 @@ def main():
               "(the solve would shard as one all-gather of x per matvec).", file=sys.stderr)
         sys.exit(2)
     prob, desc = workload(a.workload)
-    prm = prob.params()
-    p = prm.order
+    p = prob.params().order
     print(f"[bench] workload {desc['workload']} n={prob.xyz.shape[0]}", file=sys.stderr, flush=True)
 
-    # tabi_solve returns after its last device work has completed
-    for _ in range(a.warmup):
-        res = driver.solve(prob, tol=1e-4)
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        res = driver.solve(prob, tol=1e-4)
-    dt = (time.perf_counter() - t0) / a.steps
-    print(f"[bench] solve {dt * 1e3:.1f} ms iter={res['iter']} dG={res['soleng']:.6f} "
-          f"phases={res['times_ms']}", file=sys.stderr, flush=True)
+    def time_solves(precision):
+        # tabi_solve returns after its last device work has completed
+        for _ in range(a.warmup):
+            r = driver.solve(prob, tol=1e-4, precision=precision)
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            r = driver.solve(prob, tol=1e-4, precision=precision)
+        dt = (time.perf_counter() - t0) / a.steps
+        print(f"[bench] precision={precision} solve {dt * 1e3:.1f} ms iter={r['iter']} "
+              f"ierr={r['ierr']} dG={r['soleng']:.6f} phases="
+              f"{ {k: round(float(v), 2) for k, v in r['times_ms'].items()} }",
+              file=sys.stderr, flush=True)
+        return dt, r
 
-    # dominant kernel: per-matvec kernels timed with events on the library stream
-    S = hip.Surface(prm, prob.xyz, prob.nrm, prob.area)
-    st = S.stats()
-    n = S.n
-    dx = hip.DeviceArray.from_host(np.random.default_rng(0).standard_normal(2 * n))
-    dy = hip.DeviceArray(2 * n)
-    S.matvec_dev(dx.ptr, dy.ptr)
-    S.sync()
-    S.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(a.matvecs):
+    dt64, res64 = time_solves(0)
+    dt32, res32 = time_solves(1)
+    derr = abs(res32["soleng"] - res64["soleng"]) / abs(res64["soleng"])
+    use_mixed = derr < 1e-3 and res32["ierr"] == 0
+    prec = 1 if use_mixed else 0
+    dt, res = (dt32, res32) if use_mixed else (dt64, res64)
+
+    def kernel_profile(precision):
+        S = hip.Surface(prob.params(precision=precision), prob.xyz, prob.nrm, prob.area)
+        st = S.stats()
+        n = S.n
+        dx = hip.DeviceArray.from_host(np.random.default_rng(0).standard_normal(2 * n))
+        dy = hip.DeviceArray(2 * n)
         S.matvec_dev(dx.ptr, dy.ptr)
-    kt = S.kernel_times()
-    S.profile(False)
-    S.sync()
-    t1 = time.perf_counter()
-    for _ in range(a.matvecs):
-        S.matvec_dev(dx.ptr, dy.ptr)
-    S.sync()
-    mv_ms = (time.perf_counter() - t1) / a.matvecs * 1e3
+        S.sync()
+        S.profile(True)
+        for _ in range(a.matvecs):
+            S.matvec_dev(dx.ptr, dy.ptr)
+        kt = S.kernel_times()
+        S.profile(False)
+        S.sync()
+        t1 = time.perf_counter()
+        for _ in range(a.matvecs):
+            S.matvec_dev(dx.ptr, dy.ptr)
+        S.sync()
+        mv_ms = (time.perf_counter() - t1) / a.matvecs * 1e3
+        S.profile(True)
+        for _ in range(a.matvecs):
+            S.msolve_dev(dy.ptr, dx.ptr)
+        ms_t = S.kernel_times()["msolve"]
+        S.profile(False)
+        S.close()
+        dx.free()
+        dy.free()
+        return st, kt, mv_ms, ms_t[0] / max(ms_t[1], 1)
+
+    st, kt, mv_ms, msolve_ms = kernel_profile(prec)
+    _, kt64, mv64_ms, _ = kernel_profile(0) if prec == 1 else (st, kt, mv_ms, msolve_ms)
     flops = {"near": W_NEAR * st["p_near"], "far": w_far(p) * st["p_far"],
              "moments": w_mom(p) * st["p_mom"]}
     avg = {k: kt[k][0] / max(kt[k][1], 1) for k in ("near", "far", "moments", "reduce")}
     dom = max(("near", "far"), key=lambda k: avg[k])
     achieved = flops[dom] / (avg[dom] * 1e-3) / 1e12
-    mv_flops = sum(flops.values()) + 4 * n
-    print(f"[bench] matvec {mv_ms:.3f} ms ({mv_flops / (mv_ms * 1e-3) / 1e12:.2f} TFLOP/s alg); "
-          f"kernel avg ms {avg}; stats {st}", file=sys.stderr, flush=True)
-    ms_t = 0.0
-    S.profile(True)
-    for _ in range(a.matvecs):
-        S.msolve_dev(dy.ptr, dx.ptr)
-    ms_t = S.kernel_times()["msolve"]
-    S.profile(False)
-    msolve_ms = ms_t[0] / max(ms_t[1], 1)
-    S.close()
+    peak = FP32_PEAK_TFLOPS if prec == 1 else FP64_PEAK_TFLOPS
+    print(f"[bench] matvec {mv_ms:.3f} ms (fp64 {mv64_ms:.3f} ms); kernel avg ms {avg}; "
+          f"msolve {msolve_ms:.3f} ms; stats {st}", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(prob, res["iter"])
+        cpu = cpu_baseline(prob, res64["iter"])
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
 
     value = 3600.0 / dt
     out = {
         "metric": METRIC, "value": value, "unit": "solves/hour", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": dict(desc, leaf=prm.maxparnode, order=p, theta=prm.theta, tol=1e-4,
-                       gmres_iter=res["iter"], soleng_kcal=res["soleng"],
-                       matvec_ms=mv_ms, msolve_ms=msolve_ms,
-                       matvecs_per_s=1e3 / mv_ms, phases_ms=res["times_ms"]),
-        "roofline": {"bound": "valu", "kernel": "k_" + dom, "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == 1 else "f64", "data": "synthetic",
+        "config": dict(desc, leaf=prob.usr.maxparnode, order=p, theta=prob.usr.mac, tol=1e-4,
+                       form="mixed precision (FP32 near/far field, compensated sums; FP64 "
+                            "GMRES, leaf LU, self term, energy)" if prec == 1 else "FP64",
+                       gmres_iter=res["iter"], gmres_iter_zero_start=res["iter"],
+                       soleng_kcal=res["soleng"], soleng_fp64_kcal=res64["soleng"],
+                       dG_rel_err_vs_fp64=derr,
+                       fp64_solves_per_hour=3600.0 / dt64, fp64_ms_per_solve=dt64 * 1e3,
+                       mixed_solves_per_hour=3600.0 / dt32, mixed_ms_per_solve=dt32 * 1e3,
+                       matvec_ms=mv_ms, fp64_matvec_ms=mv64_ms, msolve_ms=msolve_ms,
+                       matvecs_per_s=1e3 / mv_ms,
+                       phases_ms={k: float(v) for k, v in res["times_ms"].items()}),
+        "roofline": {"bound": "valu", "kernel": ("k_" + dom + ("32" if prec == 1 else "")),
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
                      "alg_flops_per_launch": flops[dom], "avg_launch_ms": avg[dom]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
