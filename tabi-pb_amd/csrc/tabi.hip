@@ -551,14 +551,15 @@ __global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks,
     if (act) {
 #pragma unroll 2
         for (int j = tk.s0; j < tk.s1; ++j) {
-            const double *r = src8 + (size_t)j * 8;
+            const double4 ra = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8);
+            const double4 rb = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8 + 4);
             const bool use = (j != i);
-            const double dx = r[0] - sx, dy = r[1] - sy, dz = r[2] - sz;
+            const double dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
             double r2 = dx * dx + dy * dy + dz * dz;
             r2 = use ? r2 : 1.0;
-            const PairL L = pair_l(dx, dy, dz, r2, r[3], r[4], r[5], v0x, v0y, v0z, kappa,
+            const PairL L = pair_l(dx, dy, dz, r2, ra.w, rb.x, rb.y, v0x, v0y, v0z, kappa,
                                    kappa2, eps, inv_eps);
-            const double w1 = use ? r[6] : 0.0, w2 = use ? r[7] : 0.0;
+            const double w1 = use ? rb.z : 0.0, w2 = use ? rb.w : 0.0;
             p1 = fma(L.l1, w1, fma(L.l2, w2, p1));
             p2 = fma(L.l3, w1, fma(L.l4, w2, p2));
         }
@@ -636,9 +637,10 @@ __global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ task
     if (act) {
 #pragma unroll 4
         for (int j = tk.s0; j < tk.s1; ++j) {
-            const float *r = src32 + (size_t)j * 8;
+            const float4 ra = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8);
+            const float4 rb = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8 + 4);
             const bool use = (j != i);
-            const float dx = r[0] - sx, dy = r[1] - sy, dz = r[2] - sz;
+            const float dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
             float r2 = dx * dx + dy * dy + dz * dz;
             r2 = use ? r2 : 1.f;
             const float inv = __builtin_amdgcn_rsqf(r2);
@@ -647,17 +649,17 @@ __global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ task
             const float ek = __expf(-krs);
             const float g0 = c4 * inv;
             const float gk = ek * g0;
-            const float ct = (r[3] * dx + r[4] * dy + r[5] * dz) * inv;
+            const float ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
             const float ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
             const float tp1 = g0 * inv;
             const float tp2 = fmaf(krs, ek, ek);
             const float g1 = ct * tp1, g10 = ct0 * tp1;
-            const float g3 = (v0x * r[3] + v0y * r[4] + v0z * r[5] - 3.f * ct0 * ct) * inv * tp1;
+            const float g3 = (v0x * ra.w + v0y * rb.x + v0z * rb.y - 3.f * ct0 * ct) * inv * tp1;
             const float l1 = g1 * fmaf(-eps, tp2, 1.f);
             const float l2 = g0 - gk;
             const float l3 = fmaf(g3, tp2 - 1.f, -kappa2 * ct0 * ct * gk);
             const float l4 = g10 * fmaf(-inv_eps, tp2, 1.f);
-            const float w1 = use ? r[6] : 0.f, w2 = use ? r[7] : 0.f;
+            const float w1 = use ? rb.z : 0.f, w2 = use ? rb.w : 0.f;
             p1.add(fmaf(l1, w1, l2 * w2));
             p2.add(fmaf(l3, w1, l4 * w2));
         }
@@ -1139,19 +1141,20 @@ __global__ void __launch_bounds__(64) k_energy(int n, int nq, int nchunk,
     double s = 0.0;
 #pragma unroll 2
     for (int j = f0; j < f1; ++j) {
-        const double *r = src8 + (size_t)j * 8;
-        const double dx = r[0] - cx, dy = r[1] - cy, dz = r[2] - cz;
+        const double4 ra = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8);
+        const double4 rb = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8 + 4);
+        const double dx = ra.x - cx, dy = ra.y - cy, dz = ra.z - cz;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const double inv = rsq_nr(r2);
         const double rs = r2 * inv;
         const double krs = kappa * rs;
         const double ek = exp(-krs);
         const double g0 = kOneOver4Pi * inv;
-        const double ct = (r[3] * dx + r[4] * dy + r[5] * dz) * inv;
+        const double ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
         const double g1 = ct * g0 * inv;
         const double h1 = g1 * fma(-eps, fma(krs, ek, ek), 1.0);   // G1 - eps G2
         const double h2 = g0 - ek * g0;                            // G0 - Gk
-        s = fma(h1, r[6], fma(h2, r[7], s));
+        s = fma(h1, rb.z, fma(h2, rb.w, s));
     }
     part[(size_t)task * kWave + lane] = s;
 }
