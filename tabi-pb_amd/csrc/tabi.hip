@@ -814,19 +814,21 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
     }
 }
 
-// Panel factorisation of columns k0..k0+nb-1 of every block with m > k0:
-// partial pivoting with the first-maximum rule (IDAMAX), reciprocal scaling and
-// rank-1 updates restricted to the panel (the unblocked step of DGETRF2's
-// semantics, src/dgetrf_dep.f:757-914).
-__global__ void __launch_bounds__(256) k_lu_panel(int k0, const int32_t *__restrict__ blk_m,
-    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
-    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info)
+// Batched LU: leaf blocks are processed largest first (lorder[] sorts leaves by
+// block size, descending), so the blocks still active at panel step k0 are a
+// prefix of lorder and every launch covers only active blocks.
+
+// Panel factorisation of columns k0..k0+nb-1: partial pivoting with the
+// first-maximum rule (IDAMAX), reciprocal scaling and rank-1 updates restricted
+// to the panel (the unblocked step of DGETRF2's semantics,
+// src/dgetrf_dep.f:757-914).  One wavefront per block; reductions by shuffles.
+__global__ void __launch_bounds__(64) k_lu_panel(int k0, const int32_t *__restrict__ lorder,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    const int32_t *__restrict__ piv_off, double *__restrict__ lu, int32_t *__restrict__ piv,
+    int32_t *__restrict__ info)
 {
-    __shared__ double sv[256];
-    __shared__ int si[256];
-    const int l = blockIdx.x, t = threadIdx.x;
+    const int l = lorder[blockIdx.x], lane = threadIdx.x;
     const int m = blk_m[l];
-    if (m <= k0) return;
     const int nbe = min(kLuNB, m - k0);
     double *A = lu + blk_off[l];
     int32_t *pv = piv + piv_off[l];
@@ -835,25 +837,20 @@ __global__ void __launch_bounds__(256) k_lu_panel(int k0, const int32_t *__restr
         double *Ac = A + (size_t)col * m;
         double best = -1.0;
         int bi = 0x7fffffff;
-        for (int r = col + t; r < m; r += 256) {
+        for (int r = col + lane; r < m; r += kWave) {
             const double v = fabs(Ac[r]);
             if (v > best) { best = v; bi = r; }
         }
-        sv[t] = best; si[t] = bi;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (t < s) {
-                const double v2 = sv[t + s];
-                const int i2 = si[t + s];
-                if (v2 > sv[t] || (v2 == sv[t] && i2 < si[t])) { sv[t] = v2; si[t] = i2; }
-            }
-            __syncthreads();
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
-        const int prow = si[0];
-        __syncthreads();
-        if (t == 0) pv[col] = prow;
-        if (prow != col && t < nbe) {
-            double *cc = A + (size_t)(k0 + t) * m;
+        const int prow = __shfl(bi, 0);
+        if (lane == 0) pv[col] = prow;
+        if (prow != col && lane < nbe) {
+            double *cc = A + (size_t)(k0 + lane) * m;
             const double tmp = cc[col];
             cc[col] = cc[prow];
             cc[prow] = tmp;
@@ -861,34 +858,33 @@ __global__ void __launch_bounds__(256) k_lu_panel(int k0, const int32_t *__restr
         __syncthreads();
         const double pivot = Ac[col];
         if (pivot == 0.0) {
-            if (t == 0) atomicCAS(info + l, 0, col + 1);
-            __syncthreads();
+            if (lane == 0) atomicCAS(info + l, 0, col + 1);
             continue;
         }
         const double rinv = 1.0 / pivot;
-        for (int r = col + 1 + t; r < m; r += 256) Ac[r] *= rinv;
+        for (int r = col + 1 + lane; r < m; r += kWave) Ac[r] *= rinv;
         __syncthreads();
-        const int rows = m - col - 1, cols = nbe - c - 1;
-        for (int e = t; e < rows * cols; e += 256) {
-            const int cc = e / rows, r = col + 1 + e % rows;
-            double *Aj = A + (size_t)(col + 1 + cc) * m;
-            Aj[r] -= Ac[r] * Aj[col];
+        for (int cc = col + 1; cc < k0 + nbe; ++cc) {
+            double *Aj = A + (size_t)cc * m;
+            const double u = Aj[col];
+            for (int r = col + 1 + lane; r < m; r += kWave) Aj[r] = fma(-Ac[r], u, Aj[r]);
         }
         __syncthreads();
     }
 }
 
 // Apply the panel's row interchanges to the columns outside the panel and form
-// U12 = L11^-1 A12 for the columns to its right.  One thread per column.
-__global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__restrict__ blk_m,
-    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
-    double *__restrict__ lu, const int32_t *__restrict__ piv)
+// U12 = L11^-1 A12 for the columns to its right.  One thread per column; grid
+// y runs over the active blocks.
+__global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__restrict__ lorder,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    const int32_t *__restrict__ piv_off, double *__restrict__ lu, const int32_t *__restrict__ piv)
 {
     __shared__ double L11[kLuNB][kLuNB + 1];
-    const int l = blockIdx.y;
+    const int l = lorder[blockIdx.y];
     const int m = blk_m[l];
-    if (m <= k0) return;
     const int nbe = min(kLuNB, m - k0);
+    if (blockIdx.x * blockDim.x >= m) return;
     double *A = lu + blk_off[l];
     const int32_t *pv = piv + piv_off[l];
     for (int e = threadIdx.x; e < nbe * nbe; e += blockDim.x) {
@@ -908,7 +904,6 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
         }
     }
     if (j < k0) return;
-    // U12 column j: forward substitution with unit-lower L11, in place
     for (int c = 1; c < nbe; ++c) {
         double s = Aj[k0 + c];
         for (int c2 = 0; c2 < c; ++c2) s -= L11[c][c2] * Aj[k0 + c2];
@@ -916,18 +911,19 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
     }
 }
 
-// Trailing update A22 -= L21 U12 on 64x64 tiles; 256 threads x (4x4) outputs.
-__global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int32_t *__restrict__ blk_m,
-    const int64_t *__restrict__ blk_off, double *__restrict__ lu)
+// Trailing update A22 -= L21 U12 on 64x64 tiles from a precomputed tile list
+// {block, tile row, tile col}; 256 threads x (4x4) outputs.
+__global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int4 *__restrict__ tiles,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    double *__restrict__ lu)
 {
     __shared__ double sL[kLuNB][64 + 1];
     __shared__ double sU[kLuNB][64 + 1];
-    const int l = blockIdx.z;
+    const int4 tl = tiles[blockIdx.x];
+    const int l = tl.x;
     const int m = blk_m[l];
-    if (m <= k0 + kLuNB) return;
     const int nbe = kLuNB;
-    const int r0 = k0 + nbe + blockIdx.y * 64, c0 = k0 + nbe + blockIdx.x * 64;
-    if (r0 >= m || c0 >= m) return;
+    const int r0 = k0 + nbe + tl.y * 64, c0 = k0 + nbe + tl.z * 64;
     double *A = lu + blk_off[l];
     const int t = threadIdx.x;
     for (int e = t; e < nbe * 64; e += 256) {
@@ -1543,19 +1539,43 @@ void build_precond(tabi_ctx *c)
         c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
         c->prm.kappa, c->prm.eps, c->prm.flags, c->lu.p);
     HIPCHK(hipGetLastError());
-    const int tiles = (c->mmax + 63) / 64;
+    // largest blocks first: the active blocks of each panel step are a prefix
+    std::vector<int32_t> lorder(nl);
+    for (int l = 0; l < nl; ++l) lorder[l] = l;
+    std::stable_sort(lorder.begin(), lorder.end(), [&](int a, int b) { return bm[a] > bm[b]; });
+    std::vector<int> nact;
+    std::vector<int4> tl;
+    std::vector<int64_t> tl_off(1, 0);
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB) {
-        k_lu_panel<<<nl, 256, 0, c->s>>>(k0, c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p,
-                                         c->piv.p, c->lu_info.p);
-        k_lu_swap_trsm<<<dim3(nblk(c->mmax, 256), nl), 256, 0, c->s>>>(k0, c->blk_m.p,
-            c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
-        const int rem = c->mmax - k0 - kLuNB;
-        if (rem > 0) {
+        int na = 0;
+        while (na < nl && bm[lorder[na]] > k0) ++na;
+        nact.push_back(na);
+        for (int q = 0; q < na; ++q) {
+            const int rem = bm[lorder[q]] - k0 - kLuNB;
+            if (rem <= 0) continue;
             const int tt = (rem + 63) / 64;
-            k_lu_gemm<<<dim3(tt, tt, nl), 256, 0, c->s>>>(k0, c->blk_m.p, c->blk_off.p, c->lu.p);
+            for (int y = 0; y < tt; ++y)
+                for (int x = 0; x < tt; ++x) tl.push_back(make_int4(lorder[q], y, x, 0));
         }
-        (void)tiles;
+        tl_off.push_back((int64_t)tl.size());
     }
+    DBuf<int32_t> dlorder;
+    DBuf<int4> dtl;
+    dlorder.upload(lorder.data(), nl, c->s);
+    if (!tl.empty()) dtl.upload(tl.data(), tl.size(), c->s);
+    int step = 0;
+    for (int k0 = 0; k0 < c->mmax; k0 += kLuNB, ++step) {
+        const int na = nact[step];
+        k_lu_panel<<<na, kWave, 0, c->s>>>(k0, dlorder.p, c->blk_m.p, c->blk_off.p, c->piv_off.p,
+                                          c->lu.p, c->piv.p, c->lu_info.p);
+        k_lu_swap_trsm<<<dim3(nblk(bm[lorder[0]], 256), na), 256, 0, c->s>>>(k0, dlorder.p,
+            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
+        const int64_t nt = tl_off[step + 1] - tl_off[step];
+        if (nt > 0)
+            k_lu_gemm<<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
+                                                      c->blk_off.p, c->lu.p);
+    }
+    HIPCHK(hipGetLastError());
     c->perm.alloc(pof);
     k_lu_perm<<<nblk(nl, 64), 64, 0, c->s>>>(nl, c->blk_m.p, c->piv_off.p, c->piv.p, c->perm.p);
     HIPCHK(hipGetLastError());
