@@ -110,6 +110,10 @@ int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q
 int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
                    int32_t nrmax, int32_t *iter, double *err, int32_t *ierr);
 
+/* The library keeps freed device blocks for reuse by later contexts and solves;
+ * this returns all cached (unused) blocks to the runtime. */
+int tabi_release_memory(void);
+
 /* Device-memory plumbing for callers that hold no other GPU runtime (so the
  * device-pointer entry points can be used from C, Fortran or ctypes). */
 int tabi_device_alloc(int64_t bytes, void **out);

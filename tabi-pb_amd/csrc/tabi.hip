@@ -25,7 +25,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -60,6 +62,72 @@ struct Error : std::runtime_error {
             throw Error(std::string(#expr) + " failed: " + hipGetErrorString(e_));     \
     } while (0)
 
+// Caching device allocator: freed blocks are kept per size class and handed
+// out again, so repeated solves of similar surfaces do not pay hipMalloc /
+// hipFree (which map and unmap pages) each time.  Memory is returned to the
+// runtime by tabi_release_memory().
+class DevicePool {
+public:
+    static DevicePool &get()
+    {
+        static DevicePool p;
+        return p;
+    }
+    void *alloc(size_t bytes)
+    {
+        const size_t cls = size_class(bytes);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = free_.find(cls);
+            if (it != free_.end() && !it->second.empty()) {
+                void *p = it->second.back();
+                it->second.pop_back();
+                return p;
+            }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, cls);
+        if (e != hipSuccess) {
+            release_all();
+            HIPCHK(hipMalloc(&p, cls));
+        }
+        std::lock_guard<std::mutex> g(mu_);
+        size_of_[p] = cls;
+        return p;
+    }
+    void free(void *p)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        free_[size_of_[p]].push_back(p);
+    }
+    void release_all()
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        (void)hipDeviceSynchronize();
+        for (auto &kv : free_)
+            for (void *p : kv.second) {
+                (void)hipFree(p);
+                size_of_.erase(p);
+            }
+        free_.clear();
+    }
+
+private:
+    static size_t size_class(size_t b)
+    {
+        if (b < 256) return 256;
+        // 1/8-octave classes: at most 12.5% rounding waste
+        size_t top = 1;
+        while ((top << 1) <= b) top <<= 1;
+        const size_t step = std::max<size_t>(top / 8, 256);
+        return (b + step - 1) / step * step;
+    }
+    std::mutex mu_;
+    std::map<size_t, std::vector<void *>> free_;
+    std::map<void *, size_t> size_of_;
+};
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -73,12 +141,12 @@ struct DBuf {
         if (count <= n && p) return;
         release();
         if (count == 0) count = 1;
-        HIPCHK(hipMalloc(&p, count * sizeof(T)));
+        p = static_cast<T *>(DevicePool::get().alloc(count * sizeof(T)));
         n = count;
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) DevicePool::get().free(p);
         p = nullptr;
         n = 0;
     }
@@ -2051,6 +2119,11 @@ int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z)
         if (!c) throw Error("null context");
         msolve_dev(c, d_r, d_z);
     });
+}
+
+int tabi_release_memory(void)
+{
+    return guarded([&] { DevicePool::get().release_all(); });
 }
 
 int tabi_device_alloc(int64_t bytes, void **out)

@@ -21,7 +21,7 @@ EXPORTS = (
     "tabi_ctx_stats", "tabi_ctx_matvec_dev", "tabi_ctx_msolve_dev", "tabi_ctx_matvec",
     "tabi_ctx_msolve", "tabi_ctx_rhs", "tabi_ctx_energy", "tabi_ctx_gmres", "tabi_solve",
     "tabi_last_solve_times", "tabi_ctx_profile", "tabi_ctx_kernel_times", "tabi_ctx_sync",
-    "tabi_device_alloc", "tabi_device_free", "tabi_copy_to_device", "tabi_copy_to_host",
+    "tabi_device_alloc", "tabi_device_free", "tabi_release_memory", "tabi_copy_to_device", "tabi_copy_to_host",
 )
 
 KERNEL_SLOTS = ("moments", "near", "far", "reduce", "msolve", "lists", "lu_factor", "rhs_energy")
