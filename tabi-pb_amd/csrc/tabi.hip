@@ -70,8 +70,10 @@ class DevicePool {
 public:
     static DevicePool &get()
     {
-        static DevicePool p;
-        return p;
+        // never destroyed: contexts held in static storage return their blocks
+        // during exit-time destruction, after function-local statics are gone
+        static DevicePool *p = new DevicePool;
+        return *p;
     }
     void *alloc(size_t bytes)
     {
