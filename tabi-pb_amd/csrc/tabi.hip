@@ -891,25 +891,48 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
 // Panel factorisation of columns k0..k0+nb-1: partial pivoting with the
 // first-maximum rule (IDAMAX), reciprocal scaling and rank-1 updates restricted
 // to the panel (the unblocked step of DGETRF2's semantics,
-// src/dgetrf_dep.f:757-914).  One wavefront per block; reductions by shuffles.
-__global__ void __launch_bounds__(64) k_lu_panel(int k0, const int32_t *__restrict__ lorder,
-    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    const int32_t *__restrict__ piv_off, double *__restrict__ lu, int32_t *__restrict__ piv,
-    int32_t *__restrict__ info)
+// src/dgetrf_dep.f:757-914).  One workgroup of kPanelThreads per block holds
+// the panel in registers, RPT rows per thread (row k0 + t + j*kPanelThreads);
+// each column step is a block-wide arg-max, a row exchange through LDS and a
+// broadcast of the pivot row.
+constexpr int kPanelThreads = 1024;
+template <int RPT>
+__global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0,
+    const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
+    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info)
 {
-    const int l = lorder[blockIdx.x], lane = threadIdx.x;
+    __shared__ double s_best[kPanelThreads / kWave];
+    __shared__ int s_bi[kPanelThreads / kWave];
+    __shared__ double s_row[2][kLuNB];
+    __shared__ int s_prow;
+    const int l = lorder[blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
     double *A = lu + blk_off[l];
     int32_t *pv = piv + piv_off[l];
-    for (int c = 0; c < nbe; ++c) {
+    double p[RPT][kLuNB];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int row = k0 + t + j * kPanelThreads;
+#pragma unroll
+        for (int c = 0; c < kLuNB; ++c)
+            p[j][c] = (row < m && c < nbe) ? A[(size_t)(k0 + c) * m + row] : 0.0;
+    }
+    int bad = 0;
+#pragma clang loop unroll(full)
+    for (int c = 0; c < kLuNB; ++c) {
+        if (c < nbe) {  // nbe is uniform over the workgroup
         const int col = k0 + c;
-        double *Ac = A + (size_t)col * m;
         double best = -1.0;
         int bi = 0x7fffffff;
-        for (int r = col + lane; r < m; r += kWave) {
-            const double v = fabs(Ac[r]);
-            if (v > best) { best = v; bi = r; }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int row = k0 + t + j * kPanelThreads;
+            if (row >= col && row < m) {
+                const double v = fabs(p[j][c]);
+                if (v > best) { best = v; bi = row; }
+            }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -917,67 +940,146 @@ __global__ void __launch_bounds__(64) k_lu_panel(int k0, const int32_t *__restri
             const int oi = __shfl_xor(bi, off);
             if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
-        const int prow = __shfl(bi, 0);
-        if (lane == 0) pv[col] = prow;
-        if (prow != col && lane < nbe) {
-            double *cc = A + (size_t)(k0 + lane) * m;
-            const double tmp = cc[col];
-            cc[col] = cc[prow];
-            cc[prow] = tmp;
+        if (lane == 0) { s_best[w] = best; s_bi[w] = bi; }
+        __syncthreads();
+        if (w == 0) {
+            best = lane < kPanelThreads / kWave ? s_best[lane] : -1.0;
+            bi = lane < kPanelThreads / kWave ? s_bi[lane] : 0x7fffffff;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (lane == 0) { s_prow = bi; pv[col] = bi; }
         }
         __syncthreads();
-        const double pivot = Ac[col];
+        const int prow = s_prow;
+        // exchange rows col and prow: both owners publish, then take the other
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int row = k0 + t + j * kPanelThreads;
+            if (row == col || row == prow) {
+#pragma unroll
+                for (int cc = 0; cc < kLuNB; ++cc) s_row[row == col ? 0 : 1][cc] = p[j][cc];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int row = k0 + t + j * kPanelThreads;
+            if (prow != col && (row == col || row == prow)) {
+#pragma unroll
+                for (int cc = 0; cc < kLuNB; ++cc) p[j][cc] = s_row[row == col ? 1 : 0][cc];
+            }
+        }
+        // s_row[1] now holds the pivot row (old row prow) for everyone
+        const double pivot = s_row[prow == col ? 0 : 1][c];
         if (pivot == 0.0) {
-            if (lane == 0) atomicCAS(info + l, 0, col + 1);
-            continue;
+            bad = col + 1;
+        } else {
+            const double rinv = 1.0 / pivot;
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int row = k0 + t + j * kPanelThreads;
+                if (row > col && row < m) {
+                    p[j][c] *= rinv;
+                    const double lv = p[j][c];
+#pragma unroll
+                    for (int cc = c + 1; cc < kLuNB; ++cc)
+                        p[j][cc] = fma(-lv, s_row[prow == col ? 0 : 1][cc], p[j][cc]);
+                }
+            }
         }
-        const double rinv = 1.0 / pivot;
-        for (int r = col + 1 + lane; r < m; r += kWave) Ac[r] *= rinv;
         __syncthreads();
-        for (int cc = col + 1; cc < k0 + nbe; ++cc) {
-            double *Aj = A + (size_t)cc * m;
-            const double u = Aj[col];
-            for (int r = col + 1 + lane; r < m; r += kWave) Aj[r] = fma(-Ac[r], u, Aj[r]);
         }
-        __syncthreads();
     }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int row = k0 + t + j * kPanelThreads;
+        if (row < m) {
+#pragma unroll
+            for (int c = 0; c < kLuNB; ++c)
+                if (c < nbe) A[(size_t)(k0 + c) * m + row] = p[j][c];
+        }
+    }
+    if (bad && t == 0) atomicCAS(info + l, 0, bad);
 }
 
 // Apply the panel's row interchanges to the columns outside the panel and form
-// U12 = L11^-1 A12 for the columns to its right.  One thread per column; grid
-// y runs over the active blocks.
+// U12 = L11^-1 A12 for the columns to its right.  A workgroup takes 64 columns
+// of one active block: the touched rows (the panel rows and the pivot rows
+// below them) are staged in LDS with coalesced loads, the interchanges and the
+// unit-lower solve run there, and the rows are written back.
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__restrict__ lorder,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     const int32_t *__restrict__ piv_off, double *__restrict__ lu, const int32_t *__restrict__ piv)
 {
     __shared__ double L11[kLuNB][kLuNB + 1];
+    __shared__ double T[2 * kLuNB][64 + 1];
+    __shared__ int rows[2 * kLuNB];
+    __shared__ int slot_of_piv[kLuNB];
+    __shared__ int s_nr;
     const int l = lorder[blockIdx.y];
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
-    if (blockIdx.x * blockDim.x >= m) return;
+    const int j0 = blockIdx.x * 64;
+    if (j0 >= m) return;
     double *A = lu + blk_off[l];
     const int32_t *pv = piv + piv_off[l];
-    for (int e = threadIdx.x; e < nbe * nbe; e += blockDim.x) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int nr = nbe;
+        for (int c = 0; c < nbe; ++c) rows[c] = k0 + c;
+        for (int c = 0; c < nbe; ++c) {
+            const int pr = pv[k0 + c];
+            int sl = pr - k0;
+            if (pr >= k0 + nbe) {
+                sl = -1;
+                for (int q = nbe; q < nr; ++q)
+                    if (rows[q] == pr) sl = q;
+                if (sl < 0) { rows[nr] = pr; sl = nr++; }
+            }
+            slot_of_piv[c] = sl;
+        }
+        s_nr = nr;
+    }
+    for (int e = t; e < nbe * nbe; e += 256) {
         const int r = e % nbe, c = e / nbe;
         L11[r][c] = A[(size_t)(k0 + c) * m + k0 + r];
     }
     __syncthreads();
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m || (j >= k0 && j < k0 + nbe)) return;
-    double *Aj = A + (size_t)j * m;
-    for (int c = 0; c < nbe; ++c) {
-        const int pr = pv[k0 + c];
-        if (pr != k0 + c) {
-            const double tmp = Aj[k0 + c];
-            Aj[k0 + c] = Aj[pr];
-            Aj[pr] = tmp;
+    const int nr = s_nr;
+    for (int e = t; e < nr * 64; e += 256) {
+        const int jj = e / nr, q = e % nr, j = j0 + jj;
+        const bool col_ok = j < m && (j < k0 || j >= k0 + nbe);
+        T[q][jj] = col_ok ? A[(size_t)j * m + rows[q]] : 0.0;
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int j = j0 + t;
+        if (j < m && (j < k0 || j >= k0 + nbe)) {
+            for (int c = 0; c < nbe; ++c) {
+                const int sl = slot_of_piv[c];
+                if (sl != c) {
+                    const double tmp = T[c][t];
+                    T[c][t] = T[sl][t];
+                    T[sl][t] = tmp;
+                }
+            }
+            if (j >= k0 + nbe) {
+                for (int c = 1; c < nbe; ++c) {
+                    double sacc = T[c][t];
+                    for (int c2 = 0; c2 < c; ++c2) sacc = fma(-L11[c][c2], T[c2][t], sacc);
+                    T[c][t] = sacc;
+                }
+            }
         }
     }
-    if (j < k0) return;
-    for (int c = 1; c < nbe; ++c) {
-        double s = Aj[k0 + c];
-        for (int c2 = 0; c2 < c; ++c2) s -= L11[c][c2] * Aj[k0 + c2];
-        Aj[k0 + c] = s;
+    __syncthreads();
+    for (int e = t; e < nr * 64; e += 256) {
+        const int jj = e / nr, q = e % nr, j = j0 + jj;
+        if (j < m && (j < k0 || j >= k0 + nbe)) A[(size_t)j * m + rows[q]] = T[q][jj];
     }
 }
 
@@ -1636,9 +1738,13 @@ void build_precond(tabi_ctx *c)
     int step = 0;
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB, ++step) {
         const int na = nact[step];
-        k_lu_panel<<<na, kWave, 0, c->s>>>(k0, dlorder.p, c->blk_m.p, c->blk_off.p, c->piv_off.p,
-                                          c->lu.p, c->piv.p, c->lu_info.p);
-        k_lu_swap_trsm<<<dim3(nblk(bm[lorder[0]], 256), na), 256, 0, c->s>>>(k0, dlorder.p,
+        if (bm[lorder[0]] - k0 <= kPanelThreads)
+            k_lu_panel<1><<<na, kPanelThreads, 0, c->s>>>(k0, dlorder.p, c->blk_m.p, c->blk_off.p,
+                c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+        else
+            k_lu_panel<2><<<na, kPanelThreads, 0, c->s>>>(k0, dlorder.p, c->blk_m.p, c->blk_off.p,
+                c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+        k_lu_swap_trsm<<<dim3(nblk(bm[lorder[0]], 64), na), 256, 0, c->s>>>(k0, dlorder.p,
             c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
         if (nt > 0)
