@@ -895,9 +895,8 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
 // the panel in registers, RPT rows per thread (row k0 + t + j*kPanelThreads);
 // each column step is a block-wide arg-max, a row exchange through LDS and a
 // broadcast of the pivot row.
-constexpr int kPanelThreads = 1024;
-template <int RPT>
-__global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0,
+template <int RPT, int kPanelThreads>
+__global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
     double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info)
@@ -906,7 +905,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0,
     __shared__ int s_bi[kPanelThreads / kWave];
     __shared__ double s_row[2][kLuNB];
     __shared__ int s_prow;
-    const int l = lorder[blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int l = lorder[first + blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
     double *A = lu + blk_off[l];
@@ -1738,12 +1737,30 @@ void build_precond(tabi_ctx *c)
     int step = 0;
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB, ++step) {
         const int na = nact[step];
-        if (bm[lorder[0]] - k0 <= kPanelThreads)
-            k_lu_panel<1><<<na, kPanelThreads, 0, c->s>>>(k0, dlorder.p, c->blk_m.p, c->blk_off.p,
-                c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
-        else
-            k_lu_panel<2><<<na, kPanelThreads, 0, c->s>>>(k0, dlorder.p, c->blk_m.p, c->blk_off.p,
-                c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+        // one launch per size class of the remaining rows (lorder is sorted, so
+        // each class is a contiguous range of active blocks)
+        int q = 0;
+        while (q < na) {
+            const int rem = bm[lorder[q]] - k0;
+            int cls = rem > 1024 ? 2048 : rem > 512 ? 1024 : rem > 256 ? 512 : 256;
+            int e = q;
+            const int lo = cls == 256 ? 0 : cls / 2;
+            while (e < na && bm[lorder[e]] - k0 > lo) ++e;
+            const int cnt = e - q;
+            if (cls == 2048)
+                k_lu_panel<2, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+            else if (cls == 1024)
+                k_lu_panel<1, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+            else if (cls == 512)
+                k_lu_panel<1, 512><<<cnt, 512, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+            else
+                k_lu_panel<1, 256><<<cnt, 256, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+            q = e;
+        }
         k_lu_swap_trsm<<<dim3(nblk(bm[lorder[0]], 64), na), 256, 0, c->s>>>(k0, dlorder.p,
             c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
