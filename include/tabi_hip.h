@@ -29,12 +29,14 @@ typedef struct {
     double eps;
     double kappa;
     double theta;
-    int32_t order;       /* 1..8 */
+    int32_t order;       /* 1..6 */
     int32_t maxparnode;  /* >= 1 */
     int32_t precision;   /* 0: FP64 throughout; 1: mixed (FP32 near/far field with
                             compensated per-target sums, FP64 elsewhere) */
     int32_t flags;       /* bit 0: preconditioner with matvec-consistent L2/L3
-                            placement instead of the reference's (tc.f:1481-1488) */
+                            placement instead of the reference's (tc.f:1481-1488);
+                            bit 1: build the tree with the sequential host
+                            partition instead of the GPU one (same result) */
 } tabi_params;
 
 typedef struct tabi_ctx tabi_ctx;

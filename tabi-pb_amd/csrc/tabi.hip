@@ -159,6 +159,15 @@ struct DBuf {
     }
 };
 
+}  // namespace
+
+namespace tabi {
+void *device_alloc(size_t bytes) { return DevicePool::get().alloc(bytes); }
+void device_free(void *p) { DevicePool::get().free(p); }
+}  // namespace tabi
+
+namespace {
+
 struct NearTask {
     int32_t g, s0, s1, leaf;  // target group, source range [s0,s1), leaf node
     u64 mask;
@@ -1027,21 +1036,33 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
     double *A = lu + blk_off[l];
     const int32_t *pv = piv + piv_off[l];
     const int t = threadIdx.x;
-    if (t == 0) {
-        int nr = nbe;
-        for (int c = 0; c < nbe; ++c) rows[c] = k0 + c;
-        for (int c = 0; c < nbe; ++c) {
-            const int pr = pv[k0 + c];
+    // touched rows: the panel rows, then each distinct pivot row below the
+    // panel in order of first use (built by 32 threads in parallel)
+    if (t < kLuNB) {
+        if (t < nbe) rows[t] = k0 + t;
+        int pr = -1, isnew = 0;
+        if (t < nbe) {
+            pr = pv[k0 + t];
+            isnew = pr >= k0 + nbe;
+            for (int c2 = 0; c2 < t && isnew; ++c2)
+                if (pv[k0 + c2] == pr) isnew = 0;
+        }
+        // rank among new extras = number of new extras before t
+        const u64 newmask = __ballot(isnew) & ((1ull << kLuNB) - 1);
+        const int rank = __popcll(newmask & ((1ull << t) - 1));
+        if (isnew) rows[nbe + rank] = pr;
+        if (t == 0) s_nr = nbe + __popcll(newmask);
+        if (t < nbe) {
             int sl = pr - k0;
             if (pr >= k0 + nbe) {
-                sl = -1;
-                for (int q = nbe; q < nr; ++q)
-                    if (rows[q] == pr) sl = q;
-                if (sl < 0) { rows[nr] = pr; sl = nr++; }
+                // slot of the first occurrence of pr
+                int first = t;
+                for (int c2 = 0; c2 < t; ++c2)
+                    if (pv[k0 + c2] == pr) { first = c2; break; }
+                sl = nbe + __popcll(newmask & ((1ull << first) - 1));
             }
-            slot_of_piv[c] = sl;
+            slot_of_piv[t] = sl;
         }
-        s_nr = nr;
     }
     for (int e = t; e < nbe * nbe; e += 256) {
         const int r = e % nbe, c = e / nbe;
@@ -1780,6 +1801,34 @@ void build_precond(tabi_ctx *c)
                                  std::to_string(info[l]) + ")");
 }
 
+// Centroids from interleaved {x,y,z} to three arrays.
+__global__ void k_deinterleave(int n, const double *__restrict__ xyz, double *__restrict__ x,
+                               double *__restrict__ y, double *__restrict__ z)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2];
+}
+
+// Normals and areas into tree order, and the source records {x,y,z,nx,ny,nz,
+// 0,0} (the weights are written per matvec).
+__global__ void k_gather_geom(int n, const int32_t *__restrict__ order,
+                              const double *__restrict__ nrm, const double *__restrict__ area,
+                              const double *__restrict__ tx, const double *__restrict__ ty,
+                              const double *__restrict__ tz, double *__restrict__ tnx,
+                              double *__restrict__ tny, double *__restrict__ tnz,
+                              double *__restrict__ tarea, double *__restrict__ src8)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = order[i];
+    const double nx = nrm[3 * o], ny = nrm[3 * o + 1], nz = nrm[3 * o + 2];
+    tnx[i] = nx; tny[i] = ny; tnz[i] = nz; tarea[i] = area[o];
+    double4 *r = reinterpret_cast<double4 *>(src8 + (size_t)8 * i);
+    r[0] = make_double4(tx[i], ty[i], tz[i], nx);
+    r[1] = make_double4(ny, nz, 0.0, 0.0);
+}
+
 void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz,
                const double *nrm, const double *area, int32_t *orderarr_out, double *t_phase)
 {
@@ -1796,27 +1845,39 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     c->ngroups = (n + kWave - 1) / kWave;
     if (!c->s) HIPCHK(hipStreamCreate(&c->s));
     auto t0 = std::chrono::steady_clock::now();
-    std::vector<double> x(n), y(n), z(n);
-    for (int i = 0; i < n; ++i) { x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2]; }
-    tabi::build_tree_host(n, x.data(), y.data(), z.data(), prm->maxparnode, c->tree);
+    DBuf<double> d_xyz, d_nrm, d_area;
+    DBuf<int32_t> d_ord;
+    d_xyz.upload(xyz, (size_t)3 * n, c->s);
+    d_nrm.upload(nrm, (size_t)3 * n, c->s);
+    d_area.upload(area, n, c->s);
+    c->tx.alloc(n); c->ty.alloc(n); c->tz.alloc(n);
+    d_ord.alloc(n);
+    if (prm->flags & 2) {
+        std::vector<double> x(n), y(n), z(n);
+        for (int i = 0; i < n; ++i) { x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2]; }
+        tabi::build_tree_host(n, x.data(), y.data(), z.data(), prm->maxparnode, c->tree);
+        c->tx.upload(x.data(), n, c->s); c->ty.upload(y.data(), n, c->s); c->tz.upload(z.data(), n, c->s);
+        d_ord.upload(c->tree.order.data(), n, c->s);
+    } else {
+        DBuf<double> dx, dy, dz;
+        dx.alloc(n); dy.alloc(n); dz.alloc(n);
+        k_deinterleave<<<nblk(n, 256), 256, 0, c->s>>>(n, d_xyz.p, dx.p, dy.p, dz.p);
+        HIPCHK(hipGetLastError());
+        tabi::build_tree_gpu(n, dx.p, dy.p, dz.p, prm->maxparnode, c->s, c->tree, c->tx.p,
+                             c->ty.p, c->tz.p, d_ord.p);
+    }
     const HostTree &T = c->tree;
     if (orderarr_out)
         for (int i = 0; i < n; ++i) orderarr_out[i] = T.order[i] + 1;
-    std::vector<double> nx(n), ny(n), nz(n), ar(n), s8((size_t)n * 8);
-    for (int i = 0; i < n; ++i) {
-        const int o = T.order[i];
-        nx[i] = nrm[3 * o]; ny[i] = nrm[3 * o + 1]; nz[i] = nrm[3 * o + 2]; ar[i] = area[o];
-        double *r = &s8[(size_t)i * 8];
-        r[0] = x[i]; r[1] = y[i]; r[2] = z[i]; r[3] = nx[i]; r[4] = ny[i]; r[5] = nz[i];
-        r[6] = 0.0; r[7] = 0.0;  // source weights, set per matvec
-    }
+    c->tnx.alloc(n); c->tny.alloc(n); c->tnz.alloc(n); c->tarea.alloc(n);
+    c->src8.alloc((size_t)n * 8);
+    k_gather_geom<<<nblk(n, 256), 256, 0, c->s>>>(n, d_ord.p, d_nrm.p, d_area.p, c->tx.p, c->ty.p,
+                                                  c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
+                                                  c->tarea.p, c->src8.p);
+    HIPCHK(hipGetLastError());
     const int nn = T.nnode();
     std::vector<double4> geo(nn);
     for (int k = 0; k < nn; ++k) geo[k] = make_double4(T.xmid[k], T.ymid[k], T.zmid[k], T.radius[k]);
-    c->tx.upload(x.data(), n, c->s); c->ty.upload(y.data(), n, c->s); c->tz.upload(z.data(), n, c->s);
-    c->tnx.upload(nx.data(), n, c->s); c->tny.upload(ny.data(), n, c->s); c->tnz.upload(nz.data(), n, c->s);
-    c->tarea.upload(ar.data(), n, c->s);
-    c->src8.upload(s8.data(), (size_t)n * 8, c->s);
     c->ngeo.upload(geo.data(), nn, c->s);
     c->nnum.upload(T.numpar.data(), nn, c->s);
     c->nchild.upload(T.nchild.data(), nn, c->s);

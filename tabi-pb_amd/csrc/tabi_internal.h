@@ -25,4 +25,16 @@ struct HostTree {
 void build_tree_host(int32_t n, double *x, double *y, double *z, int32_t maxparnode,
                      HostTree &t);
 
+// The same tree built on the GPU (level by level, data-parallel partition).
+// d_x, d_y, d_z: centroids in original order on the device; d_tx, d_ty, d_tz
+// receive them in tree order and d_order the 0-based permutation.  t receives
+// the node table and order exactly as build_tree_host produces them.
+void build_tree_gpu(int32_t n, const double *d_x, const double *d_y, const double *d_z,
+                    int32_t maxparnode, void *stream, HostTree &t, double *d_tx,
+                    double *d_ty, double *d_tz, int32_t *d_order);
+
+// Device memory from the library's caching allocator.
+void *device_alloc(size_t bytes);
+void device_free(void *p);
+
 }  // namespace tabi

@@ -197,6 +197,55 @@ def test_matvec_linearity_and_gmres_residual_at_larger_size():
     assert np.linalg.norm(res) / np.linalg.norm(S.msolve(rhs)) <= 1e-4 * (1 + 1e-6)
 
 
+def _tree_cases():
+    """Centroid sets for the GPU tree build: a symmetric icosphere (many exactly
+    tied coordinates at every split value), a synthetic molecular surface, and
+    tiny inputs (one face, fewer faces than a leaf, one face over a leaf)."""
+    from tabipb import meshio, surface
+    out = []
+    v, nv, f = meshio.icosphere(6, 2.0)
+    xyz, nrm, area = driver.face_geometry(v, nv, f)
+    out += [("ico6_leaf500", xyz, nrm, area, 500), ("ico6_leaf7", xyz, nrm, area, 7)]
+    mx, _, mr = surface.synth_molecule(150, seed=11)
+    v, nv, f = surface.ses_mesh(mx, mr, probe=1.4, h=0.5)
+    f, _ = driver.cull_faces(v, f)
+    xyz, nrm, area = driver.face_geometry(v, nv, f)
+    out += [("mol150_leaf100", xyz, nrm, area, 100), ("mol150_leaf1", xyz, nrm, area, 1)]
+    rng = np.random.default_rng(5)
+    for n, leaf in ((1, 10), (9, 10), (11, 10), (3, 1)):
+        p = rng.integers(-3, 4, size=(n, 3)).astype(np.float64) + 0.5 * rng.random((n, 3))
+        nr = rng.standard_normal((n, 3))
+        nr /= np.linalg.norm(nr, axis=1, keepdims=True)
+        out.append((f"rand{n}_leaf{leaf}", p, nr, np.ones(n), leaf))
+    return out
+
+
+def test_gpu_tree_build_matches_host_build_and_oracle(oracle_lib):
+    """The data-parallel GPU partition against the sequential restatement of
+    the reference's hole walk (flags bit 1, same library) and the oracle's tree:
+    identical permutation, node count and leaves, and bit-identical matvecs
+    (identical trees give identical lists and summation order)."""
+    O = oracle_lib
+    for name, xyz, nrm, area, leaf in _tree_cases():
+        print(f"[gpu] tree {name} n={len(xyz)} leaf={leaf}", flush=True)
+        prm_g = hip.make_params(80.0, 0.1, 0.8, 3, leaf)
+        prm_h = hip.make_params(80.0, 0.1, 0.8, 3, leaf, 0, 2)
+        Sg = hip.Surface(prm_g, xyz, nrm, area)
+        Sh = hip.Surface(prm_h, xyz, nrm, area)
+        np.testing.assert_array_equal(Sg.orderarr, Sh.orderarr)
+        _, order, nodes = O.build_tree(xyz, leaf)
+        np.testing.assert_array_equal(Sg.orderarr - 1, order)
+        sg, sh = Sg.stats(), Sh.stats()
+        assert sg == sh
+        assert sg["nnode"] == len(nodes)
+        if len(xyz) > 1:
+            x = np.random.default_rng(1).standard_normal(2 * Sg.n)
+            assert np.array_equal(Sg.matvec(x), Sh.matvec(x))
+            assert np.array_equal(Sg.msolve(x), Sh.msolve(x))
+        Sg.close()
+        Sh.close()
+
+
 # ---------------------------------------------------------------------------
 # mixed precision (precision=1): FP32 near/far field with compensated sums,
 # FP64 GMRES, leaf LU, self term and energy.  Tolerances: matvec rel. L2 <= 1e-5
