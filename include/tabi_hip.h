@@ -143,9 +143,57 @@ int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const doubl
                int32_t *iter, double *err, int32_t *ierr, double *soleng,
                double *couleng);
 
-/* Per-phase device times (ms) of the last tabi_solve on this thread:
- * {tree, lists, precond_factor, rhs, gmres, energy, total}. */
-int tabi_last_solve_times(double *out7);
+/* Per-phase device times (ms) of the last tabi_solve / tabi_solve_ws on this
+ * thread: {tree, lists, precond_factor, rhs, gmres, energy, total, warmstart}. */
+int tabi_last_solve_times(double *out8);
+
+/* ---- learned initial guess (no counterpart in the reference) -------------
+ * A per-face MLP, 12 features -> 3 x 128 GELU -> 2 outputs, BF16 weights and
+ * activations with FP32 accumulation, predicts the tree-order surface potential
+ * and its normal derivative; the prediction replaces the zero initial guess
+ * that the reference passes to DGMRES (xvct = 0, src/main.f90:115).  GMRES
+ * still iterates to the same tolerance, and falls back to the zero start when
+ * the guess's preconditioned residual exceeds that of x0 = 0.
+ *
+ * Packed parameters (tabi_ws_param_count() floats): feature means[12], feature
+ * scales[12], output scales[2], 6 unused, W1[128][12], b1[128], W2[128][128],
+ * b2[128], W3[128][128], b3[128], W4[2][128], b4[2] (row-major, out x in).
+ * Features per face (unnormalised, as tabi_ctx_ws_features returns them):
+ * s*b_i, s*b_{n+i} (right-hand side scaled by s = 1/rms(b_1..b_n)), unit
+ * normal (3), area, curvature (leaf-local normal divergence), distances to
+ * the five nearest charges (ascending).  Targets for training are the
+ * tree-order solution scaled by s: (s*phi_i, s*dphi_i/dn). */
+typedef struct tabi_ws tabi_ws;
+int64_t tabi_ws_param_count(void);
+int tabi_ws_create(const float *params, int64_t count, tabi_ws **out);
+void tabi_ws_destroy(tabi_ws *m);
+int tabi_ws_get_params(tabi_ws *m, float *out, int64_t count);
+/* features feat_out[n][12] (tree order) of a context's surface for nq charges;
+ * scale_out receives s */
+int tabi_ctx_ws_features(tabi_ctx *c, int32_t nq, const double *qpos, const double *q,
+                         double eps_p, float *feat_out, double *scale_out);
+/* x0_out[2n] = predicted tree-order guess (phi then dphi/dn) from feat[n][12] */
+int tabi_ws_predict(tabi_ws *m, int64_t n, const float *feat, double scale, double *x0_out);
+/* gradient of the loss (1/2ns) sum |out - target/out_scale|^2 over ns samples
+ * (packed layout; the first 32 entries are zero) and the loss */
+int tabi_ws_grad(tabi_ws *m, int64_t ns, const float *feat, const float *target,
+                 float *grad_out, double *loss_out);
+/* Adam training on the GPU: epochs over ns samples in shuffled mini-batches
+ * (batch <= 131072), cosine-decayed learning rate from lr to lr/20;
+ * epoch_loss[epochs] (may be NULL) receives the mean loss of each epoch */
+int tabi_ws_train(tabi_ws *m, int64_t ns, const float *feat, const float *target,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss);
+/* GMRES (as tabi_ctx_gmres) from the model's guess for charges at qpos; x gets
+ * the solution, ws_used 1 if the guess was kept, 0 if the zero start was used */
+int tabi_ctx_gmres_ws(tabi_ctx *c, tabi_ws *model, int32_t nq, const double *qpos,
+                      const double *b, double *x, double tol, int32_t maxl, int32_t nrmax,
+                      int32_t *iter, double *err, int32_t *ierr, int32_t *ws_used);
+/* tabi_solve with a learned initial guess (model may be NULL: zero start) */
+int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+                  const double *area, int32_t nq, const double *qpos, const double *q,
+                  double eps_p, double tol, tabi_ws *model, double *x_out,
+                  int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
+                  double *soleng, double *couleng, int32_t *ws_used);
 
 #ifdef __cplusplus
 }

@@ -49,7 +49,7 @@ constexpr int kStack = 512;      // per-wave traversal stack (entries)
 constexpr int kLuNB = 32;        // LU panel width
 
 thread_local std::string g_err;
-thread_local double g_times[7] = {0, 0, 0, 0, 0, 0, 0};
+thread_local double g_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
 struct Error : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -177,6 +177,10 @@ struct FarTask {
 };
 
 }  // namespace
+
+struct tabi_ws {
+    tabi::WsModel *m = nullptr;
+};
 
 struct tabi_ctx {
     tabi_params prm{};
@@ -1922,7 +1926,8 @@ void ctx_stats(tabi_ctx *c, int64_t *o)
 // and restarts from the Krylov basis (DRLCAL, 1948-1977), least squares update
 // (DHELS, 925-947).  Vectors stay on the device; only scalars come back.
 void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl, int nrmax,
-               int *iter_out, double *err_out, int *ierr_out)
+               int *iter_out, double *err_out, int *ierr_out, bool fallback = false,
+               int *x0_used = nullptr)
 {
     const int64_t N = 2 * (int64_t)c->n;
     if (maxl > N) maxl = (int)N;
@@ -1948,12 +1953,14 @@ void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl,
     HIPCHK(hipStreamSynchronize(c->s));
     double sabs = 0.0;
     for (int64_t i = 0; i < N; ++i) sabs += std::fabs(hx[i]);
-    if (sabs < 1.0e-10) {
+    const bool x0_zero = sabs < 1.0e-10;
+    if (x0_zero) {
         HIPCHK(hipMemcpyAsync(r.p, dxs, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
     } else {
         matvec_guarded(dxs, r.p);
     }
     k_sub<<<nb, 256, 0, c->s>>>(N, db, r.p, r.p);
+    if (x0_used) *x0_used = x0_zero ? 0 : 1;
     int iter = 0, ierr = 0, nrsts = 0;
     double err = 0.0;
     for (;;) {
@@ -1966,7 +1973,15 @@ void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl,
             msolve_dev(c, wk.p, r.p);
         }
         HIPCHK(hipMemcpyAsync(vcol(0), r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
-        const double r0nrm = nrm2(vcol(0));
+        double r0nrm = nrm2(vcol(0));
+        if (nrsts == 0 && fallback && !x0_zero && !(r0nrm <= bnrm)) {
+            // the initial guess is worse than none: restart from x0 = 0
+            HIPCHK(hipMemsetAsync(dxs, 0, sizeof(double) * N, c->s));
+            msolve_dev(c, db, r.p);
+            HIPCHK(hipMemcpyAsync(vcol(0), r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
+            r0nrm = bnrm;
+            if (x0_used) *x0_used = 0;
+        }
         err = r0nrm / bnrm;
         bool done_early = err <= tol;
         if (!done_early) {
@@ -2095,6 +2110,28 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
         c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, db);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
+}
+
+// Warm-start features of this surface for the charges (rhs db already formed);
+// returns the charge scale s = 1/rms(b_1..b_n).
+double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *db, float *dfeat)
+{
+    const double bb = dev_dot(c, c->n, db, db);
+    const double scale = bb > 0.0 ? 1.0 / std::sqrt(bb / c->n) : 1.0;
+    DBuf<double> dqp;
+    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
+    tabi::WsFeatIn in;
+    in.n = c->n;
+    in.tx = c->tx.p; in.ty = c->ty.p; in.tz = c->tz.p;
+    in.tnx = c->tnx.p; in.tny = c->tny.p; in.tnz = c->tnz.p; in.tarea = c->tarea.p;
+    in.leaf_of = c->leaf_of.p; in.leaf_beg = c->blk_ibeg.p; in.leaf_m2 = c->blk_m.p;
+    in.nq = nq;
+    in.qpos = dqp.p;
+    in.b = db;
+    in.scale = scale;
+    tabi::ws_features(in, dfeat, c->s);
+    HIPCHK(hipStreamSynchronize(c->s));
+    return scale;
 }
 
 void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
@@ -2400,7 +2437,18 @@ int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const doubl
                double eps_p, double tol, double *x_out, int32_t *orderarr_out, int32_t *iter,
                double *err, int32_t *ierr, double *soleng, double *couleng)
 {
+    return tabi_solve_ws(prm, n, xyz, nrm, area, nq, qpos, q, eps_p, tol, nullptr, x_out,
+                         orderarr_out, iter, err, ierr, soleng, couleng, nullptr);
+}
+
+int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+                  const double *area, int32_t nq, const double *qpos, const double *q,
+                  double eps_p, double tol, tabi_ws *model, double *x_out,
+                  int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
+                  double *soleng, double *couleng, int32_t *ws_used)
+{
     return guarded([&] {
+        if (model && !model->m) throw Error("null warm-start model");
         auto t0 = std::chrono::steady_clock::now();
         tabi_ctx c;
         double ph[3] = {0, 0, 0};
@@ -2413,8 +2461,16 @@ int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const doubl
         rhs_dev(&c, nq, qpos, q, eps_p, db.p);
         HIPCHK(hipMemsetAsync(dxs.p, 0, sizeof(double) * N, c.s));
         auto t2 = std::chrono::steady_clock::now();
-        int it = 0, ie = 0;
-        gmres_dev(&c, db.p, dxs.p, tol, 10, 10, &it, err, &ie);
+        if (model) {
+            DBuf<float> feat;
+            feat.alloc((size_t)n * tabi::ws::kF);
+            const double scale = ws_features_dev(&c, nq, qpos, db.p, feat.p);
+            tabi::ws_predict_dev(model->m, n, feat.p, scale, dxs.p, c.s);
+            HIPCHK(hipStreamSynchronize(c.s));
+        }
+        auto t2b = std::chrono::steady_clock::now();
+        int it = 0, ie = 0, used = 0;
+        gmres_dev(&c, db.p, dxs.p, tol, 10, 10, &it, err, &ie, model != nullptr, &used);
         auto t3 = std::chrono::steady_clock::now();
         energy_dev(&c, nq, qpos, q, eps_p, dxs.p, soleng, couleng);
         auto t4 = std::chrono::steady_clock::now();
@@ -2424,13 +2480,133 @@ int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const doubl
         }
         *iter = it;
         *ierr = ie;
+        if (ws_used) *ws_used = used;
         auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
         g_times[0] = ph[0]; g_times[1] = ph[1]; g_times[2] = ph[2];
-        g_times[3] = ms(t1, t2); g_times[4] = ms(t2, t3); g_times[5] = ms(t3, t4);
-        g_times[6] = ms(t0, t4);
+        g_times[3] = ms(t1, t2); g_times[4] = ms(t2b, t3); g_times[5] = ms(t3, t4);
+        g_times[6] = ms(t0, t4); g_times[7] = ms(t2, t2b);
         hipStream_t s = c.s;
         c.s = nullptr;
         (void)hipStreamDestroy(s);
+    });
+}
+
+int64_t tabi_ws_param_count(void) { return tabi::ws::kNP; }
+
+int tabi_ws_create(const float *params, int64_t count, tabi_ws **out)
+{
+    return guarded([&] {
+        if (!out) throw Error("null output handle");
+        *out = nullptr;
+        if (!params) throw Error("null parameters");
+        if (count != tabi::ws::kNP)
+            throw Error("warm-start model needs " + std::to_string(tabi::ws::kNP) + " parameters, got " +
+                        std::to_string(count));
+        for (int k = 0; k < tabi::ws::kF; ++k)
+            if (!(params[tabi::ws::oSig + k] > 0.0f)) throw Error("feature scales must be positive");
+        for (int k = 0; k < tabi::ws::kO; ++k)
+            if (!(params[tabi::ws::oTs + k] > 0.0f)) throw Error("output scales must be positive");
+        auto *h = new tabi_ws;
+        try {
+            h->m = tabi::ws_create(params);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void tabi_ws_destroy(tabi_ws *m)
+{
+    if (!m) return;
+    tabi::ws_destroy(m->m);
+    delete m;
+}
+
+int tabi_ws_get_params(tabi_ws *m, float *out, int64_t count)
+{
+    return guarded([&] {
+        if (!m || !m->m) throw Error("null warm-start model");
+        if (count != tabi::ws::kNP) throw Error("parameter count mismatch");
+        tabi::ws_get_params(m->m, out);
+    });
+}
+
+int tabi_ctx_ws_features(tabi_ctx *c, int32_t nq, const double *qpos, const double *q,
+                         double eps_p, float *feat_out, double *scale_out)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (nq < 1) throw Error("need at least one charge");
+        DBuf<double> db;
+        DBuf<float> feat;
+        db.alloc((size_t)2 * c->n);
+        feat.alloc((size_t)c->n * tabi::ws::kF);
+        rhs_dev(c, nq, qpos, q, eps_p, db.p);
+        const double scale = ws_features_dev(c, nq, qpos, db.p, feat.p);
+        HIPCHK(hipMemcpyAsync(feat_out, feat.p, sizeof(float) * c->n * tabi::ws::kF,
+                              hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        if (scale_out) *scale_out = scale;
+    });
+}
+
+int tabi_ws_predict(tabi_ws *m, int64_t n, const float *feat, double scale, double *x0_out)
+{
+    return guarded([&] {
+        if (!m || !m->m) throw Error("null warm-start model");
+        if (n < 0) throw Error("negative face count");
+        if (!(scale > 0.0)) throw Error("scale must be positive");
+        tabi::ws_predict(m->m, n, feat, scale, x0_out);
+    });
+}
+
+int tabi_ws_grad(tabi_ws *m, int64_t ns, const float *feat, const float *target,
+                 float *grad_out, double *loss_out)
+{
+    return guarded([&] {
+        if (!m || !m->m) throw Error("null warm-start model");
+        if (ns < 1) throw Error("need at least one sample");
+        tabi::ws_grad(m->m, ns, feat, target, grad_out, loss_out);
+    });
+}
+
+int tabi_ws_train(tabi_ws *m, int64_t ns, const float *feat, const float *target,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss)
+{
+    return guarded([&] {
+        if (!m || !m->m) throw Error("null warm-start model");
+        if (batch < 1 || batch > 4096 * 32) throw Error("batch must be 1..131072");
+        tabi::ws_train(m->m, ns, feat, target, epochs, batch, lr, seed, epoch_loss);
+    });
+}
+
+int tabi_ctx_gmres_ws(tabi_ctx *c, tabi_ws *model, int32_t nq, const double *qpos,
+                      const double *b, double *x, double tol,
+                      int32_t maxl, int32_t nrmax, int32_t *iter, double *err, int32_t *ierr,
+                      int32_t *ws_used)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (!model || !model->m) throw Error("null warm-start model");
+        const int64_t N = 2 * (int64_t)c->n;
+        DBuf<double> db, dxs;
+        DBuf<float> feat;
+        db.upload(b, N, c->s);
+        dxs.alloc(N);
+        feat.alloc((size_t)c->n * tabi::ws::kF);
+        if (nq < 1) throw Error("need at least one charge");
+        const double scale = ws_features_dev(c, nq, qpos, db.p, feat.p);
+        tabi::ws_predict_dev(model->m, c->n, feat.p, scale, dxs.p, c->s);
+        HIPCHK(hipStreamSynchronize(c->s));
+        int it = 0, ie = 0, used = 0;
+        gmres_dev(c, db.p, dxs.p, tol, maxl, nrmax, &it, err, &ie, true, &used);
+        HIPCHK(hipMemcpyAsync(x, dxs.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        *iter = it;
+        *ierr = ie;
+        if (ws_used) *ws_used = used;
     });
 }
 
@@ -2455,9 +2631,9 @@ int tabi_ctx_kernel_times(tabi_ctx *c, double *out_ms, int64_t *out_launches)
     });
 }
 
-int tabi_last_solve_times(double *out7)
+int tabi_last_solve_times(double *out8)
 {
-    for (int i = 0; i < 7; ++i) out7[i] = g_times[i];
+    for (int i = 0; i < 8; ++i) out8[i] = g_times[i];
     return 0;
 }
 

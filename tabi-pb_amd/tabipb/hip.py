@@ -22,7 +22,12 @@ EXPORTS = (
     "tabi_ctx_msolve", "tabi_ctx_rhs", "tabi_ctx_energy", "tabi_ctx_gmres", "tabi_solve",
     "tabi_last_solve_times", "tabi_ctx_profile", "tabi_ctx_kernel_times", "tabi_ctx_sync",
     "tabi_device_alloc", "tabi_device_free", "tabi_release_memory", "tabi_copy_to_device", "tabi_copy_to_host",
+    "tabi_ws_param_count", "tabi_ws_create", "tabi_ws_destroy", "tabi_ws_get_params",
+    "tabi_ctx_ws_features", "tabi_ws_predict", "tabi_ws_grad", "tabi_ws_train",
+    "tabi_ctx_gmres_ws", "tabi_solve_ws",
 )
+
+SOLVE_PHASES = ("tree", "lists", "precond", "rhs", "gmres", "energy", "total", "warmstart")
 
 KERNEL_SLOTS = ("moments", "near", "far", "reduce", "msolve", "lists", "lu_factor", "rhs_energy")
 
@@ -52,6 +57,8 @@ def load():
     lib.tabi_last_error.restype = C.c_char_p
     lib.tabi_finalize.restype = None
     lib.tabi_ctx_destroy.restype = None
+    lib.tabi_ws_destroy.restype = None
+    lib.tabi_ws_param_count.restype = C.c_int64
     _LIB = lib
     return lib
 
@@ -241,6 +248,99 @@ class Surface:
                                      C.c_int32(nrmax), C.byref(it), C.byref(err), C.byref(ierr)))
         return x, it.value, err.value, ierr.value
 
+    def ws_features(self, qpos, q, eps_p):
+        """Warm-start features [n, 12] (tree order) for these charges and the
+        charge scale s (tabi_ctx_ws_features)."""
+        qpos = _d(qpos, (-1, 3))
+        q = _d(q)
+        feat = np.zeros((self.n, 12), dtype=np.float32)
+        sc = C.c_double()
+        _check(load().tabi_ctx_ws_features(self._h, C.c_int32(q.shape[0]), _p(qpos), _p(q),
+                                           C.c_double(eps_p), _p(feat), C.byref(sc)))
+        return feat, sc.value
+
+    def gmres_ws(self, model: "WsModel", qpos, b, tol=1e-4, maxl=10, nrmax=10):
+        """GMRES from the model's predicted guess (tabi_ctx_gmres_ws); returns
+        (x, iter, err, ierr, used) where used is 0 if the zero start was taken."""
+        qpos = _d(qpos, (-1, 3))
+        b = _d(b)
+        self._len(b)
+        x = np.zeros_like(b)
+        it, err, ierr, used = C.c_int32(), C.c_double(), C.c_int32(), C.c_int32()
+        _check(load().tabi_ctx_gmres_ws(self._h, model._h, C.c_int32(qpos.shape[0]), _p(qpos),
+                                        _p(b), _p(x), C.c_double(tol), C.c_int32(maxl),
+                                        C.c_int32(nrmax), C.byref(it), C.byref(err),
+                                        C.byref(ierr), C.byref(used)))
+        return x, it.value, err.value, ierr.value, used.value
+
+
+class WsModel:
+    """The learned initial guess on the GPU (tabi_ws_*): packed FP32 master
+    parameters, BF16 images for the MFMA kernels, Adam state."""
+
+    def __init__(self, params):
+        lib = load()
+        params = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        h = C.c_void_p()
+        _check(lib.tabi_ws_create(_p(params), C.c_int64(params.size), C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def param_count() -> int:
+        return int(load().tabi_ws_param_count())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tabi_ws_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.param_count(), dtype=np.float32)
+        _check(load().tabi_ws_get_params(self._h, _p(out), C.c_int64(out.size)))
+        return out
+
+    @staticmethod
+    def _feat(feat):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        if feat.ndim != 2 or feat.shape[1] != 12:
+            raise TabiError("features must have shape (n, 12)")
+        return feat
+
+    def predict(self, feat, scale):
+        feat = self._feat(feat)
+        x0 = np.zeros(2 * feat.shape[0])
+        _check(load().tabi_ws_predict(self._h, C.c_int64(feat.shape[0]), _p(feat),
+                                      C.c_double(scale), _p(x0)))
+        return x0
+
+    def grad(self, feat, target):
+        feat = self._feat(feat)
+        target = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, 2)
+        if target.shape[0] != feat.shape[0]:
+            raise TabiError("features and targets disagree in length")
+        g = np.zeros(self.param_count(), dtype=np.float32)
+        loss = C.c_double()
+        _check(load().tabi_ws_grad(self._h, C.c_int64(feat.shape[0]), _p(feat), _p(target),
+                                   _p(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, feat, target, epochs, batch=8192, lr=1e-3, seed=0):
+        feat = self._feat(feat)
+        target = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, 2)
+        if target.shape[0] != feat.shape[0]:
+            raise TabiError("features and targets disagree in length")
+        losses = np.zeros(int(epochs), dtype=np.float32)
+        _check(load().tabi_ws_train(self._h, C.c_int64(feat.shape[0]), _p(feat), _p(target),
+                                    C.c_int32(epochs), C.c_int32(batch), C.c_float(lr),
+                                    C.c_uint64(seed), _p(losses)))
+        return losses
+
 
 # ---- default-context drop-in API (what a Fortran or C driver binds) --------
 def dropin_init(prm: Params, xyz, nrm, area):
@@ -274,9 +374,10 @@ def dropin_finalize():
     load().tabi_finalize()
 
 
-def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4):
-    """One full PB solve on the GPU (tabi_solve).  Returns a dict with the
-    tree-order solution, permutation, GMRES counters, energies and phase times."""
+def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4, model: WsModel = None):
+    """One full PB solve on the GPU (tabi_solve, or tabi_solve_ws with a learned
+    initial guess).  Returns a dict with the tree-order solution, permutation,
+    GMRES counters, energies and phase times."""
     lib = load()
     xyz = _d(xyz, (-1, 3))
     nrm = _d(nrm, (-1, 3))
@@ -291,13 +392,14 @@ def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4):
     ierr = C.c_int32()
     sol = C.c_double()
     cou = C.c_double()
-    _check(lib.tabi_solve(C.byref(prm), C.c_int32(n), _p(xyz), _p(nrm), _p(area),
-                          C.c_int32(q.shape[0]), _p(qpos), _p(q), C.c_double(eps_p),
-                          C.c_double(tol), _p(x), _p(order), C.byref(it), C.byref(err),
-                          C.byref(ierr), C.byref(sol), C.byref(cou)))
-    t = np.zeros(7)
+    used = C.c_int32()
+    _check(lib.tabi_solve_ws(C.byref(prm), C.c_int32(n), _p(xyz), _p(nrm), _p(area),
+                             C.c_int32(q.shape[0]), _p(qpos), _p(q), C.c_double(eps_p),
+                             C.c_double(tol), model._h if model is not None else None, _p(x),
+                             _p(order), C.byref(it), C.byref(err), C.byref(ierr),
+                             C.byref(sol), C.byref(cou), C.byref(used)))
+    t = np.zeros(8)
     lib.tabi_last_solve_times(_p(t))
     return dict(x=x, orderarr=order, iter=it.value, err=err.value, ierr=ierr.value,
-                soleng=sol.value, couleng=cou.value,
-                times_ms=dict(zip(("tree", "lists", "precond", "rhs", "gmres", "energy",
-                                   "total"), t)))
+                soleng=sol.value, couleng=cou.value, ws_used=used.value,
+                times_ms=dict(zip(SOLVE_PHASES, t)))
