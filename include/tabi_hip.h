@@ -158,10 +158,13 @@ int tabi_last_solve_times(double *out8);
  * Packed parameters (tabi_ws_param_count() floats): feature means[12], feature
  * scales[12], output scales[2], 6 unused, W1[128][12], b1[128], W2[128][128],
  * b2[128], W3[128][128], b3[128], W4[2][128], b4[2] (row-major, out x in).
- * Features per face (unnormalised, as tabi_ctx_ws_features returns them):
- * s*b_i, s*b_{n+i} (right-hand side scaled by s = 1/rms(b_1..b_n)), unit
- * normal (3), area, curvature (leaf-local normal divergence), distances to
- * the five nearest charges (ascending).  Targets for training are the
+ * Features per face (unnormalised, as tabi_ctx_ws_features returns them),
+ * with s = 1/rms(b_1..b_n): s*b_i and s*b_{n+i} (the right-hand side: Coulomb
+ * potential and normal field of all charges); the same potential and normal
+ * field restricted by Gaussian windows of 5 A and 10 A around the face (4);
+ * area; curvature (leaf-local normal divergence); distances to the nearest
+ * and second-nearest charge; normal . direction from the nearest charge;
+ * a smooth count of charges within 10 A.  Targets for training are the
  * tree-order solution scaled by s: (s*phi_i, s*dphi_i/dn). */
 typedef struct tabi_ws tabi_ws;
 int64_t tabi_ws_param_count(void);
@@ -183,10 +186,11 @@ int tabi_ws_grad(tabi_ws *m, int64_t ns, const float *feat, const float *target,
  * epoch_loss[epochs] (may be NULL) receives the mean loss of each epoch */
 int tabi_ws_train(tabi_ws *m, int64_t ns, const float *feat, const float *target,
                   int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss);
-/* GMRES (as tabi_ctx_gmres) from the model's guess for charges at qpos; x gets
+/* GMRES (as tabi_ctx_gmres) from the model's guess for the charges; x gets
  * the solution, ws_used 1 if the guess was kept, 0 if the zero start was used */
 int tabi_ctx_gmres_ws(tabi_ctx *c, tabi_ws *model, int32_t nq, const double *qpos,
-                      const double *b, double *x, double tol, int32_t maxl, int32_t nrmax,
+                      const double *q, double eps_p, const double *b, double *x, double tol,
+                      int32_t maxl, int32_t nrmax,
                       int32_t *iter, double *err, int32_t *ierr, int32_t *ws_used);
 /* tabi_solve with a learned initial guess (model may be NULL: zero start) */
 int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,

@@ -6,8 +6,10 @@ initial guess xvct = 0 before DGMRES (src/main.f90:115).  This module restates,
 in numpy, what the GPU kernels compute:
 
 * ``features``: the 12 per-face features (right-hand side scaled by
-  s = 1/rms(b_1..b_n), unit normal, area, leaf-local curvature, distances to the
-  five nearest charges), from tree-order geometry and leaf ranges.
+  s = 1/rms(b_1..b_n), the potential and normal field of the charges within
+  Gaussian windows of 5 and 10 A, area, leaf-local curvature, the two nearest
+  charge distances, the normal's cosine with the nearest-charge direction and a
+  smooth burial count), from tree-order geometry and leaf ranges.
 * ``forward`` / ``loss_grad``: the MLP 12 -> 3 x 128 GELU -> 2 and the gradient of
   L = (1/2B) sum |out - t/ts|^2, either in FP64 throughout (``emulate=False``,
   the mathematical reference pinned by finite differences in the tests) or with
@@ -138,19 +140,20 @@ def loss_only(P, feat, target):
     return float((e ** 2).sum() / (2.0 * out.shape[0]))
 
 
-def features(txyz, tnrm, tarea, leaf_ranges, qpos, b, h=1.0):
+def features(txyz, tnrm, tarea, leaf_ranges, qpos, q, eps_p, b, h=1.0, R1=5.0, R2=10.0):
     """The 12 features per face (tree order).  leaf_ranges: (ibeg, iend)
-    inclusive per leaf; b the tree-order right-hand side [2n]."""
+    inclusive per leaf; b the tree-order right-hand side [2n].  Charge sums use
+    float32 coordinates and charges, as the kernel does, accumulated in FP64."""
     txyz = np.asarray(txyz, dtype=np.float64)
     tnrm = np.asarray(tnrm, dtype=np.float64)
     n = txyz.shape[0]
     b = np.asarray(b, dtype=np.float64)
     s = 1.0 / np.sqrt(np.mean(b[:n] ** 2))
+    su = s * 0.25 / (np.pi * eps_p)
     f = np.zeros((n, 12))
     f[:, 0] = s * b[:n]
     f[:, 1] = s * b[n:]
-    f[:, 2:5] = tnrm
-    f[:, 5] = tarea
+    f[:, 6] = tarea
     # curvature: Gaussian-weighted mean of (n_j-n_i).(c_j-c_i)/|c_j-c_i|^2 in the leaf
     for ib, ie in leaf_ranges:
         c = txyz[ib:ie + 1]
@@ -162,13 +165,33 @@ def features(txyz, tnrm, tarea, leaf_ranges, qpos, b, h=1.0):
         w = np.where(ok, np.exp(-r2 / (h * h)), 0.0)
         val = np.where(ok, (dn * dc).sum(-1) / np.where(ok, r2, 1.0), 0.0)
         den = w.sum(1)
-        f[ib:ie + 1, 6] = np.where(den > 0, (w * val).sum(1) / np.where(den > 0, den, 1.0), 0.0)
-    # five nearest charges (brute force, float32 coordinates as the kernel uses)
+        f[ib:ie + 1, 7] = np.where(den > 0, (w * val).sum(1) / np.where(den > 0, den, 1.0), 0.0)
     q32 = np.asarray(qpos, dtype=np.float32).astype(np.float64)
+    qq = np.asarray(q, dtype=np.float32).astype(np.float64)
     c32 = txyz.astype(np.float32).astype(np.float64)
-    k = min(5, q32.shape[0])
-    for i0 in range(0, n, 4096):
-        d2 = ((c32[i0:i0 + 4096, None, :] - q32[None, :, :]) ** 2).sum(-1)
-        part = np.sort(d2, axis=1)[:, :k]
-        f[i0:i0 + 4096, 7:7 + k] = np.sqrt(part)
+    n32 = tnrm.astype(np.float32).astype(np.float64)
+    for i0 in range(0, n, 2048):
+        c = c32[i0:i0 + 2048]
+        nv = n32[i0:i0 + 2048]
+        d = q32[None, :, :] - c[:, None, :]
+        r2 = (d ** 2).sum(-1)
+        srt = np.argsort(r2, axis=1, kind="stable")
+        r1 = np.sqrt(np.take_along_axis(r2, srt[:, :1], 1))[:, 0]
+        rr2 = np.sqrt(np.take_along_axis(r2, srt[:, 1:2], 1))[:, 0] if r2.shape[1] > 1 else r1
+        a1 = q32[srt[:, 0]]
+        ok = (r2 > 0) & (r2 < 9.0 * R2 * R2)
+        rs = np.sqrt(np.where(ok, r2, 1.0))
+        pot = np.where(ok, qq[None, :] / rs, 0.0)
+        fld = pot * (nv[:, None, :] * d).sum(-1) / (rs * rs)
+        w1 = np.where(ok, np.exp(-r2 / (R1 * R1)), 0.0)
+        w2 = np.where(ok, np.exp(-r2 / (R2 * R2)), 0.0)
+        sl = slice(i0, i0 + 2048)
+        f[sl, 2] = su * (pot * w1).sum(1)
+        f[sl, 3] = su * (fld * w1).sum(1)
+        f[sl, 4] = su * (pot * w2).sum(1)
+        f[sl, 5] = su * (fld * w2).sum(1)
+        f[sl, 8] = r1
+        f[sl, 9] = rr2
+        f[sl, 10] = np.where(r1 > 0, (nv * (c - a1)).sum(1) / np.where(r1 > 0, r1, 1.0), 0.0)
+        f[sl, 11] = w2.sum(1)
     return f, s

@@ -2114,12 +2114,14 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
 
 // Warm-start features of this surface for the charges (rhs db already formed);
 // returns the charge scale s = 1/rms(b_1..b_n).
-double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *db, float *dfeat)
+double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
+                       const double *db, float *dfeat)
 {
     const double bb = dev_dot(c, c->n, db, db);
     const double scale = bb > 0.0 ? 1.0 / std::sqrt(bb / c->n) : 1.0;
-    DBuf<double> dqp;
+    DBuf<double> dqp, dq;
     dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
+    dq.upload(q, std::max(nq, 1), c->s);
     tabi::WsFeatIn in;
     in.n = c->n;
     in.tx = c->tx.p; in.ty = c->ty.p; in.tz = c->tz.p;
@@ -2127,6 +2129,8 @@ double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *db
     in.leaf_of = c->leaf_of.p; in.leaf_beg = c->blk_ibeg.p; in.leaf_m2 = c->blk_m.p;
     in.nq = nq;
     in.qpos = dqp.p;
+    in.q = dq.p;
+    in.eps_p = eps_p;
     in.b = db;
     in.scale = scale;
     tabi::ws_features(in, dfeat, c->s);
@@ -2464,7 +2468,7 @@ int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const do
         if (model) {
             DBuf<float> feat;
             feat.alloc((size_t)n * tabi::ws::kF);
-            const double scale = ws_features_dev(&c, nq, qpos, db.p, feat.p);
+            const double scale = ws_features_dev(&c, nq, qpos, q, eps_p, db.p, feat.p);
             tabi::ws_predict_dev(model->m, n, feat.p, scale, dxs.p, c.s);
             HIPCHK(hipStreamSynchronize(c.s));
         }
@@ -2544,7 +2548,7 @@ int tabi_ctx_ws_features(tabi_ctx *c, int32_t nq, const double *qpos, const doub
         db.alloc((size_t)2 * c->n);
         feat.alloc((size_t)c->n * tabi::ws::kF);
         rhs_dev(c, nq, qpos, q, eps_p, db.p);
-        const double scale = ws_features_dev(c, nq, qpos, db.p, feat.p);
+        const double scale = ws_features_dev(c, nq, qpos, q, eps_p, db.p, feat.p);
         HIPCHK(hipMemcpyAsync(feat_out, feat.p, sizeof(float) * c->n * tabi::ws::kF,
                               hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));
@@ -2583,7 +2587,7 @@ int tabi_ws_train(tabi_ws *m, int64_t ns, const float *feat, const float *target
 }
 
 int tabi_ctx_gmres_ws(tabi_ctx *c, tabi_ws *model, int32_t nq, const double *qpos,
-                      const double *b, double *x, double tol,
+                      const double *q, double eps_p, const double *b, double *x, double tol,
                       int32_t maxl, int32_t nrmax, int32_t *iter, double *err, int32_t *ierr,
                       int32_t *ws_used)
 {
@@ -2597,7 +2601,7 @@ int tabi_ctx_gmres_ws(tabi_ctx *c, tabi_ws *model, int32_t nq, const double *qpo
         dxs.alloc(N);
         feat.alloc((size_t)c->n * tabi::ws::kF);
         if (nq < 1) throw Error("need at least one charge");
-        const double scale = ws_features_dev(c, nq, qpos, db.p, feat.p);
+        const double scale = ws_features_dev(c, nq, qpos, q, eps_p, db.p, feat.p);
         tabi::ws_predict_dev(model->m, c->n, feat.p, scale, dxs.p, c->s);
         HIPCHK(hipStreamSynchronize(c->s));
         int it = 0, ie = 0, used = 0;

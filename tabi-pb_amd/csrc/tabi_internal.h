@@ -50,6 +50,8 @@ struct WsFeatIn {
                                                       // and 2*size of each leaf
     int nq = 0;
     const double *qpos;                               // [3nq]
+    const double *q;                                  // [nq]
+    double eps_p = 1.0;
     const double *b;                                  // right-hand side [2n]
     double scale = 1.0;                               // 1/rms(b[0..n-1])
 };

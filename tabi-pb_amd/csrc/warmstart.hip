@@ -114,16 +114,25 @@ __device__ __forceinline__ float gelu_d(float z)
 // ---------------------------------------------------------------------------
 // features
 // ---------------------------------------------------------------------------
-// Per face i (tree order), written unnormalised to feat[i][0..11]:
-//   0  s * b_i           Coulomb potential of the charges at the centroid
-//   1  s * b_{n+i}       its normal derivative (the right-hand side, both
-//                        scaled by s = 1/rms(b_1..b_n), so the features and
-//                        the scaled solution do not depend on the charge scale)
-//   2-4 unit normal      5 area
-//   6  curvature         Gaussian-weighted mean of (n_j-n_i).(c_j-c_i)/|c_j-c_i|^2
-//                        over faces j of the same leaf (exactly 1/R on a sphere)
-//   7-11 distances from the centroid to the five nearest atoms, ascending
+// Per face i (tree order), written unnormalised to feat[i][0..11], with
+// s = 1/rms(b_1..b_n) (so features and the scaled solution do not depend on
+// the overall charge scale), u = 1/(4 pi eps_p), r the centroid-charge
+// distance, cos the cosine between the normal and the centroid-to-charge
+// direction, and Gaussian windows w1 = exp(-r^2/R1^2), w2 = exp(-r^2/R2^2):
+//   0  s * b_i                 Coulomb potential of all charges (right-hand side)
+//   1  s * b_{n+i}             its normal derivative (right-hand side)
+//   2  s*u*sum q w1/r          potential of the charges near the face (R1 = 5 A)
+//   3  s*u*sum q w1 cos/r^2    their normal field
+//   4  s*u*sum q w2/r          the same within R2 = 10 A
+//   5  s*u*sum q w2 cos/r^2
+//   6  area                    7  curvature: Gaussian-weighted mean of
+//                                 (n_j-n_i).(c_j-c_i)/|c_j-c_i|^2 over faces j
+//                                 of the same leaf (exactly 1/R on a sphere)
+//   8-9 distances to the nearest and second-nearest charge
+//   10 normal . (centroid - nearest charge)/distance
+//   11 sum w2 (how buried the face is: a smooth count of charges within R2)
 constexpr float kCurvH = 1.0f;      // curvature kernel width (Angstrom)
+constexpr float kR1 = 5.0f, kR2 = 10.0f;
 constexpr int kAtomTile = 256;
 
 __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict__ tx,
@@ -131,15 +140,19 @@ __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict
     const double *__restrict__ tny, const double *__restrict__ tnz,
     const double *__restrict__ tarea, const int32_t *__restrict__ leaf_of,
     const int32_t *__restrict__ leaf_beg, const int32_t *__restrict__ leaf_m2, int nq,
-    const double *__restrict__ qpos, const double *__restrict__ b, double s,
-    float *__restrict__ feat)
+    const double *__restrict__ qpos, const double *__restrict__ qch, double eps_p,
+    const double *__restrict__ b, double s, float *__restrict__ feat)
 {
-    __shared__ float sx[kAtomTile], sy[kAtomTile], sz[kAtomTile];
+    __shared__ float sx[kAtomTile], sy[kAtomTile], sz[kAtomTile], sq[kAtomTile];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     const int ii = valid ? i : n - 1;
     const float cx = (float)tx[ii], cy = (float)ty[ii], cz = (float)tz[ii];
-    float d[5] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};  // squared, ascending
+    const float nx = (float)tnx[ii], ny = (float)tny[ii], nz = (float)tnz[ii];
+    const float i1 = 1.0f / (kR1 * kR1), i2 = 1.0f / (kR2 * kR2);
+    float d1 = INFINITY, d2 = INFINITY;          // squared distances, ascending
+    float ax = 0.f, ay = 0.f, az = 0.f;          // nearest charge
+    float p1 = 0.f, f1 = 0.f, p2 = 0.f, f2 = 0.f, bur = 0.f;
     for (int a0 = 0; a0 < nq; a0 += kAtomTile) {
         __syncthreads();
         const int a = a0 + threadIdx.x;
@@ -147,23 +160,35 @@ __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict
             sx[threadIdx.x] = (float)qpos[3 * a];
             sy[threadIdx.x] = (float)qpos[3 * a + 1];
             sz[threadIdx.x] = (float)qpos[3 * a + 2];
+            sq[threadIdx.x] = (float)qch[a];
         }
         __syncthreads();
         const int na = min(kAtomTile, nq - a0);
         for (int k = 0; k < na; ++k) {
             const float dx = sx[k] - cx, dy = sy[k] - cy, dz = sz[k] - cz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < d[4]) {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    if (r2 < d[q]) { const float t = d[q]; d[q] = r2; r2 = t; }
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < d2) {
+                if (r2 < d1) {
+                    d2 = d1; d1 = r2;
+                    ax = sx[k]; ay = sy[k]; az = sz[k];
+                } else {
+                    d2 = r2;
                 }
+            }
+            if (r2 < 9.0f * kR2 * kR2 && r2 > 0.f) {
+                const float ir = 1.0f / sqrtf(r2);
+                const float w1 = __expf(-r2 * i1), w2 = __expf(-r2 * i2);
+                const float pot = sq[k] * ir;
+                const float fld = pot * (nx * dx + ny * dy + nz * dz) * ir * ir;
+                p1 += pot * w1; f1 += fld * w1;
+                p2 += pot * w2; f2 += fld * w2;
+                bur += w2;
             }
         }
     }
     if (!valid) return;
-    const double nx = tnx[i], ny = tny[i], nz = tnz[i];
     // curvature over the leaf
+    const double dnx = tnx[i], dny = tny[i], dnz = tnz[i];
     const int l = leaf_of[i];
     const int j0 = leaf_beg[l], j1 = j0 + leaf_m2[l] / 2;
     const double xi = tx[i], yi = ty[i], zi = tz[i];
@@ -174,17 +199,24 @@ __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict
         const double r2 = dx * dx + dy * dy + dz * dz;
         if (j == i || r2 <= 0.0 || r2 * ih2 > 9.0) continue;
         const double w = exp(-r2 * ih2);
-        num += w * ((tnx[j] - nx) * dx + (tny[j] - ny) * dy + (tnz[j] - nz) * dz) / r2;
+        num += w * ((tnx[j] - dnx) * dx + (tny[j] - dny) * dy + (tnz[j] - dnz) * dz) / r2;
         den += w;
     }
+    const double su = s * 0.25 / (3.141592653589793 * eps_p);
     float *f = feat + (size_t)12 * i;
     f[0] = (float)(s * b[i]);
     f[1] = (float)(s * b[n + i]);
-    f[2] = (float)nx; f[3] = (float)ny; f[4] = (float)nz;
-    f[5] = (float)tarea[i];
-    f[6] = den > 0.0 ? (float)(num / den) : 0.0f;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) f[7 + q] = d[q] < INFINITY ? sqrtf(d[q]) : 0.0f;
+    f[2] = (float)(su * p1);
+    f[3] = (float)(su * f1);
+    f[4] = (float)(su * p2);
+    f[5] = (float)(su * f2);
+    f[6] = (float)tarea[i];
+    f[7] = den > 0.0 ? (float)(num / den) : 0.0f;
+    const float r1 = d1 < INFINITY ? sqrtf(d1) : 0.0f;
+    f[8] = r1;
+    f[9] = d2 < INFINITY ? sqrtf(d2) : r1;
+    f[10] = r1 > 0.f ? (nx * (cx - ax) + ny * (cy - ay) + nz * (cz - az)) / r1 : 0.0f;
+    f[11] = bur;
 }
 
 // ---------------------------------------------------------------------------
@@ -607,7 +639,8 @@ void ws_features(const WsFeatIn &in, float *d_feat, void *stream)
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (in.n <= 0) return;
     k_ws_feat<<<nb(in.n, 256), 256, 0, s>>>(in.n, in.tx, in.ty, in.tz, in.tnx, in.tny, in.tnz,
-        in.tarea, in.leaf_of, in.leaf_beg, in.leaf_m2, in.nq, in.qpos, in.b, in.scale, d_feat);
+        in.tarea, in.leaf_of, in.leaf_beg, in.leaf_m2, in.nq, in.qpos, in.q, in.eps_p, in.b,
+        in.scale, d_feat);
     WCHK(hipGetLastError());
 }
 

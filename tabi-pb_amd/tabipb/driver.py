@@ -119,11 +119,11 @@ def load_run_dir(path: str) -> Problem:
     return load_problem(base + ".pqr", base, usr)
 
 
-def solve(prob: Problem, tol=1e-4, precision=0, flags=0, **over):
-    """Full GPU solve; returns the dict of hip.solve plus the original-order
-    solution."""
+def solve(prob: Problem, tol=1e-4, precision=0, flags=0, model=None, **over):
+    """Full GPU solve (from a zero start, or from the learned guess of a
+    hip.WsModel); returns the dict of hip.solve plus the original-order solution."""
     res = hip.solve(prob.params(precision, flags, **over), prob.xyz, prob.nrm, prob.area,
-                    prob.qpos, prob.q, prob.usr.epsp, tol)
+                    prob.qpos, prob.q, prob.usr.epsp, tol, model=model)
     perm = res["orderarr"].astype(np.int64) - 1
     n = prob.xyz.shape[0]
     xo = np.empty(2 * n)

@@ -259,15 +259,17 @@ class Surface:
                                            C.c_double(eps_p), _p(feat), C.byref(sc)))
         return feat, sc.value
 
-    def gmres_ws(self, model: "WsModel", qpos, b, tol=1e-4, maxl=10, nrmax=10):
+    def gmres_ws(self, model: "WsModel", qpos, q, eps_p, b, tol=1e-4, maxl=10, nrmax=10):
         """GMRES from the model's predicted guess (tabi_ctx_gmres_ws); returns
         (x, iter, err, ierr, used) where used is 0 if the zero start was taken."""
         qpos = _d(qpos, (-1, 3))
+        q = _d(q)
         b = _d(b)
         self._len(b)
         x = np.zeros_like(b)
         it, err, ierr, used = C.c_int32(), C.c_double(), C.c_int32(), C.c_int32()
         _check(load().tabi_ctx_gmres_ws(self._h, model._h, C.c_int32(qpos.shape[0]), _p(qpos),
+                                        _p(q), C.c_double(eps_p),
                                         _p(b), _p(x), C.c_double(tol), C.c_int32(maxl),
                                         C.c_int32(nrmax), C.byref(it), C.byref(err),
                                         C.byref(ierr), C.byref(used)))

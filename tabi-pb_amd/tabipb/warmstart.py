@@ -29,8 +29,8 @@ import numpy as np
 from . import driver, hip, meshio, surface
 
 NF, NH, NO = 12, 128, 2
-FEATURES = ("s*b_phi", "s*b_dphi", "nx", "ny", "nz", "area", "curvature",
-            "d_atom1", "d_atom2", "d_atom3", "d_atom4", "d_atom5")
+FEATURES = ("s*b_phi", "s*b_dphi", "s*phi_5A", "s*dphi_5A", "s*phi_10A", "s*dphi_10A",
+            "area", "curvature", "d_atom1", "d_atom2", "normal.dir_atom1", "burial_10A")
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(_HERE, "data", "warmstart_mlp.npz")
 

@@ -7,7 +7,8 @@ the zero start when the guess is worse than none.
 Tolerances: predictions within 1e-2 relative L2 of the emulating oracle (BF16
 rounding of near-equal FP32 values can flip a last bit), gradients within 2e-2
 relative L2 per parameter block, the loss within 1e-3; features within 1e-5
-relative (distances in FP32); the solvation energy of the warm-started solve
+relative (FP32 sums), the windowed sums within 1e-4 of their largest value;
+the solvation energy of the warm-started solve
 within 1e-3 of the FP64 solve.
 """
 import numpy as np
@@ -100,10 +101,12 @@ def test_features_match_oracle():
     assert np.array_equal(order, perm)
     ib, ie = O.leaves(nodes)
     fo, so = W.features(m["xyz"][perm], m["nrm"][perm], m["area"][perm], list(zip(ib, ie)),
-                        m["qpos"], b)
+                        m["qpos"], m["q"], WS.EPS_P, b)
     assert s == pytest.approx(so, rel=1e-12)
     for k in range(12):
-        np.testing.assert_allclose(feat[:, k], fo[:, k], rtol=1e-5, atol=1e-6 * np.abs(fo[:, k]).max())
+        tol = 1e-4 if k in (2, 3, 4, 5, 11) else 1e-5
+        np.testing.assert_allclose(feat[:, k], fo[:, k], rtol=tol,
+                                   atol=tol * np.abs(fo[:, k]).max(), err_msg=str(k))
     S.close()
 
 
@@ -146,7 +149,7 @@ def test_learned_start_cuts_iterations_and_falls_back():
     # the surface-level entry point agrees with the full solve
     S = hip.Surface(WS.params_for(0), m["xyz"], m["nrm"], m["area"])
     b = S.rhs(m["qpos"], m["q"], WS.EPS_P)
-    x, it, err, ierr, used = S.gmres_ws(M, m["qpos"], b, tol=1e-4)
+    x, it, err, ierr, used = S.gmres_ws(M, m["qpos"], m["q"], WS.EPS_P, b, tol=1e-4)
     assert it == r1["iter"] and used == 1 and ierr == 0
     S.close()
     M.close()
@@ -166,3 +169,19 @@ def test_model_api_errors():
         M.predict(np.zeros((4, 11)), 1.0)
     np.testing.assert_array_equal(M.params(), P)
     M.close()
+
+
+def test_committed_model_gains_on_held_out_molecules():
+    """The shipped model (trained on 1024 seeded molecules of 300-3000 atoms)
+    on held-out seeds: GMRES from its guess converges to the same tolerance in
+    fewer iterations, with the energy within 1e-3 of the FP64 solve."""
+    P = WS.load()
+    seeds = list(range(900000, 900004))
+    sizes = WS.molecule_sizes(seeds, salt=1)
+    rows = WS.evaluate(P, seeds, sizes, workers=1, log=lambda s: print(s, flush=True))
+    it0, it1, used = rows[:, 1], rows[:, 2], rows[:, 3]
+    err = np.abs(rows[:, 5] - rows[:, 6]) / np.abs(rows[:, 6])
+    assert np.all(used == 1)
+    assert it1.sum() < it0.sum()
+    assert np.all(it1 <= it0)
+    assert np.all(err < 1e-3)

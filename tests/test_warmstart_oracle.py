@@ -100,13 +100,21 @@ def test_curvature_feature_is_inverse_radius_on_sphere():
     edges = np.linspace(0, n, 21).astype(int)
     leaves = [(edges[i], edges[i + 1] - 1) for i in range(20)]
     b = np.ones(2 * n)
-    feat, s = W.features(xyz, nrm, area, leaves, np.zeros((1, 3)), b)
+    feat, s = W.features(xyz, nrm, area, leaves, np.zeros((1, 3)), np.ones(1), 1.0, b)
     assert s == pytest.approx(1.0)
-    k = feat[:, 6]
+    k = feat[:, 7]
     # centroids sit slightly inside the sphere and normals are vertex-normal
     # averages: the estimate is 1/R to within a percent on average
     assert np.median(k) == pytest.approx(1.0 / R, rel=1e-2)
-    assert np.all(np.abs(feat[:, 7] - np.linalg.norm(xyz, axis=1)) < 1e-5)
+    r = np.linalg.norm(xyz, axis=1)
+    assert np.all(np.abs(feat[:, 8] - r) < 1e-5)
+    # one unit charge at the centre: windowed potentials are exactly
+    # w(r)/(4 pi r), the normal field w(r)/(4 pi r^2) pointing inward (cos = -1)
+    np.testing.assert_allclose(feat[:, 2], np.exp(-r ** 2 / 25.0) / (4 * np.pi * r), rtol=1e-5)
+    np.testing.assert_allclose(feat[:, 5], -np.exp(-r ** 2 / 100.0) / (4 * np.pi * r ** 2),
+                               rtol=2e-3)
+    np.testing.assert_allclose(feat[:, 10], 1.0, atol=2e-3)
+    np.testing.assert_allclose(feat[:, 11], np.exp(-r ** 2 / 100.0), rtol=1e-5)
 
 
 def test_nearest_charge_distances_match_kdtree():
@@ -114,6 +122,8 @@ def test_nearest_charge_distances_match_kdtree():
     c = rng.uniform(-10, 10, (500, 3))
     q = rng.uniform(-10, 10, (300, 3))
     feat, _ = W.features(c, np.tile([0.0, 0.0, 1.0], (500, 1)), np.ones(500), [(0, 499)], q,
-                         np.ones(1000))
-    d, _ = cKDTree(q).query(c, k=5)
-    np.testing.assert_allclose(feat[:, 7:12], d, rtol=1e-5)
+                         np.ones(300), 1.0, np.ones(1000))
+    d, idx = cKDTree(q).query(c, k=2)
+    np.testing.assert_allclose(feat[:, 8:10], d, rtol=1e-5)
+    cosv = (c[:, 2] - q[idx[:, 0], 2]) / d[:, 0]
+    np.testing.assert_allclose(feat[:, 10], cosv, atol=1e-5)
