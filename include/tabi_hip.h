@@ -112,6 +112,27 @@ int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q
 int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
                    int32_t nrmax, int32_t *iter, double *err, int32_t *ierr);
 
+/* ---- blocks of right-hand sides on one surface ------------------------------
+ * Several vectors (charge sets, Krylov vectors) share one tree pass: the near-
+ * field kernel values and far-field Taylor expansions are formed once per pair
+ * and applied to every vector, and the leaf factors are read once per group of
+ * vectors.  Per vector the arithmetic is that of the single-vector entry points
+ * (column k of a block result equals the single result for vector k).  Vectors
+ * are stored one after another: x[k*2n .. k*2n+2n-1] is vector k, tree order.
+ * No counterpart in the reference, which solves one charge set per run. */
+int tabi_ctx_matvec_block(tabi_ctx *c, int32_t nrhs, const double *x, double *y);
+int tabi_ctx_msolve_block(tabi_ctx *c, int32_t nrhs, const double *r, double *z);
+/* nrhs runs of tabi_ctx_gmres (initial guesses in x) in lockstep; iter, err and
+ * ierr are per run */
+int tabi_ctx_gmres_block(tabi_ctx *c, int32_t nrhs, const double *b, double *x, double tol,
+                         int32_t maxl, int32_t nrmax, int32_t *iter, double *err, int32_t *ierr);
+/* Solves for nrhs charge sets at common positions qpos[3nq] with charges
+ * q[k*nq .. k*nq+nq-1] of set k: right-hand sides, block GMRES from zero
+ * (MAXL = NRMAX = 10, tol), energies; x_out[nrhs*2n] may be NULL. */
+int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qpos,
+                         const double *q, double eps_p, double tol, double *x_out, int32_t *iter,
+                         double *err, int32_t *ierr, double *soleng, double *couleng);
+
 /* The library keeps freed device blocks for reuse by later contexts and solves;
  * this returns all cached (unused) blocks to the runtime. */
 int tabi_release_memory(void);

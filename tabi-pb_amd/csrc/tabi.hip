@@ -219,6 +219,10 @@ struct tabi_ctx {
     DBuf<double> lu;
     // vectors
     DBuf<double> vx, vy, red;
+    // blocks of vectors
+    DBuf<double> wbd, mom_partb, mom_wb, blk_red;
+    DBuf<float> wbf, mom_wb32;
+    DBuf<double2> part_nearb, part_farb;
     // per-kernel timing (events on stream s)
     bool prof = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -374,7 +378,7 @@ constexpr int kMomChunk = 512;
 //   M_psi(k) = sum A_j psi_j d^k,  M_d(k) = sum (n_d A_j) phi_j d^k   (COMP_MS,
 // src/treecode3d_pb.f:859-916, with pb_kernel's source charges, src/main.f90:671-692)
 template <int P>
-__global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restrict__ chunk_node,
+__device__ __forceinline__ void mom_raw_body(int c, int n, const int32_t *__restrict__ chunk_node,
     const int32_t *__restrict__ chunk_beg, const int32_t *__restrict__ chunk_end,
     const double4 *__restrict__ ngeo, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
@@ -386,7 +390,7 @@ __global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restric
     constexpr int NPART = 256 / NK;
     __shared__ double sd[7][kMomChunk];
     __shared__ double sred[4][256];
-    const int c = blockIdx.x, t = threadIdx.x;
+    const int t = threadIdx.x;
     const int nd = chunk_node[c], b = chunk_beg[c], e = chunk_end[c];
     const double4 geo = ngeo[nd];
     const int cnt = e - b;
@@ -433,6 +437,36 @@ __global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restric
     }
 }
 
+template <int P>
+__global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restrict__ chunk_node,
+    const int32_t *__restrict__ chunk_beg, const int32_t *__restrict__ chunk_end,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, const double *__restrict__ tarea,
+    const double *__restrict__ x, double *__restrict__ part)
+{
+    mom_raw_body<P>(blockIdx.x, n, chunk_node, chunk_beg, chunk_end, ngeo, tx, ty, tz, tnx, tny,
+                    tnz, tarea, x, part);
+}
+
+// the same for column blockIdx.y of a block: vector act[col] at x + act[col]*xs,
+// partials at part + col*pstride
+template <int P>
+__global__ void __launch_bounds__(256) k_mom_rawb(int n, const int32_t *__restrict__ chunk_node,
+    const int32_t *__restrict__ chunk_beg, const int32_t *__restrict__ chunk_end,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, const double *__restrict__ tarea,
+    const double *__restrict__ x, int64_t xs, const int32_t *__restrict__ act,
+    double *__restrict__ part, int64_t pstride)
+{
+    const int col = blockIdx.y;
+    mom_raw_body<P>(blockIdx.x, n, chunk_node, chunk_beg, chunk_end, ngeo, tx, ty, tz, tnx, tny,
+                    tnz, tarea, x + (size_t)act[col] * xs, part + (size_t)col * pstride);
+}
+
 // Sum a node's chunk partials in chunk order and fold the moments with the
 // derivative factors der_cof (src/main.f90:380-399) and the kernel combination
 // of COMPP_TREE_PB (src/treecode3d_pb.f:479-485) into 8 vectors over |j| <= P+2:
@@ -440,13 +474,13 @@ __global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restric
 //   far normal der. peng2 = sum_d n0_d sum_j a0_j W[2+d][j] + ak_j W[5+d][j]
 // where a0/ak are the Taylor coefficients for kappa = 0 and kappa.
 template <int P>
-__global__ void __launch_bounds__(256) k_mom_final(const int32_t *__restrict__ mom_chunk_off,
+__device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict__ mom_chunk_off,
     const double *__restrict__ part, double eps, double *__restrict__ mom_w)
 {
     constexpr int NK = MomShape<P>::NK;
     constexpr int NJ = MomShape<P>::NJ;
     __shared__ double M[4][NK];
-    const int mi = blockIdx.x, t = threadIdx.x;
+    const int t = threadIdx.x;
     const int c0 = mom_chunk_off[mi], c1 = mom_chunk_off[mi + 1];
     for (int q = t; q < 4 * NK; q += 256) {
         double s = 0.0;
@@ -495,6 +529,25 @@ __global__ void __launch_bounds__(256) k_mom_final(const int32_t *__restrict__ m
             W[(5 + d) * NJ + j] = hc[d] / eps + he[d];
         }
     }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_mom_final(const int32_t *__restrict__ mom_chunk_off,
+    const double *__restrict__ part, double eps, double *__restrict__ mom_w)
+{
+    mom_final_body<P>(blockIdx.x, mom_chunk_off, part, eps, mom_w);
+}
+
+// block column blockIdx.y: partials at part + col*pstride, vectors at
+// mom_w + col*wstride
+template <int P>
+__global__ void __launch_bounds__(256) k_mom_finalb(const int32_t *__restrict__ mom_chunk_off,
+    const double *__restrict__ part, int64_t pstride, double eps, double *__restrict__ mom_w,
+    int64_t wstride)
+{
+    const int col = blockIdx.y;
+    mom_final_body<P>(blockIdx.x, mom_chunk_off, part + (size_t)col * pstride, eps,
+                      mom_w + (size_t)col * wstride);
 }
 
 // ---------------------------------------------------------------------------
@@ -1456,6 +1509,661 @@ __global__ void k_sub(int64_t n, const double *__restrict__ a, const double *__r
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
+// blocks of vectors (several right-hand sides, or the Krylov vectors of several
+// GMRES runs): column `col` of a block is the vector act[col] of a strided set,
+// base + act[col]*stride.  The field kernels evaluate each (target, source)
+// kernel value and each (target, cluster) Taylor expansion once and apply it to
+// RC columns held in registers.  Per column the arithmetic (operation order,
+// fused multiply-adds, compensation) is that of the single-vector kernels above.
+// ---------------------------------------------------------------------------
+// source weights wb[j][2*Rp]: {A_j x[j], A_j x[n+j]} of column r < R, zero for
+// the padding columns R <= r < Rp
+template <typename T>
+__global__ void k_src_weightsb(int n, int R, int Rp, const double *__restrict__ tarea,
+    const double *__restrict__ x, int64_t xs, const int32_t *__restrict__ act, T *__restrict__ wb)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (j >= n) return;
+    T w1 = (T)0, w2 = (T)0;
+    if (r < R) {
+        const double *xr = x + (size_t)act[r] * xs;
+        const double a = tarea[j];
+        w1 = (T)(a * xr[j]);
+        w2 = (T)(a * xr[n + j]);
+    }
+    wb[(size_t)j * 2 * Rp + 2 * r] = w1;
+    wb[(size_t)j * 2 * Rp + 2 * r + 1] = w2;
+}
+
+template <int RC>
+__global__ void __launch_bounds__(64) k_nearb(const NearTask *__restrict__ tasks, int ntask, int n,
+    int R, int Rp, const double *__restrict__ src8, const double *__restrict__ wb,
+    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double kappa, double eps, double2 *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const int r0 = blockIdx.y * RC;
+    const NearTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool act = (tk.mask >> lane) & 1ull;
+    const bool valid = i < n;
+    const double sx = valid ? tx[i] : 0.0, sy = valid ? ty[i] : 0.0, sz = valid ? tz[i] : 0.0;
+    const double v0x = valid ? tnx[i] : 0.0, v0y = valid ? tny[i] : 0.0,
+                 v0z = valid ? tnz[i] : 0.0;
+    const double kappa2 = kappa * kappa, inv_eps = 1.0 / eps;
+    double p1[RC], p2[RC];
+#pragma unroll
+    for (int k = 0; k < RC; ++k) { p1[k] = 0.0; p2[k] = 0.0; }
+    if (act) {
+        for (int j = tk.s0; j < tk.s1; ++j) {
+            const double4 ra = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8);
+            const double4 rb = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8 + 4);
+            const bool use = (j != i);
+            const double dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            r2 = use ? r2 : 1.0;
+            const PairL L = pair_l(dx, dy, dz, r2, ra.w, rb.x, rb.y, v0x, v0y, v0z, kappa,
+                                   kappa2, eps, inv_eps);
+            const double *w = wb + (size_t)j * 2 * Rp + 2 * r0;
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                const double w1 = use ? w[2 * k] : 0.0, w2 = use ? w[2 * k + 1] : 0.0;
+                p1[k] = fma(L.l1, w1, fma(L.l2, w2, p1[k]));
+                p2[k] = fma(L.l3, w1, fma(L.l4, w2, p2[k]));
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RC; ++k)
+        if (r0 + k < R) part[((size_t)task * R + r0 + k) * kWave + lane] = make_double2(p1[k], p2[k]);
+}
+
+template <int RC>
+__global__ void __launch_bounds__(64) k_near32b(const NearTask *__restrict__ tasks, int ntask,
+    int n, int R, int Rp, const float *__restrict__ src32, const float *__restrict__ wb,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, float kappa, float eps, double2 *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const int r0 = blockIdx.y * RC;
+    const NearTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool act = (tk.mask >> lane) & 1ull;
+    const bool valid = i < n;
+    const double4 lg = ngeo[tk.leaf];
+    const float sx = valid ? (float)(tx[i] - lg.x) : 0.f, sy = valid ? (float)(ty[i] - lg.y) : 0.f,
+                sz = valid ? (float)(tz[i] - lg.z) : 0.f;
+    const float v0x = valid ? (float)tnx[i] : 0.f, v0y = valid ? (float)tny[i] : 0.f,
+                v0z = valid ? (float)tnz[i] : 0.f;
+    const float kappa2 = kappa * kappa, inv_eps = 1.f / eps;
+    const float c4 = (float)kOneOver4Pi;
+    Kahan p1[RC], p2[RC];
+    if (act) {
+        for (int j = tk.s0; j < tk.s1; ++j) {
+            const float4 ra = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8);
+            const float4 rb = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8 + 4);
+            const bool use = (j != i);
+            const float dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            r2 = use ? r2 : 1.f;
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float rs = r2 * inv;
+            const float krs = kappa * rs;
+            const float ek = __expf(-krs);
+            const float g0 = c4 * inv;
+            const float gk = ek * g0;
+            const float ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
+            const float ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
+            const float tp1 = g0 * inv;
+            const float tp2 = fmaf(krs, ek, ek);
+            const float g1 = ct * tp1, g10 = ct0 * tp1;
+            const float g3 = (v0x * ra.w + v0y * rb.x + v0z * rb.y - 3.f * ct0 * ct) * inv * tp1;
+            const float l1 = g1 * fmaf(-eps, tp2, 1.f);
+            const float l2 = g0 - gk;
+            const float l3 = fmaf(g3, tp2 - 1.f, -kappa2 * ct0 * ct * gk);
+            const float l4 = g10 * fmaf(-inv_eps, tp2, 1.f);
+            const float *w = wb + (size_t)j * 2 * Rp + 2 * r0;
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                const float w1 = use ? w[2 * k] : 0.f, w2 = use ? w[2 * k + 1] : 0.f;
+                p1[k].add(fmaf(l1, w1, l2 * w2));
+                p2[k].add(fmaf(l3, w1, l4 * w2));
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RC; ++k)
+        if (r0 + k < R)
+            part[((size_t)task * R + r0 + k) * kWave + lane] =
+                make_double2((double)p1[k].s, (double)p2[k].s);
+}
+
+// moment vectors of column r: mom_w + r*wstride (wstride = nmom*8*NJ)
+template <int P, int RC>
+__global__ void __launch_bounds__(64) k_farb(const FarTask *__restrict__ tasks, int ntask, int n,
+    int R, const int32_t *__restrict__ far_node, const u64 *__restrict__ far_mask,
+    const int32_t *__restrict__ node2mom, const double *__restrict__ mom_w, int64_t wstride,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double kappa, double2 *__restrict__ part)
+{
+    constexpr int L = MomShape<P>::L;
+    constexpr int NJ = MomShape<P>::NJ;
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
+    const FarTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    double acc1[RC], ax[RC], ay[RC], az[RC];
+#pragma unroll
+    for (int k = 0; k < RC; ++k) { acc1[k] = 0.0; ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
+    for (int it = tk.i0; it < tk.i1; ++it) {
+        const int nd = far_node[it];
+        const u64 m = far_mask[it];
+        if (!((m >> lane) & 1ull)) continue;
+        const double4 geo = ngeo[nd];
+        const double *W0 = mom_w + (size_t)r0 * wstride + (size_t)node2mom[nd] * 8 * NJ;
+        const double dx = px - geo.x, dy = py - geo.y, dz = pz - geo.z;
+        {
+            double a[NJ];
+            tabi::taylor<L, false, double>(dx, dy, dz, 0.0, a);
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                if (k >= rc) break;
+                const double *W = W0 + (size_t)k * wstride;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    acc1[k] = fma(a[j], W[j], acc1[k]);
+                    ax[k] = fma(a[j], W[2 * NJ + j], ax[k]);
+                    ay[k] = fma(a[j], W[3 * NJ + j], ay[k]);
+                    az[k] = fma(a[j], W[4 * NJ + j], az[k]);
+                }
+            }
+        }
+        {
+            double a[NJ];
+            tabi::taylor<L, true, double>(dx, dy, dz, kappa, a);
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                if (k >= rc) break;
+                const double *W = W0 + (size_t)k * wstride;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    acc1[k] = fma(a[j], W[NJ + j], acc1[k]);
+                    ax[k] = fma(a[j], W[5 * NJ + j], ax[k]);
+                    ay[k] = fma(a[j], W[6 * NJ + j], ay[k]);
+                    az[k] = fma(a[j], W[7 * NJ + j], az[k]);
+                }
+            }
+        }
+    }
+    const double nx = valid ? tnx[i] : 0.0, ny = valid ? tny[i] : 0.0, nz = valid ? tnz[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+        if (k >= rc) break;
+        double p2 = 0.0;
+        if (valid) p2 = nx * ax[k] + ny * ay[k] + nz * az[k];
+        part[((size_t)task * R + r0 + k) * kWave + lane] = make_double2(acc1[k], p2);
+    }
+}
+
+template <int P, int RC>
+__global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks, int ntask, int n,
+    int R, const int32_t *__restrict__ far_node, const u64 *__restrict__ far_mask,
+    const int32_t *__restrict__ node2mom, const float *__restrict__ mom_w, int64_t wstride,
+    const double4 *__restrict__ ngeo, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, float kappa, double2 *__restrict__ part)
+{
+    constexpr int L = MomShape<P>::L;
+    constexpr int NJ = MomShape<P>::NJ;
+    const int task = blockIdx.x;
+    if (task >= ntask) return;
+    const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
+    const FarTask tk = tasks[task];
+    const int lane = lane_id();
+    const int i = tk.g * kWave + lane;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    const float nx = valid ? (float)tnx[i] : 0.f, ny = valid ? (float)tny[i] : 0.f,
+                nz = valid ? (float)tnz[i] : 0.f;
+    Kahan s1[RC], s2[RC];
+    float acc1[RC], ax[RC], ay[RC], az[RC];
+    for (int it = tk.i0; it < tk.i1; ++it) {
+        const int nd = far_node[it];
+        const u64 m = far_mask[it];
+        if (!((m >> lane) & 1ull)) continue;
+        const double4 geo = ngeo[nd];
+        const float *W0 = mom_w + (size_t)r0 * wstride + (size_t)node2mom[nd] * 8 * NJ;
+        const float dx = (float)(px - geo.x), dy = (float)(py - geo.y), dz = (float)(pz - geo.z);
+#pragma unroll
+        for (int k = 0; k < RC; ++k) { acc1[k] = 0.f; ax[k] = 0.f; ay[k] = 0.f; az[k] = 0.f; }
+        {
+            float a[NJ];
+            tabi::taylor<L, false, float>(dx, dy, dz, 0.f, a);
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                if (k >= rc) break;
+                const float *W = W0 + (size_t)k * wstride;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    acc1[k] = fmaf(a[j], W[j], acc1[k]);
+                    ax[k] = fmaf(a[j], W[2 * NJ + j], ax[k]);
+                    ay[k] = fmaf(a[j], W[3 * NJ + j], ay[k]);
+                    az[k] = fmaf(a[j], W[4 * NJ + j], az[k]);
+                }
+            }
+        }
+        {
+            float a[NJ];
+            tabi::taylor<L, true, float>(dx, dy, dz, kappa, a);
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                if (k >= rc) break;
+                const float *W = W0 + (size_t)k * wstride;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    acc1[k] = fmaf(a[j], W[NJ + j], acc1[k]);
+                    ax[k] = fmaf(a[j], W[5 * NJ + j], ax[k]);
+                    ay[k] = fmaf(a[j], W[6 * NJ + j], ay[k]);
+                    az[k] = fmaf(a[j], W[7 * NJ + j], az[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RC; ++k) {
+            if (k >= rc) break;
+            s1[k].add(acc1[k]);
+            s2[k].add(nx * ax[k] + ny * ay[k] + nz * az[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+        if (k >= rc) break;
+        part[((size_t)task * R + r0 + k) * kWave + lane] =
+            make_double2((double)s1[k].s, (double)s2[k].s);
+    }
+}
+
+// y = pre*x - sum of the group's task partials, for block column blockIdx.y
+__global__ void k_reduceb(int n, int R, const int32_t *__restrict__ near_task_off,
+    const int32_t *__restrict__ far_task_off, const double2 *__restrict__ part_near,
+    const double2 *__restrict__ part_far, const double *__restrict__ x, int64_t xs,
+    double *__restrict__ y, int64_t ys, const int32_t *__restrict__ act, double pre1, double pre2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const int g = i / kWave, lane = i % kWave;
+    double p1 = 0.0, p2 = 0.0;
+    for (int t = near_task_off[g]; t < near_task_off[g + 1]; ++t) {
+        const double2 v = part_near[((size_t)t * R + r) * kWave + lane];
+        p1 += v.x; p2 += v.y;
+    }
+    for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
+        const double2 v = part_far[((size_t)t * R + r) * kWave + lane];
+        p1 += v.x; p2 += v.y;
+    }
+    const double *xr = x + (size_t)act[r] * xs;
+    double *yr = y + (size_t)act[r] * ys;
+    yr[i] = pre1 * xr[i] - p1;
+    yr[n + i] = pre2 * xr[n + i] - p2;
+}
+
+// z = M^-1 r for RC block columns per workgroup (blockIdx.y): the factor
+// stream is read once and applied to every column; z lives in LDS [RC][m]
+template <int RC>
+__device__ __forceinline__ void gemv_rowb(const double *__restrict__ Ac, int m, int bs,
+                                          const double *zt, int ldz, double (&s)[RC])
+{
+    double s1[RC];
+#pragma unroll
+    for (int k = 0; k < RC; ++k) s1[k] = 0.0;
+#pragma unroll 1
+    for (int h = 0; h < kSolveB; h += kHalf) {
+        double a[kHalf];
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? Ac[(size_t)(h + c) * m] : 0.0;
+#pragma unroll
+        for (int k = 0; k < RC; ++k) {
+#pragma unroll
+            for (int c = 0; c < kHalf; c += 2) {
+                s[k] = fma(-a[c], (h + c < bs) ? zt[k * ldz + h + c] : 0.0, s[k]);
+                s1[k] = fma(-a[c + 1], (h + c + 1 < bs) ? zt[k * ldz + h + c + 1] : 0.0, s1[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RC; ++k) s[k] = s[k] + s1[k];
+}
+
+template <int RC>
+__global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *__restrict__ blk_ibeg,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    const int32_t *__restrict__ piv_off, const double *__restrict__ lu,
+    const int32_t *__restrict__ perm, const double *__restrict__ rv, int64_t rs_,
+    double *__restrict__ zv, int64_t zs_, const int32_t *__restrict__ act, int ldz)
+{
+    extern __shared__ double zsh[];
+    const int l = blockIdx.x, lane = threadIdx.x;
+    const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
+    const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
+    const double *A = lu + blk_off[l];
+    const int32_t *pm = perm + piv_off[l];
+    for (int k = 0; k < RC; ++k) {
+        const double *r = k < rc ? rv + (size_t)act[r0 + k] * rs_ : nullptr;
+        for (int q = lane; q < m; q += kWave) {
+            const int src = pm[q];
+            zsh[k * ldz + q] = k < rc ? (src < nl ? r[ib + src] : r[n + ib + src - nl]) : 0.0;
+        }
+    }
+    __syncthreads();
+    const int nb = (m + kSolveB - 1) / kSolveB;
+    for (int bi = 0; bi < nb; ++bi) {
+        const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
+        const bool in = lane < bs;
+        double zl[RC];
+#pragma unroll
+        for (int k = 0; k < RC; ++k) zl[k] = in ? zsh[k * ldz + kb + lane] : 0.0;
+#pragma unroll 1
+        for (int h = 0; h < kSolveB; h += kHalf) {
+            double tr[kHalf];
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c)
+                tr[c] = (in && h + c < lane) ? A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+#pragma unroll
+            for (int k = 0; k < RC; ++k)
+#pragma unroll
+                for (int c = 0; c < kHalf; ++c) zl[k] -= tr[c] * __shfl(zl[k], h + c);
+        }
+        if (in) {
+#pragma unroll
+            for (int k = 0; k < RC; ++k) zsh[k * ldz + kb + lane] = zl[k];
+        }
+        __syncthreads();
+        for (int q0 = kb + bs; q0 < m; q0 += kWave) {
+            const int row = q0 + lane;
+            if (row < m) {
+                double sv[RC];
+#pragma unroll
+                for (int k = 0; k < RC; ++k) sv[k] = zsh[k * ldz + row];
+                gemv_rowb<RC>(A + (size_t)kb * m + row, m, bs, zsh + kb, ldz, sv);
+#pragma unroll
+                for (int k = 0; k < RC; ++k) zsh[k * ldz + row] = sv[k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
+        const bool in = lane < bs;
+        double zl[RC];
+#pragma unroll
+        for (int k = 0; k < RC; ++k) zl[k] = in ? zsh[k * ldz + kb + lane] : 0.0;
+#pragma unroll 1
+        for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
+            double tr[kHalf];
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c)
+                tr[c] = (in && h + c >= lane && h + c < bs) ? A[(size_t)(kb + h + c) * m + kb + lane]
+                                                           : 0.0;
+#pragma unroll
+            for (int c = kHalf - 1; c >= 0; --c) {
+#pragma unroll
+                for (int k = 0; k < RC; ++k) {
+                    if (lane == h + c && in) zl[k] = zl[k] / tr[c];
+                    const double zc = __shfl(zl[k], h + c);
+                    if (lane < h + c) zl[k] -= tr[c] * zc;
+                }
+            }
+        }
+        if (in) {
+#pragma unroll
+            for (int k = 0; k < RC; ++k) zsh[k * ldz + kb + lane] = zl[k];
+        }
+        __syncthreads();
+        for (int q0 = 0; q0 < kb; q0 += kWave) {
+            double sv[RC];
+#pragma unroll
+            for (int k = 0; k < RC; ++k) sv[k] = zsh[k * ldz + q0 + lane];
+            gemv_rowb<RC>(A + (size_t)kb * m + q0 + lane, m, bs, zsh + kb, ldz, sv);
+#pragma unroll
+            for (int k = 0; k < RC; ++k) zsh[k * ldz + q0 + lane] = sv[k];
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < rc; ++k) {
+        double *z = zv + (size_t)act[r0 + k] * zs_;
+        for (int q = lane; q < nl; q += kWave) {
+            z[ib + q] = zsh[k * ldz + q];
+            z[n + ib + q] = zsh[k * ldz + nl + q];
+        }
+    }
+}
+
+// right-hand sides of R charge sets at the same positions (charges q[r][k]):
+// the geometric factors are formed once per (face, charge) and applied to RC
+// sets per workgroup column (blockIdx.y)
+template <int RC>
+__global__ void __launch_bounds__(256) k_rhsb(int n, int nq, int R, const double *__restrict__ qpos,
+    const double *__restrict__ q, double eps_p, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double *__restrict__ b)
+{
+    __shared__ double sq[3 + RC][256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    const double vx = valid ? tnx[i] : 0.0, vy = valid ? tny[i] : 0.0, vz = valid ? tnz[i] : 0.0;
+    double b1[RC], b2[RC];
+#pragma unroll
+    for (int k = 0; k < RC; ++k) { b1[k] = 0.0; b2[k] = 0.0; }
+    for (int k0 = 0; k0 < nq; k0 += 256) {
+        const int kk = k0 + threadIdx.x;
+        __syncthreads();
+        if (kk < nq) {
+            sq[0][threadIdx.x] = qpos[3 * kk];
+            sq[1][threadIdx.x] = qpos[3 * kk + 1];
+            sq[2][threadIdx.x] = qpos[3 * kk + 2];
+#pragma unroll
+            for (int k = 0; k < RC; ++k) sq[3 + k][threadIdx.x] = k < rc ? q[(size_t)(r0 + k) * nq + kk] : 0.0;
+        }
+        __syncthreads();
+        const int cnt = min(256, nq - k0);
+        for (int c = 0; c < cnt; ++c) {
+            const double dx = sq[0][c] - px, dy = sq[1][c] - py, dz = sq[2][c] - pz;
+            const double rs = sqrt(dx * dx + dy * dy + dz * dz);
+            const double inv = 1.0 / rs;
+            const double ct = (vx * dx + vy * dy + vz * dz) * inv;
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                b1[k] += sq[3 + k][c] * inv;
+                b2[k] += sq[3 + k][c] * ct * inv * inv;
+            }
+        }
+    }
+    if (valid) {
+        const double s = 1.0 / (4.0 * kPi * eps_p);
+#pragma unroll
+        for (int k = 0; k < RC; ++k) {
+            if (k >= rc) break;
+            double *br = b + (size_t)(r0 + k) * 2 * n;
+            br[i] = b1[k] * s;
+            br[n + i] = b2[k] * s;
+        }
+    }
+}
+
+// potential at each charge from R surface solutions (weights wb[j][2*Rp] as
+// k_src_weightsb writes them); tasks as k_energy's, RC columns per blockIdx.y
+template <int RC>
+__global__ void __launch_bounds__(64) k_energyb(int n, int nq, int nchunk, int R, int Rp,
+    const double *__restrict__ qpos, const double *__restrict__ src8,
+    const double *__restrict__ wb, double eps, double kappa, double *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    const int r0 = blockIdx.y * RC;
+    const int g = task / nchunk, ch = task % nchunk;
+    const int lane = lane_id();
+    const int k = g * kWave + lane;
+    const bool valid = k < nq;
+    const double cx = valid ? qpos[3 * k] : 0.0, cy = valid ? qpos[3 * k + 1] : 0.0,
+                 cz = valid ? qpos[3 * k + 2] : 0.0;
+    const int f0 = ch * kEnergyChunk, f1 = min(n, f0 + kEnergyChunk);
+    double s[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) s[c] = 0.0;
+    for (int j = f0; j < f1; ++j) {
+        const double4 ra = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8);
+        const double4 rb = *reinterpret_cast<const double4 *>(src8 + (size_t)j * 8 + 4);
+        const double dx = ra.x - cx, dy = ra.y - cy, dz = ra.z - cz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double inv = rsq_nr(r2);
+        const double rs = r2 * inv;
+        const double krs = kappa * rs;
+        const double ek = exp(-krs);
+        const double g0 = kOneOver4Pi * inv;
+        const double ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
+        const double g1 = ct * g0 * inv;
+        const double h1 = g1 * fma(-eps, fma(krs, ek, ek), 1.0);
+        const double h2 = g0 - ek * g0;
+        const double *w = wb + (size_t)j * 2 * Rp + 2 * r0;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) s[c] = fma(h1, w[2 * c], fma(h2, w[2 * c + 1], s[c]));
+    }
+#pragma unroll
+    for (int c = 0; c < RC; ++c)
+        if (r0 + c < R) part[((size_t)task * R + r0 + c) * kWave + lane] = s[c];
+}
+
+__global__ void k_energy_reduceb(int nq, int nchunk, int R, const double *__restrict__ part,
+                                 double *__restrict__ ptl)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (k >= nq) return;
+    const int g = k / kWave, lane = k % kWave;
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) s += part[(((size_t)g * nchunk + ch) * R + r) * kWave + lane];
+    ptl[(size_t)r * nq + k] = s;
+}
+
+// batched BLAS-1 over block columns (blockIdx.y); per column the arithmetic of
+// the single-vector kernels
+__global__ void __launch_bounds__(256) k_dot_partb(int64_t n, const double *__restrict__ a,
+    int64_t as, const double *__restrict__ b, int64_t bs, const int32_t *__restrict__ act,
+    double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int col = blockIdx.y;
+    const double *ar = a + (size_t)act[col] * as, *br = b + (size_t)act[col] * bs;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        s += ar[i] * br[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)col * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[act[col]*os + oofs] = sum of column col's partials
+__global__ void __launch_bounds__(256) k_sum_partb(int np, const double *__restrict__ part,
+    double *__restrict__ out, int64_t os, int oofs, const int32_t *__restrict__ act)
+{
+    __shared__ double red[256];
+    const int col = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[(size_t)col * np + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)act[col] * os + oofs] = red[0];
+}
+
+// y += (sgn * coef[act[col]*cs + cofs]) * x
+__global__ void k_axpyb(int64_t n, const double *__restrict__ coef, int64_t cs, int cofs, double sgn,
+    const double *__restrict__ x, int64_t xs, double *__restrict__ y, int64_t ys,
+    const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = act[blockIdx.y];
+    if (i >= n) return;
+    const double a = sgn * coef[(size_t)r * cs + cofs];
+    y[(size_t)r * ys + i] += a * x[(size_t)r * xs + i];
+}
+
+// x *= coef[act[col]*cs + cofs]
+__global__ void k_scalb(int64_t n, const double *__restrict__ coef, int64_t cs, int cofs,
+                        double *__restrict__ x, int64_t xs, const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = act[blockIdx.y];
+    if (i < n) x[(size_t)r * xs + i] *= coef[(size_t)r * cs + cofs];
+}
+
+// out = sum_k coef[r*cs + k] v_k, v_k = v + r*vs + k*ld
+__global__ void k_lincombb(int64_t n, int nv, const double *__restrict__ coef, int64_t cs,
+    const double *__restrict__ v, int64_t vs, int64_t ld, double *__restrict__ out, int64_t os,
+    const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = act[blockIdx.y];
+    if (i >= n) return;
+    const double *vr = v + (size_t)r * vs;
+    const double *cr = coef + (size_t)r * cs;
+    double s = 0.0;
+    for (int k = 0; k < nv; ++k) s += cr[k] * vr[(int64_t)k * ld + i];
+    out[(size_t)r * os + i] = s;
+}
+
+// out = a - b
+__global__ void k_subb(int64_t n, const double *__restrict__ a, int64_t as,
+    const double *__restrict__ b, int64_t bs, double *__restrict__ out, int64_t os,
+    const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = act[blockIdx.y];
+    if (i < n) out[(size_t)r * os + i] = a[(size_t)r * as + i] - b[(size_t)r * bs + i];
+}
+
+__global__ void k_copyb(int64_t n, const double *__restrict__ a, int64_t as,
+    double *__restrict__ out, int64_t os, const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = act[blockIdx.y];
+    if (i < n) out[(size_t)r * os + i] = a[(size_t)r * as + i];
+}
+
+__global__ void k_fillb(int64_t n, double v, double *__restrict__ out, int64_t os,
+                        const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = act[blockIdx.y];
+    if (i < n) out[(size_t)r * os + i] = v;
+}
+
+// ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
 template <int P>
@@ -1561,6 +2269,138 @@ double dev_dot(tabi_ctx *c, int64_t n, const double *a, const double *b)
     HIPCHK(hipMemcpyAsync(&h, c->red.p + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return h;
+}
+
+// ---- blocks of vectors ------------------------------------------------------
+constexpr int64_t kBlockPartBudget = (int64_t)4 << 30;   // bytes of task partials per pass
+
+template <int P, int RC>
+void launch_farb(tabi_ctx *c, int R, int64_t wstride)
+{
+    const dim3 grid(c->n_far_task, (R + RC - 1) / RC);
+    if (c->prm.precision == 1)
+        k_far32b<P, RC><<<grid, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n, R,
+            c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_wb32.p, wstride, c->ngeo.p,
+            c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, (float)c->prm.kappa,
+            c->part_farb.p);
+    else
+        k_farb<P, RC><<<grid, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n, R,
+            c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_wb.p, wstride, c->ngeo.p,
+            c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
+            c->part_farb.p);
+}
+
+template <int P>
+void block_fields(tabi_ctx *c, int R, const double *X, int64_t xs, const int32_t *d_act)
+{
+    constexpr int NK = MomShape<P>::NK, NJ = MomShape<P>::NJ;
+    // moments of every column
+    const int64_t pstride = (int64_t)c->nchunk * 4 * NK, wstride = (int64_t)c->nmom * 8 * NJ;
+    if (c->nmom > 0 && c->n_far_task > 0) {
+        KTimer kt(c, 0);
+        c->mom_partb.alloc((size_t)R * pstride);
+        c->mom_wb.alloc((size_t)R * wstride);
+        k_mom_rawb<P><<<dim3(c->nchunk, R), 256, 0, c->s>>>(c->n, c->chunk_node.p, c->chunk_beg.p,
+            c->chunk_end.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
+            c->tarea.p, X, xs, d_act, c->mom_partb.p, pstride);
+        k_mom_finalb<P><<<dim3(c->nmom, R), 256, 0, c->s>>>(c->mom_chunk_off.p, c->mom_partb.p,
+            pstride, c->prm.eps, c->mom_wb.p, wstride);
+        if (c->prm.precision == 1) {
+            c->mom_wb32.alloc((size_t)R * wstride);
+            k_to_float<<<nblk((int64_t)R * wstride, 256), 256, 0, c->s>>>((int64_t)R * wstride,
+                c->mom_wb.p, c->mom_wb32.p);
+        }
+    }
+    // near field
+    constexpr int RCN = 8;
+    const int Rp = (R + RCN - 1) / RCN * RCN;
+    if (c->n_near_task) {
+        KTimer kt(c, 1);
+        const dim3 gw(nblk(c->n, 256), Rp), gn(c->n_near_task, Rp / RCN);
+        if (c->prm.precision == 1) {
+            c->wbf.alloc((size_t)c->n * 2 * Rp);
+            k_src_weightsb<float><<<gw, 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, X, xs, d_act, c->wbf.p);
+            k_near32b<RCN><<<gn, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, R, Rp,
+                c->src32.p, c->wbf.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p,
+                c->tnz.p, (float)c->prm.kappa, (float)c->prm.eps, c->part_nearb.p);
+        } else {
+            c->wbd.alloc((size_t)c->n * 2 * Rp);
+            k_src_weightsb<double><<<gw, 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, X, xs, d_act, c->wbd.p);
+            k_nearb<RCN><<<gn, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, R, Rp,
+                c->src8.p, c->wbd.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
+                c->prm.kappa, c->prm.eps, c->part_nearb.p);
+        }
+    }
+    // far field: columns per workgroup chosen so the expansion and the
+    // accumulators stay in registers
+    if (c->n_far_task) {
+        KTimer kt(c, 2);
+        const bool f32 = c->prm.precision == 1;
+        if (P <= 3 || (f32 && P <= 4)) launch_farb<P, 8>(c, R, wstride);
+        else if (P == 4 || f32) launch_farb<P, 4>(c, R, wstride);
+        else launch_farb<P, 2>(c, R, wstride);
+    }
+}
+
+// Y[act[col]] = A X[act[col]] for the R block columns (vectors at X + k*xs,
+// results at Y + k*ys); columns are processed in passes that bound the task
+// partial buffers
+void matvec_block_dev(tabi_ctx *c, int R, const double *X, int64_t xs, double *Y, int64_t ys,
+                      const int32_t *d_act)
+{
+    const int64_t per_col = ((int64_t)c->n_near_task + c->n_far_task) * kWave * (int64_t)sizeof(double2);
+    const int Rpass = (int)std::max<int64_t>(8, std::min<int64_t>(R, kBlockPartBudget / std::max<int64_t>(per_col, 1)));
+    const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
+    for (int c0 = 0; c0 < R; c0 += Rpass) {
+        const int Rc = std::min(Rpass, R - c0);
+        const int32_t *act = d_act + c0;
+        c->part_nearb.alloc((size_t)std::max(c->n_near_task, 1) * Rc * kWave);
+        c->part_farb.alloc((size_t)std::max(c->n_far_task, 1) * Rc * kWave);
+        switch (c->p) {
+        case 1: block_fields<1>(c, Rc, X, xs, act); break;
+        case 2: block_fields<2>(c, Rc, X, xs, act); break;
+        case 3: block_fields<3>(c, Rc, X, xs, act); break;
+        case 4: block_fields<4>(c, Rc, X, xs, act); break;
+        case 5: block_fields<5>(c, Rc, X, xs, act); break;
+        case 6: block_fields<6>(c, Rc, X, xs, act); break;
+        default: throw Error("unsupported Taylor order");
+        }
+        KTimer kt(c, 3);
+        k_reduceb<<<dim3(nblk(c->n, 256), Rc), 256, 0, c->s>>>(c->n, Rc, c->near_task_off.p,
+            c->far_task_off.p, c->part_nearb.p, c->part_farb.p, X, xs, Y, ys, act, pre1, pre2);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+template <int RC>
+void launch_lu_solveb(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *Z, int64_t zs,
+                      const int32_t *d_act, int ldz)
+{
+    const size_t shm = sizeof(double) * (size_t)RC * ldz;
+    k_lu_solveb<RC><<<dim3(c->nleaf, (R + RC - 1) / RC), kWave, shm, c->s>>>(c->n, R, c->blk_ibeg.p,
+        c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, Rv, rs, Z, zs, d_act, ldz);
+}
+
+void msolve_block_dev(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *Z, int64_t zs,
+                      const int32_t *d_act)
+{
+    KTimer kt(c, 4);
+    const int ldz = c->mmax;
+    const size_t lim = 64 * 1024;
+    if (4 * sizeof(double) * (size_t)ldz <= lim) launch_lu_solveb<4>(c, R, Rv, rs, Z, zs, d_act, ldz);
+    else if (2 * sizeof(double) * (size_t)ldz <= lim) launch_lu_solveb<2>(c, R, Rv, rs, Z, zs, d_act, ldz);
+    else launch_lu_solveb<1>(c, R, Rv, rs, Z, zs, d_act, ldz);
+    HIPCHK(hipGetLastError());
+}
+
+// dots of R column pairs into out[act[col]*os + oofs] (device), the partition
+// and summation order of dev_dot
+void dev_dot_block(tabi_ctx *c, int R, int64_t n, const double *a, int64_t as, const double *b,
+                   int64_t bs, const int32_t *d_act, double *out, int64_t os, int oofs)
+{
+    c->blk_red.alloc((size_t)R * kRedBlocks);
+    k_dot_partb<<<dim3(kRedBlocks, R), 256, 0, c->s>>>(n, a, as, b, bs, d_act, c->blk_red.p);
+    k_sum_partb<<<R, 256, 0, c->s>>>(kRedBlocks, c->blk_red.p, out, os, oofs, d_act);
 }
 
 void build_lists(tabi_ctx *c)
@@ -2101,6 +2941,317 @@ void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl,
     *ierr_out = ierr;
 }
 
+// GMRES for R right-hand sides at once: R runs of gmres_dev in lockstep (they
+// share MAXL, so they restart together and differ only in when they stop).
+// Every device operation is batched over the runs still active and repeats,
+// per run, the arithmetic of gmres_dev; the host keeps each run's Hessenberg
+// matrix, rotations and stopping state.  B and X are [R][2n] on the device.
+void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double tol, int maxl,
+                     int nrmax, bool x0_zero, int *iter_out, double *err_out, int *ierr_out)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    if (maxl > N) maxl = (int)N;
+    const int ldh = maxl + 1;
+    const int64_t vst = (int64_t)(maxl + 1) * N;      // per-run stride of the Krylov basis
+    const int64_t cst = maxl + 2;                      // per-run stride of coefficients
+    const int64_t dvs = ldh + 4;                       // per-run stride of device scalars
+    const int sN = ldh, sS = ldh + 1, sT = ldh + 2;    // slots: |w|^2, |w|^2 after MGS, temp
+    DBuf<double> V, rr, zz, wk, dl, coef, dval;
+    V.alloc((size_t)R * vst);
+    rr.alloc((size_t)R * N); zz.alloc((size_t)R * N); wk.alloc((size_t)R * N); dl.alloc((size_t)R * N);
+    coef.alloc((size_t)R * cst);
+    dval.alloc((size_t)R * dvs);
+    auto vcol = [&](int k) { return V.p + (size_t)k * N; };
+    const unsigned nb = nblk(N, 256);
+    // device copies of column lists
+    DBuf<int32_t> dact_a, dact_b;
+    std::vector<int32_t> cur_a, cur_b;
+    auto put = [&](DBuf<int32_t> &d, std::vector<int32_t> &cur, const std::vector<int32_t> &v) {
+        if (v == cur) return;
+        cur = v;
+        d.upload(v.data(), std::max<size_t>(v.size(), 1), c->s);
+    };
+    std::vector<double> hv((size_t)R * dvs), hc((size_t)R * cst);
+    auto fetch = [&]() {
+        HIPCHK(hipMemcpyAsync(hv.data(), dval.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    };
+    auto put_coef = [&]() {
+        HIPCHK(hipMemcpyAsync(coef.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, c->s));
+    };
+    auto dot = [&](const std::vector<int32_t> &cols, const int32_t *dcols, const double *a, int64_t as,
+                   const double *b, int64_t bs, int slot) {
+        dev_dot_block(c, (int)cols.size(), N, a, as, b, bs, dcols, dval.p, dvs, slot);
+    };
+    auto gy = [&](size_t cnt) { return dim3(nb, (unsigned)cnt); };
+
+    std::vector<int32_t> all(R);
+    for (int r = 0; r < R; ++r) all[r] = r;
+    put(dact_a, cur_a, all);
+    // initial residual
+    msolve_block_dev(c, R, dB, N, rr.p, N, dact_a.p);
+    dot(all, dact_a.p, rr.p, N, rr.p, N, sT);
+    fetch();
+    std::vector<double> bnrm(R), r0nrm(R), err(R, 0.0), prod(R), rho(R), snormw(R), vnrm(R);
+    std::vector<int> iter(R, 0), ierr(R, 0), lgmr(R), iflag(R);
+    std::vector<char> conv(R), stall(R), early(R);
+    std::vector<std::vector<double>> hes(R, std::vector<double>((size_t)ldh * maxl)), q(R, std::vector<double>(2 * maxl));
+    for (int r = 0; r < R; ++r) bnrm[r] = std::sqrt(hv[(size_t)r * dvs + sT]);
+    {
+        std::vector<int32_t> zero, nonzero;
+        if (x0_zero) {
+            zero = all;
+        } else {
+            std::vector<double> hx((size_t)R * N);
+            HIPCHK(hipMemcpyAsync(hx.data(), dX, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            for (int r = 0; r < R; ++r) {
+                double sabs = 0.0;
+                for (int64_t i = 0; i < N; ++i) sabs += std::fabs(hx[(size_t)r * N + i]);
+                (sabs < 1.0e-10 ? zero : nonzero).push_back(r);
+            }
+        }
+        if (!zero.empty()) {
+            put(dact_b, cur_b, zero);
+            k_copyb<<<gy(zero.size()), 256, 0, c->s>>>(N, dX, N, rr.p, N, dact_b.p);
+        }
+        if (!nonzero.empty()) {
+            put(dact_b, cur_b, nonzero);
+            matvec_block_dev(c, (int)nonzero.size(), dX, N, rr.p, N, dact_b.p);
+        }
+        k_subb<<<gy(R), 256, 0, c->s>>>(N, dB, N, rr.p, N, rr.p, N, dact_a.p);
+    }
+    std::vector<int32_t> active = all;
+    int nrsts = 0;
+    while (!active.empty()) {
+        if (nrsts > nrmax) {
+            for (int r : active) ierr[r] = 1;
+            break;
+        }
+        put(dact_a, cur_a, active);
+        const unsigned na = (unsigned)active.size();
+        if (nrsts > 0) k_copyb<<<gy(na), 256, 0, c->s>>>(N, dl.p, N, rr.p, N, dact_a.p);
+        k_fillb<<<gy(na), 256, 0, c->s>>>(N, 0.0, zz.p, N, dact_a.p);
+        if (nrsts == 0) {
+            k_copyb<<<gy(na), 256, 0, c->s>>>(N, rr.p, N, wk.p, N, dact_a.p);
+            msolve_block_dev(c, (int)na, wk.p, N, rr.p, N, dact_a.p);
+        }
+        k_copyb<<<gy(na), 256, 0, c->s>>>(N, rr.p, N, vcol(0), vst, dact_a.p);
+        dot(active, dact_a.p, vcol(0), vst, vcol(0), vst, sT);
+        fetch();
+        std::vector<int32_t> inner;
+        for (int r : active) {
+            r0nrm[r] = std::sqrt(hv[(size_t)r * dvs + sT]);
+            err[r] = r0nrm[r] / bnrm[r];
+            early[r] = err[r] <= tol;
+            lgmr[r] = 0;
+            iflag[r] = 0;
+            conv[r] = 0;
+            stall[r] = 0;
+            if (!early[r]) inner.push_back(r);
+        }
+        if (!inner.empty()) {
+            put(dact_b, cur_b, inner);
+            for (int r : inner) {
+                hc[(size_t)r * cst] = 1.0 / r0nrm[r];
+                std::fill(hes[r].begin(), hes[r].end(), 0.0);
+                prod[r] = 1.0;
+                rho[r] = 0.0;
+                snormw[r] = 0.0;
+            }
+            put_coef();
+            k_scalb<<<gy(inner.size()), 256, 0, c->s>>>(N, coef.p, cst, 0, vcol(0), vst, dact_b.p);
+        }
+        for (int ll = 1; ll <= maxl && !inner.empty(); ++ll) {
+            put(dact_b, cur_b, inner);
+            const unsigned ni = (unsigned)inner.size();
+            for (int r : inner) lgmr[r] = ll;
+            matvec_block_dev(c, (int)ni, vcol(ll - 1), vst, wk.p, N, dact_b.p);
+            msolve_block_dev(c, (int)ni, wk.p, N, vcol(ll), vst, dact_b.p);
+            dot(inner, dact_b.p, vcol(ll), vst, vcol(ll), vst, sN);
+            for (int i = 0; i < ll; ++i) {
+                dot(inner, dact_b.p, vcol(i), vst, vcol(ll), vst, i);
+                k_axpyb<<<gy(ni), 256, 0, c->s>>>(N, dval.p, dvs, i, -1.0, vcol(i), vst, vcol(ll), vst,
+                                                  dact_b.p);
+            }
+            dot(inner, dact_b.p, vcol(ll), vst, vcol(ll), vst, sS);
+            fetch();
+            std::vector<int32_t> reo;
+            for (int r : inner) {
+                double *hcol = &hes[r][(size_t)(ll - 1) * ldh];
+                for (int i = 0; i < ll; ++i) hcol[i] = hv[(size_t)r * dvs + i];
+                vnrm[r] = std::sqrt(hv[(size_t)r * dvs + sN]);
+                snormw[r] = std::sqrt(hv[(size_t)r * dvs + sS]);
+                if (!(vnrm[r] + 0.001 * snormw[r] != vnrm[r])) reo.push_back(r);
+            }
+            if (!reo.empty()) {
+                // conditional re-orthogonalisation (DORTH), one basis vector at a time
+                std::vector<double> sumdsq(R, 0.0);
+                for (int i = 0; i < ll; ++i) {
+                    std::vector<int32_t> dr;
+                    DBuf<int32_t> dreo;
+                    dreo.upload(reo.data(), reo.size(), c->s);
+                    dot(reo, dreo.p, vcol(i), vst, vcol(ll), vst, sT);
+                    fetch();
+                    for (int r : reo) {
+                        double *hcol = &hes[r][(size_t)(ll - 1) * ldh];
+                        const double tem = -hv[(size_t)r * dvs + sT];
+                        if (hcol[i] + 0.001 * tem == hcol[i]) continue;
+                        hcol[i] = hcol[i] - tem;
+                        hc[(size_t)r * cst + 1] = tem;
+                        sumdsq[r] += tem * tem;
+                        dr.push_back(r);
+                    }
+                    if (!dr.empty()) {
+                        put_coef();
+                        DBuf<int32_t> ddr;
+                        ddr.upload(dr.data(), dr.size(), c->s);
+                        k_axpyb<<<gy(dr.size()), 256, 0, c->s>>>(N, coef.p, cst, 1, 1.0, vcol(i), vst,
+                                                                vcol(ll), vst, ddr.p);
+                        HIPCHK(hipStreamSynchronize(c->s));
+                    }
+                }
+                for (int r : reo)
+                    if (sumdsq[r] != 0.0)
+                        snormw[r] = std::sqrt(std::max(0.0, snormw[r] * snormw[r] - sumdsq[r]));
+            }
+            std::vector<int32_t> go;
+            for (int r : inner) {
+                double *hcol = &hes[r][(size_t)(ll - 1) * ldh];
+                double *qq = q[r].data();
+                hcol[ll] = snormw[r];
+                for (int k = 0; k < ll - 1; ++k) {
+                    const double cc = qq[2 * k], ss = qq[2 * k + 1];
+                    const double t1 = hcol[k], t2 = hcol[k + 1];
+                    hcol[k] = cc * t1 - ss * t2;
+                    hcol[k + 1] = ss * t1 + cc * t2;
+                }
+                const double t1 = hcol[ll - 1], t2 = hcol[ll];
+                double cc, ss;
+                if (t2 == 0.0) { cc = 1.0; ss = 0.0; }
+                else if (std::fabs(t2) >= std::fabs(t1)) {
+                    const double t = t1 / t2;
+                    ss = -1.0 / std::sqrt(1.0 + t * t);
+                    cc = -ss * t;
+                } else {
+                    const double t = t2 / t1;
+                    cc = 1.0 / std::sqrt(1.0 + t * t);
+                    ss = -cc * t;
+                }
+                qq[2 * (ll - 1)] = cc;
+                qq[2 * (ll - 1) + 1] = ss;
+                hcol[ll - 1] = cc * t1 - ss * t2;
+                if (hcol[ll - 1] == 0.0) { stall[r] = 1; continue; }
+                prod[r] = prod[r] * qq[2 * ll - 1];
+                rho[r] = std::fabs(prod[r] * r0nrm[r]);
+                // DRLCAL coefficients
+                double *cf = &hc[(size_t)r * cst];
+                for (int k = 0; k <= ll; ++k) cf[k] = 0.0;
+                cf[0] = 1.0;
+                for (int i = 1; i <= ll - 1; ++i) {
+                    const double s2 = qq[2 * i - 1], c2 = qq[2 * i - 2];
+                    for (int k = 0; k < i; ++k) cf[k] *= s2;
+                    cf[i] = c2;
+                }
+                const double s2 = qq[2 * ll - 1], c2 = qq[2 * ll - 2] / snormw[r];
+                for (int k = 0; k < ll; ++k) cf[k] *= s2;
+                cf[ll] = c2;
+                const double tem = r0nrm[r] * prod[r];
+                for (int k = 0; k <= ll; ++k) cf[k] *= tem;
+                go.push_back(r);
+            }
+            std::vector<int32_t> next;
+            if (!go.empty()) {
+                put_coef();
+                put(dact_b, cur_b, go);
+                k_lincombb<<<gy(go.size()), 256, 0, c->s>>>(N, ll + 1, coef.p, cst, V.p, vst, N, dl.p, N,
+                                                            dact_b.p);
+                dot(go, dact_b.p, dl.p, N, dl.p, N, sT);
+                fetch();
+                for (int r : go) {
+                    err[r] = std::sqrt(hv[(size_t)r * dvs + sT]) / bnrm[r];
+                    if (err[r] <= tol) conv[r] = 1;
+                    else next.push_back(r);
+                }
+            }
+            if (ll == maxl) break;
+            inner = next;
+            if (!inner.empty()) {
+                for (int r : inner) hc[(size_t)r * cst] = 1.0 / snormw[r];
+                put_coef();
+                put(dact_b, cur_b, inner);
+                k_scalb<<<gy(inner.size()), 256, 0, c->s>>>(N, coef.p, cst, 0, vcol(ll), vst, dact_b.p);
+            }
+        }
+        // finish the cycle for every active run
+        std::vector<int32_t> need_z, add_z, still;
+        for (int r : active) {
+            if (early[r]) { iflag[r] = 0; continue; }
+            if (stall[r]) { iflag[r] = 2; continue; }
+            if (!conv[r]) iflag[r] = (rho[r] < r0nrm[r]) ? 1 : 2;
+            if (iflag[r] != 2) {
+                const int nn = lgmr[r];
+                std::vector<double> yls(maxl + 1, 0.0);
+                yls[0] = r0nrm[r];
+                const double *qq = q[r].data();
+                for (int k = 0; k < nn; ++k) {
+                    const double cc = qq[2 * k], ss = qq[2 * k + 1];
+                    const double t1 = yls[k], t2 = yls[k + 1];
+                    yls[k] = cc * t1 - ss * t2;
+                    yls[k + 1] = ss * t1 + cc * t2;
+                }
+                const std::vector<double> &H = hes[r];
+                for (int kb = 0; kb < nn; ++kb) {
+                    const int k = nn - 1 - kb;
+                    yls[k] = yls[k] / H[(size_t)k * ldh + k];
+                    const double t = -yls[k];
+                    for (int i = 0; i < k; ++i) yls[i] += t * H[(size_t)k * ldh + i];
+                }
+                for (int k = 0; k < nn; ++k) hc[(size_t)r * cst + k] = yls[k];
+                need_z.push_back(r);
+            }
+        }
+        if (!need_z.empty()) {
+            // all runs in need_z share lgmr except those that converged early in
+            // the cycle: group by basis size
+            put_coef();
+            std::map<int, std::vector<int32_t>> by_nn;
+            for (int r : need_z) by_nn[lgmr[r]].push_back(r);
+            for (auto &kv : by_nn) {
+                DBuf<int32_t> dg;
+                dg.upload(kv.second.data(), kv.second.size(), c->s);
+                k_lincombb<<<gy(kv.second.size()), 256, 0, c->s>>>(N, kv.first, coef.p, cst, V.p, vst,
+                                                                   N, zz.p, N, dg.p);
+                HIPCHK(hipStreamSynchronize(c->s));
+            }
+        }
+        for (int r : active) {
+            iter[r] += lgmr[r];
+            if (iflag[r] != 2) add_z.push_back(r);
+        }
+        if (!add_z.empty()) {
+            for (int r : add_z) hc[(size_t)r * cst + maxl + 1] = 1.0;
+            put_coef();
+            put(dact_b, cur_b, add_z);
+            k_axpyb<<<gy(add_z.size()), 256, 0, c->s>>>(N, coef.p, cst, maxl + 1, 1.0, zz.p, N, dX, N,
+                                                       dact_b.p);
+        }
+        for (int r : active) {
+            if (iflag[r] == 0) ierr[r] = 0;
+            else if (iflag[r] == 1) still.push_back(r);
+            else ierr[r] = 2;
+        }
+        active = still;
+        ++nrsts;
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int r = 0; r < R; ++r) {
+        iter_out[r] = iter[r];
+        err_out[r] = err[r];
+        ierr_out[r] = ierr[r];
+    }
+}
+
 void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p, double *db)
 {
     DBuf<double> dq, dqp;
@@ -2110,6 +3261,69 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
         c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, db);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
+}
+
+// Right-hand sides of R charge sets (q[R][nq], host) at common positions
+// into dB[R][2n].
+void rhs_block_dev(tabi_ctx *c, int R, int nq, const double *qpos, const double *q, double eps_p,
+                   double *dB)
+{
+    constexpr int RC = 8;
+    DBuf<double> dqp, dq;
+    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
+    dq.upload(q, (size_t)R * std::max(nq, 1), c->s);
+    k_rhsb<RC><<<dim3(nblk(c->n, 256), (R + RC - 1) / RC), 256, 0, c->s>>>(c->n, nq, R, dqp.p, dq.p,
+        eps_p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, dB);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+}
+
+// Energies of R solutions dX[R][2n] for the charge sets q[R][nq] (host).
+void energy_block_dev(tabi_ctx *c, int R, int nq, const double *qpos, const double *q,
+                      double eps_p, const double *dX, double *sol, double *cou)
+{
+    KTimer kt(c, 7);
+    const double para = 332.0716;  // src/main.f90:64
+    if (nq <= 0) {
+        for (int r = 0; r < R; ++r) { sol[r] = 0.0; cou[r] = 0.0; }
+        return;
+    }
+    constexpr int RC = 8;
+    const int Rp = (R + RC - 1) / RC * RC;
+    const int64_t N = 2 * (int64_t)c->n;
+    DBuf<double> dqp, dq, dptl, part, wb, cpart;
+    DBuf<int32_t> dall;
+    std::vector<int32_t> all(R);
+    for (int r = 0; r < R; ++r) all[r] = r;
+    dall.upload(all.data(), R, c->s);
+    dqp.upload(qpos, (size_t)3 * nq, c->s);
+    dq.upload(q, (size_t)R * nq, c->s);
+    const int ngq = (nq + kWave - 1) / kWave;
+    const int nch = (c->n + kEnergyChunk - 1) / kEnergyChunk;
+    wb.alloc((size_t)c->n * 2 * Rp);
+    part.alloc((size_t)ngq * nch * R * kWave);
+    dptl.alloc((size_t)R * nq);
+    k_src_weightsb<double><<<dim3(nblk(c->n, 256), Rp), 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, dX, N,
+                                                                      dall.p, wb.p);
+    k_energyb<RC><<<dim3(ngq * nch, Rp / RC), 64, 0, c->s>>>(c->n, nq, nch, R, Rp, dqp.p, c->src8.p,
+                                                            wb.p, c->prm.eps, c->prm.kappa, part.p);
+    k_energy_reduceb<<<dim3(nblk(nq, 256), R), 256, 0, c->s>>>(nq, nch, R, part.p, dptl.p);
+    const int ncb = (nq + 255) / 256;
+    cpart.alloc((size_t)R * ncb);
+    for (int r = 0; r < R; ++r)
+        k_coulomb<<<ncb, 256, 0, c->s>>>(nq, dqp.p, dq.p + (size_t)r * nq, cpart.p + (size_t)r * ncb);
+    HIPCHK(hipGetLastError());
+    std::vector<double> ptl((size_t)R * nq), cp((size_t)R * ncb);
+    HIPCHK(hipMemcpyAsync(ptl.data(), dptl.p, sizeof(double) * ptl.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(cp.data(), cpart.p, sizeof(double) * cp.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int r = 0; r < R; ++r) {
+        double s = 0.0, cc = 0.0;
+        for (int k = 0; k < nq; ++k) s += q[(size_t)r * nq + k] * ptl[(size_t)r * nq + k];
+        for (int b = 0; b < ncb; ++b) cc += cp[(size_t)r * ncb + b];
+        sol[r] = s * 0.5 * para * 4.0 * kPi;
+        cou[r] = cc / eps_p * para;
+    }
 }
 
 // Warm-start features of this surface for the charges (rhs db already formed);
@@ -2345,6 +3559,102 @@ int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z)
     return guarded([&] {
         if (!c) throw Error("null context");
         msolve_dev(c, d_r, d_z);
+    });
+}
+
+int tabi_ctx_matvec_block(tabi_ctx *c, int32_t nrhs, const double *x, double *y)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (nrhs < 1) throw Error("need at least one right-hand side");
+        const int64_t N = 2 * (int64_t)c->n;
+        std::vector<int32_t> live;
+        for (int r = 0; r < nrhs; ++r) {
+            double sabs = 0.0;
+            for (int64_t i = 0; i < N; ++i) sabs += std::fabs(x[(size_t)r * N + i]);
+            if (sabs < 1.0e-10)  // src/main.f90:478-479, per vector
+                std::memcpy(y + (size_t)r * N, x + (size_t)r * N, sizeof(double) * N);
+            else
+                live.push_back(r);
+        }
+        if (live.empty()) return;
+        DBuf<double> dX, dY;
+        DBuf<int32_t> dact;
+        dX.upload(x, (size_t)nrhs * N, c->s);
+        dY.alloc((size_t)nrhs * N);
+        dact.upload(live.data(), live.size(), c->s);
+        matvec_block_dev(c, (int)live.size(), dX.p, N, dY.p, N, dact.p);
+        std::vector<double> hy((size_t)nrhs * N);
+        HIPCHK(hipMemcpyAsync(hy.data(), dY.p, sizeof(double) * hy.size(), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (int r : live) std::memcpy(y + (size_t)r * N, hy.data() + (size_t)r * N, sizeof(double) * N);
+    });
+}
+
+int tabi_ctx_msolve_block(tabi_ctx *c, int32_t nrhs, const double *r, double *z)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (nrhs < 1) throw Error("need at least one right-hand side");
+        const int64_t N = 2 * (int64_t)c->n;
+        DBuf<double> dR, dZ;
+        DBuf<int32_t> dact;
+        std::vector<int32_t> all(nrhs);
+        for (int k = 0; k < nrhs; ++k) all[k] = k;
+        dR.upload(r, (size_t)nrhs * N, c->s);
+        dZ.alloc((size_t)nrhs * N);
+        dact.upload(all.data(), nrhs, c->s);
+        msolve_block_dev(c, nrhs, dR.p, N, dZ.p, N, dact.p);
+        HIPCHK(hipMemcpyAsync(z, dZ.p, sizeof(double) * nrhs * N, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    });
+}
+
+int tabi_ctx_gmres_block(tabi_ctx *c, int32_t nrhs, const double *b, double *x, double tol,
+                         int32_t maxl, int32_t nrmax, int32_t *iter, double *err, int32_t *ierr)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (nrhs < 1) throw Error("need at least one right-hand side");
+        if (maxl < 1) throw Error("maxl must be >= 1");
+        const int64_t N = 2 * (int64_t)c->n;
+        DBuf<double> dB, dX;
+        dB.upload(b, (size_t)nrhs * N, c->s);
+        dX.upload(x, (size_t)nrhs * N, c->s);
+        std::vector<int> it(nrhs), ie(nrhs);
+        gmres_block_dev(c, nrhs, dB.p, dX.p, tol, maxl, nrmax, false, it.data(), err, ie.data());
+        HIPCHK(hipMemcpyAsync(x, dX.p, sizeof(double) * nrhs * N, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (int k = 0; k < nrhs; ++k) { iter[k] = it[k]; ierr[k] = ie[k]; }
+    });
+}
+
+int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qpos,
+                         const double *q, double eps_p, double tol, double *x_out, int32_t *iter,
+                         double *err, int32_t *ierr, double *soleng, double *couleng)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (nrhs < 1) throw Error("need at least one charge set");
+        if (nq < 1) throw Error("need at least one charge");
+        const int64_t N = 2 * (int64_t)c->n;
+        DBuf<double> dB, dX;
+        DBuf<int32_t> dall;
+        std::vector<int32_t> all(nrhs);
+        for (int k = 0; k < nrhs; ++k) all[k] = k;
+        dall.upload(all.data(), nrhs, c->s);
+        dB.alloc((size_t)nrhs * N);
+        dX.alloc((size_t)nrhs * N);
+        rhs_block_dev(c, nrhs, nq, qpos, q, eps_p, dB.p);
+        k_fillb<<<dim3(nblk(N, 256), nrhs), 256, 0, c->s>>>(N, 0.0, dX.p, N, dall.p);
+        std::vector<int> it(nrhs), ie(nrhs);
+        gmres_block_dev(c, nrhs, dB.p, dX.p, tol, 10, 10, true, it.data(), err, ie.data());
+        energy_block_dev(c, nrhs, nq, qpos, q, eps_p, dX.p, soleng, couleng);
+        if (x_out) {
+            HIPCHK(hipMemcpyAsync(x_out, dX.p, sizeof(double) * nrhs * N, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+        }
+        for (int k = 0; k < nrhs; ++k) { iter[k] = it[k]; ierr[k] = ie[k]; }
     });
 }
 

@@ -24,7 +24,8 @@ EXPORTS = (
     "tabi_device_alloc", "tabi_device_free", "tabi_release_memory", "tabi_copy_to_device", "tabi_copy_to_host",
     "tabi_ws_param_count", "tabi_ws_create", "tabi_ws_destroy", "tabi_ws_get_params",
     "tabi_ctx_ws_features", "tabi_ws_predict", "tabi_ws_grad", "tabi_ws_train",
-    "tabi_ctx_gmres_ws", "tabi_solve_ws",
+    "tabi_ctx_gmres_ws", "tabi_solve_ws", "tabi_ctx_matvec_block", "tabi_ctx_msolve_block",
+    "tabi_ctx_gmres_block", "tabi_ctx_solve_block",
 )
 
 SOLVE_PHASES = ("tree", "lists", "precond", "rhs", "gmres", "energy", "total", "warmstart")
@@ -247,6 +248,53 @@ class Surface:
         _check(load().tabi_ctx_gmres(self._h, _p(b), _p(x), C.c_double(tol), C.c_int32(maxl),
                                      C.c_int32(nrmax), C.byref(it), C.byref(err), C.byref(ierr)))
         return x, it.value, err.value, ierr.value
+
+    def _blk(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.ndim != 2 or x.shape[1] != 2 * self.n:
+            raise TabiError(f"block must have shape (nrhs, {2 * self.n})")
+        return x
+
+    def matvec_block(self, x):
+        """Y[k] = A X[k] for a block X of shape (nrhs, 2n) in one tree pass."""
+        x = self._blk(x)
+        y = np.empty_like(x)
+        _check(load().tabi_ctx_matvec_block(self._h, C.c_int32(x.shape[0]), _p(x), _p(y)))
+        return y
+
+    def msolve_block(self, r):
+        r = self._blk(r)
+        z = np.empty_like(r)
+        _check(load().tabi_ctx_msolve_block(self._h, C.c_int32(r.shape[0]), _p(r), _p(z)))
+        return z
+
+    def gmres_block(self, b, x0=None, tol=1e-4, maxl=10, nrmax=10):
+        b = self._blk(b)
+        x = np.zeros_like(b) if x0 is None else self._blk(x0).copy()
+        k = b.shape[0]
+        it, ierr = np.zeros(k, dtype=np.int32), np.zeros(k, dtype=np.int32)
+        err = np.zeros(k)
+        _check(load().tabi_ctx_gmres_block(self._h, C.c_int32(k), _p(b), _p(x), C.c_double(tol),
+                                           C.c_int32(maxl), C.c_int32(nrmax), _p(it), _p(err),
+                                           _p(ierr)))
+        return x, it, err, ierr
+
+    def solve_block(self, qpos, q, eps_p, tol=1e-4, want_x=True):
+        """Solves for the charge sets q (nrhs, nq) at common positions qpos:
+        returns dict(x, iter, err, ierr, soleng, couleng) with per-set arrays."""
+        qpos = _d(qpos, (-1, 3))
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.ndim != 2 or q.shape[1] != qpos.shape[0]:
+            raise TabiError("charges must have shape (nrhs, nq)")
+        k, nq = q.shape
+        x = np.zeros((k, 2 * self.n)) if want_x else None
+        it, ierr = np.zeros(k, dtype=np.int32), np.zeros(k, dtype=np.int32)
+        err, sol, cou = np.zeros(k), np.zeros(k), np.zeros(k)
+        _check(load().tabi_ctx_solve_block(self._h, C.c_int32(k), C.c_int32(nq), _p(qpos), _p(q),
+                                           C.c_double(eps_p), C.c_double(tol),
+                                           _p(x) if want_x else None, _p(it), _p(err), _p(ierr),
+                                           _p(sol), _p(cou)))
+        return dict(x=x, iter=it, err=err, ierr=ierr, soleng=sol, couleng=cou)
 
     def ws_features(self, qpos, q, eps_p):
         """Warm-start features [n, 12] (tree order) for these charges and the
