@@ -126,6 +126,49 @@ def cpu_baseline(prob, iters, budget_s=12.0):
                 sample=sample, t_matvec_s=t_mv, t_msolve_s=t_ms, t_solve_s=t_solve)
 
 
+def batched_charge_sets(nsets=64, natom=1500, precision=1):
+    """Many charge sets on one surface: the 1500-atom synthetic molecule (seed 1)
+    with nsets charge vectors at the same atom positions (N(0, 0.35) per atom,
+    shifted to net -3 e), solved as one block (one tree pass per GMRES step for
+    all sets) and as nsets single solves on the same set-up surface."""
+    from tabipb import surface
+    xyz, q, r = surface.synth_molecule(natom, seed=1)
+    v, nv, f = surface.ses_mesh(xyz, r, probe=1.4, h=0.5)
+    f, _ = driver.cull_faces(v, f)
+    fx, fn, fa = driver.face_geometry(v, nv, f)
+    qpos = xyz.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(7)
+    Q = rng.normal(0.0, 0.35, (nsets, natom))
+    Q -= (Q.sum(1, keepdims=True) + 3.0) / natom
+    Q = Q.astype(np.float32).astype(np.float64)
+    usr = meshio.UsrData(maxparnode=500, order=3, mac=0.8)
+    prm = hip.make_params(usr.epsw / usr.epsp, driver.kappa_of(usr.bulk_strength, usr.epsw),
+                          usr.mac, usr.order, usr.maxparnode, precision)
+    S = hip.Surface(prm, fx, fn, fa)
+    S.solve_block(qpos, Q[:8], usr.epsp, want_x=False)       # warm-up
+    t0 = time.perf_counter()
+    blk = S.solve_block(qpos, Q, usr.epsp, want_x=False)
+    t_blk = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    its, sols = [], []
+    for k in range(nsets):
+        b = S.rhs(qpos, Q[k], usr.epsp)
+        x, it, err, ierr = S.gmres(b, tol=1e-4)
+        sols.append(S.energy(qpos, Q[k], usr.epsp, x)[0])
+        its.append(it)
+    t_seq = time.perf_counter() - t0
+    S.close()
+    sols = np.array(sols)
+    return dict(workload=f"synthetic {natom}-atom molecule (seed 1), {len(f)} faces, "
+                         f"{nsets} charge sets", faces=int(len(f)), sets=nsets,
+                precision="mixed" if precision == 1 else "fp64",
+                block_ms=t_blk * 1e3, sequential_ms=t_seq * 1e3, speedup=t_seq / t_blk,
+                sets_per_s_block=nsets / t_blk, sets_per_s_sequential=nsets / t_seq,
+                iters_equal=bool(np.array_equal(blk["iter"], its)),
+                max_dG_rel_diff=float(np.max(np.abs(blk["soleng"] - sols) / np.abs(sols))),
+                mean_iters=float(np.mean(its)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +177,7 @@ def main():
     ap.add_argument("--workload", default="mol8000")
     ap.add_argument("--matvecs", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -210,6 +254,11 @@ def main():
     print(f"[bench] matvec {mv_ms:.3f} ms (fp64 {mv64_ms:.3f} ms); kernel avg ms {avg}; "
           f"msolve {msolve_ms:.3f} ms; stats {st}", file=sys.stderr, flush=True)
 
+    batch = None
+    if not a.no_batch:
+        batch = batched_charge_sets()
+        print(f"[bench] batched charge sets {batch}", file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(prob, res64["iter"])
@@ -240,7 +289,8 @@ def main():
                        mixed_learned_ms_per_solve=None if dtws is None else dtws * 1e3,
                        matvec_ms=mv_ms, fp64_matvec_ms=mv64_ms, msolve_ms=msolve_ms,
                        matvecs_per_s=1e3 / mv_ms,
-                       phases_ms={k: float(v) for k, v in res["times_ms"].items()}),
+                       phases_ms={k: float(v) for k, v in res["times_ms"].items()},
+                       batched_charge_sets=batch),
         "roofline": {"bound": "valu", "kernel": ("k_" + dom + ("32" if prec == 1 else "")),
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,

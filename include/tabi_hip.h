@@ -133,6 +133,24 @@ int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qp
                          const double *q, double eps_p, double tol, double *x_out, int32_t *iter,
                          double *err, int32_t *ierr, double *soleng, double *couleng);
 
+/* Full solves of nsurf independent surfaces (face arrays xyz[k], nrm[k],
+ * area[k] of nface[k] faces, original order; charges qpos[k], q[k] of nq[k]
+ * charges), all with the parameters prm: the surfaces are set up from several
+ * host threads, then their GMRES runs (MAXL = NRMAX = 10, tol, zero start)
+ * advance in lockstep; each step's matvec and preconditioner solve of every
+ * active surface is one replay of a captured HIP graph with one branch per
+ * surface.  options bit 0 launches the same work eagerly instead of through the
+ * graph.  Per-surface results: iter, err, ierr, soleng, couleng.  No
+ * counterpart in the reference, which solves one surface per run. */
+int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
+                    const double *const *xyz, const double *const *nrm,
+                    const double *const *area, const int32_t *nq, const double *const *qpos,
+                    const double *const *q, double eps_p, double tol, int32_t options,
+                    int32_t *iter, double *err, int32_t *ierr, double *soleng, double *couleng);
+/* Of the last tabi_solve_many on this thread: {graph captures, graph launches,
+ * set-up + right-hand sides ms, GMRES ms, energies ms, total ms}. */
+int tabi_last_solve_many_stats(double *out6);
+
 /* The library keeps freed device blocks for reuse by later contexts and solves;
  * this returns all cached (unused) blocks to the runtime. */
 int tabi_release_memory(void);

@@ -30,6 +30,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tabi_hip.h"
@@ -50,6 +51,7 @@ constexpr int kLuNB = 32;        // LU panel width
 
 thread_local std::string g_err;
 thread_local double g_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+thread_local double g_many[6] = {0, 0, 0, 0, 0, 0};
 
 struct Error : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -2063,15 +2065,17 @@ __global__ void k_energy_reduceb(int nq, int nchunk, int R, const double *__rest
     ptl[(size_t)r * nq + k] = s;
 }
 
-// batched BLAS-1 over block columns (blockIdx.y); per column the arithmetic of
-// the single-vector kernels
-__global__ void __launch_bounds__(256) k_dot_partb(int64_t n, const double *__restrict__ a,
-    int64_t as, const double *__restrict__ b, int64_t bs, const int32_t *__restrict__ act,
-    double *__restrict__ part)
+// Batched BLAS-1 for R GMRES runs advanced together.  Run r's vectors are
+// found through a pointer table, tab[slot*R + r], and have length nv[r]; the
+// column list act maps grid column blockIdx.y to a run.  Per run the arithmetic
+// is that of the single-vector kernels (same partition and summation order).
+__global__ void __launch_bounds__(256) k_dot_partt(double *const *__restrict__ tab, int R, int sa,
+    int sb, const int64_t *__restrict__ nv, const int32_t *__restrict__ act, double *__restrict__ part)
 {
     __shared__ double red[256];
-    const int col = blockIdx.y;
-    const double *ar = a + (size_t)act[col] * as, *br = b + (size_t)act[col] * bs;
+    const int col = blockIdx.y, r = act[col];
+    const double *ar = tab[(size_t)sa * R + r], *br = tab[(size_t)sb * R + r];
+    const int64_t n = nv[r];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         s += ar[i] * br[i];
@@ -2101,67 +2105,67 @@ __global__ void __launch_bounds__(256) k_sum_partb(int np, const double *__restr
     if (threadIdx.x == 0) out[(size_t)act[col] * os + oofs] = red[0];
 }
 
-// y += (sgn * coef[act[col]*cs + cofs]) * x
-__global__ void k_axpyb(int64_t n, const double *__restrict__ coef, int64_t cs, int cofs, double sgn,
-    const double *__restrict__ x, int64_t xs, double *__restrict__ y, int64_t ys,
-    const int32_t *__restrict__ act)
+// y += (sgn * coef[r*cs + cofs]) * x
+__global__ void k_axpyt(double *const *__restrict__ tab, int R, int sx, int sy,
+    const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int cofs,
+    double sgn, const int32_t *__restrict__ act)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i >= n) return;
+    if (i >= nv[r]) return;
     const double a = sgn * coef[(size_t)r * cs + cofs];
-    y[(size_t)r * ys + i] += a * x[(size_t)r * xs + i];
+    double *y = tab[(size_t)sy * R + r];
+    y[i] += a * tab[(size_t)sx * R + r][i];
 }
 
-// x *= coef[act[col]*cs + cofs]
-__global__ void k_scalb(int64_t n, const double *__restrict__ coef, int64_t cs, int cofs,
-                        double *__restrict__ x, int64_t xs, const int32_t *__restrict__ act)
+// x *= coef[r*cs + cofs]
+__global__ void k_scalt(double *const *__restrict__ tab, int R, int sx, const int64_t *__restrict__ nv,
+    const double *__restrict__ coef, int64_t cs, int cofs, const int32_t *__restrict__ act)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < n) x[(size_t)r * xs + i] *= coef[(size_t)r * cs + cofs];
+    if (i < nv[r]) tab[(size_t)sx * R + r][i] *= coef[(size_t)r * cs + cofs];
 }
 
-// out = sum_k coef[r*cs + k] v_k, v_k = v + r*vs + k*ld
-__global__ void k_lincombb(int64_t n, int nv, const double *__restrict__ coef, int64_t cs,
-    const double *__restrict__ v, int64_t vs, int64_t ld, double *__restrict__ out, int64_t os,
+// out = sum_{k < nvv} coef[r*cs + k] * slot(sv0 + k)
+__global__ void k_lincombt(double *const *__restrict__ tab, int R, int sv0, int nvv,
+    const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int so,
     const int32_t *__restrict__ act)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i >= n) return;
-    const double *vr = v + (size_t)r * vs;
+    if (i >= nv[r]) return;
     const double *cr = coef + (size_t)r * cs;
     double s = 0.0;
-    for (int k = 0; k < nv; ++k) s += cr[k] * vr[(int64_t)k * ld + i];
-    out[(size_t)r * os + i] = s;
+    for (int k = 0; k < nvv; ++k) s += cr[k] * tab[(size_t)(sv0 + k) * R + r][i];
+    tab[(size_t)so * R + r][i] = s;
 }
 
 // out = a - b
-__global__ void k_subb(int64_t n, const double *__restrict__ a, int64_t as,
-    const double *__restrict__ b, int64_t bs, double *__restrict__ out, int64_t os,
-    const int32_t *__restrict__ act)
+__global__ void k_subt(double *const *__restrict__ tab, int R, int sa, int sb, int so,
+    const int64_t *__restrict__ nv, const int32_t *__restrict__ act)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < n) out[(size_t)r * os + i] = a[(size_t)r * as + i] - b[(size_t)r * bs + i];
+    if (i < nv[r]) tab[(size_t)so * R + r][i] = tab[(size_t)sa * R + r][i] - tab[(size_t)sb * R + r][i];
 }
 
-__global__ void k_copyb(int64_t n, const double *__restrict__ a, int64_t as,
-    double *__restrict__ out, int64_t os, const int32_t *__restrict__ act)
+__global__ void k_copyt(double *const *__restrict__ tab, int R, int sa, int so,
+    const int64_t *__restrict__ nv, const int32_t *__restrict__ act)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < n) out[(size_t)r * os + i] = a[(size_t)r * as + i];
+    if (i < nv[r]) tab[(size_t)so * R + r][i] = tab[(size_t)sa * R + r][i];
 }
 
-__global__ void k_fillb(int64_t n, double v, double *__restrict__ out, int64_t os,
-                        const int32_t *__restrict__ act)
+__global__ void k_fillt(double *const *__restrict__ tab, int R, int so, const int64_t *__restrict__ nv,
+    double v, const int32_t *__restrict__ act)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < n) out[(size_t)r * os + i] = v;
+    if (i < nv[r]) tab[(size_t)so * R + r][i] = v;
 }
+
 
 // ---------------------------------------------------------------------------
 // host orchestration
@@ -2391,16 +2395,6 @@ void msolve_block_dev(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *
     else if (2 * sizeof(double) * (size_t)ldz <= lim) launch_lu_solveb<2>(c, R, Rv, rs, Z, zs, d_act, ldz);
     else launch_lu_solveb<1>(c, R, Rv, rs, Z, zs, d_act, ldz);
     HIPCHK(hipGetLastError());
-}
-
-// dots of R column pairs into out[act[col]*os + oofs] (device), the partition
-// and summation order of dev_dot
-void dev_dot_block(tabi_ctx *c, int R, int64_t n, const double *a, int64_t as, const double *b,
-                   int64_t bs, const int32_t *d_act, double *out, int64_t os, int oofs)
-{
-    c->blk_red.alloc((size_t)R * kRedBlocks);
-    k_dot_partb<<<dim3(kRedBlocks, R), 256, 0, c->s>>>(n, a, as, b, bs, d_act, c->blk_red.p);
-    k_sum_partb<<<R, 256, 0, c->s>>>(kRedBlocks, c->blk_red.p, out, os, oofs, d_act);
 }
 
 void build_lists(tabi_ctx *c)
@@ -2941,85 +2935,122 @@ void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl,
     *ierr_out = ierr;
 }
 
-// GMRES for R right-hand sides at once: R runs of gmres_dev in lockstep (they
-// share MAXL, so they restart together and differ only in when they stop).
-// Every device operation is batched over the runs still active and repeats,
-// per run, the arithmetic of gmres_dev; the host keeps each run's Hessenberg
-// matrix, rotations and stopping state.  B and X are [R][2n] on the device.
-void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double tol, int maxl,
-                     int nrmax, bool x0_zero, int *iter_out, double *err_out, int *ierr_out)
+// ---- R runs of GMRES in lockstep -----------------------------------------
+// Slots of the per-run vector table: fixed work vectors, then the Krylov basis.
+enum : int { kSlotR = 0, kSlotZ, kSlotWK, kSlotDL, kSlotB, kSlotX, kSlotT1, kSlotT2, kSlotV0 };
+
+// Storage and operators of R runs.  tab[slot*R + r] points at run r's vector
+// for the slot (length n[r]); apply() forms out = M^-1 (A in) for the listed
+// runs, matvec()/msolve() the two halves.
+struct RunSet {
+    int R = 0, maxl = 0;
+    hipStream_t s = nullptr;           // stream of the batched vector operations
+    std::vector<int64_t> n;
+    std::vector<double *> htab;
+    DBuf<double *> tab;
+    DBuf<int64_t> dn;
+    int nslots() const { return kSlotV0 + maxl + 1; }
+    double *ptr(int slot, int r) const { return htab[(size_t)slot * R + r]; }
+    void upload_tables()
+    {
+        tab.upload(htab.data(), htab.size(), s);
+        dn.upload(n.data(), n.size(), s);
+    }
+    virtual ~RunSet() = default;
+    virtual void matvec(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout) = 0;
+    virtual void msolve(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout) = 0;
+    virtual void apply(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout)
+    {
+        matvec(cols, dcols, sin, kSlotWK);
+        msolve(cols, dcols, kSlotWK, sout);
+    }
+};
+
+// GMRES with the reference's settings for each run (gmres_dev), the runs in
+// lockstep: they share MAXL, so they restart together and differ only in when
+// they stop.  Every device operation is batched over the runs still active and
+// repeats, per run, the arithmetic of gmres_dev; the host keeps each run's
+// Hessenberg matrix, rotations and stopping state.  The B and X slots hold the
+// right-hand sides and the initial guesses / solutions.
+void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_zero, int *iter_out,
+                double *err_out, int *ierr_out)
 {
-    const int64_t N = 2 * (int64_t)c->n;
-    if (maxl > N) maxl = (int)N;
-    const int ldh = maxl + 1;
-    const int64_t vst = (int64_t)(maxl + 1) * N;      // per-run stride of the Krylov basis
+    const int R = rs.R, maxl = rs.maxl, ldh = maxl + 1;
+    hipStream_t st = rs.s;
     const int64_t cst = maxl + 2;                      // per-run stride of coefficients
     const int64_t dvs = ldh + 4;                       // per-run stride of device scalars
     const int sN = ldh, sS = ldh + 1, sT = ldh + 2;    // slots: |w|^2, |w|^2 after MGS, temp
-    DBuf<double> V, rr, zz, wk, dl, coef, dval;
-    V.alloc((size_t)R * vst);
-    rr.alloc((size_t)R * N); zz.alloc((size_t)R * N); wk.alloc((size_t)R * N); dl.alloc((size_t)R * N);
+    int64_t nmax = 0;
+    for (int64_t v : rs.n) nmax = std::max(nmax, v);
+    const unsigned nb = nblk(nmax, 256);
+    DBuf<double> coef, dval;
     coef.alloc((size_t)R * cst);
     dval.alloc((size_t)R * dvs);
-    auto vcol = [&](int k) { return V.p + (size_t)k * N; };
-    const unsigned nb = nblk(N, 256);
-    // device copies of column lists
+    red.alloc((size_t)R * kRedBlocks);
+    double *const *T = rs.tab.p;
+    const int64_t *NV = rs.dn.p;
+    auto V = [&](int k) { return kSlotV0 + k; };
+    // device copies of run lists
     DBuf<int32_t> dact_a, dact_b;
     std::vector<int32_t> cur_a, cur_b;
     auto put = [&](DBuf<int32_t> &d, std::vector<int32_t> &cur, const std::vector<int32_t> &v) {
         if (v == cur) return;
         cur = v;
-        d.upload(v.data(), std::max<size_t>(v.size(), 1), c->s);
+        d.upload(v.data(), std::max<size_t>(v.size(), 1), st);
     };
     std::vector<double> hv((size_t)R * dvs), hc((size_t)R * cst);
     auto fetch = [&]() {
-        HIPCHK(hipMemcpyAsync(hv.data(), dval.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        HIPCHK(hipMemcpyAsync(hv.data(), dval.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
     };
     auto put_coef = [&]() {
-        HIPCHK(hipMemcpyAsync(coef.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, c->s));
-    };
-    auto dot = [&](const std::vector<int32_t> &cols, const int32_t *dcols, const double *a, int64_t as,
-                   const double *b, int64_t bs, int slot) {
-        dev_dot_block(c, (int)cols.size(), N, a, as, b, bs, dcols, dval.p, dvs, slot);
+        HIPCHK(hipMemcpyAsync(coef.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, st));
     };
     auto gy = [&](size_t cnt) { return dim3(nb, (unsigned)cnt); };
+    // dval[r][slot] = (a, b) for the listed runs, dev_dot's partition and order
+    auto dot = [&](const std::vector<int32_t> &cols, const int32_t *dcols, int sa, int sb, int slot) {
+        k_dot_partt<<<dim3(kRedBlocks, (unsigned)cols.size()), 256, 0, st>>>(T, R, sa, sb, NV, dcols,
+                                                                              red.p);
+        k_sum_partb<<<(unsigned)cols.size(), 256, 0, st>>>(kRedBlocks, red.p, dval.p, dvs, slot, dcols);
+    };
 
     std::vector<int32_t> all(R);
     for (int r = 0; r < R; ++r) all[r] = r;
     put(dact_a, cur_a, all);
     // initial residual
-    msolve_block_dev(c, R, dB, N, rr.p, N, dact_a.p);
-    dot(all, dact_a.p, rr.p, N, rr.p, N, sT);
+    rs.msolve(all, dact_a.p, kSlotB, kSlotR);
+    dot(all, dact_a.p, kSlotR, kSlotR, sT);
     fetch();
     std::vector<double> bnrm(R), r0nrm(R), err(R, 0.0), prod(R), rho(R), snormw(R), vnrm(R);
     std::vector<int> iter(R, 0), ierr(R, 0), lgmr(R), iflag(R);
     std::vector<char> conv(R), stall(R), early(R);
-    std::vector<std::vector<double>> hes(R, std::vector<double>((size_t)ldh * maxl)), q(R, std::vector<double>(2 * maxl));
+    std::vector<std::vector<double>> hes(R, std::vector<double>((size_t)ldh * maxl)),
+        q(R, std::vector<double>(2 * maxl));
     for (int r = 0; r < R; ++r) bnrm[r] = std::sqrt(hv[(size_t)r * dvs + sT]);
     {
         std::vector<int32_t> zero, nonzero;
         if (x0_zero) {
             zero = all;
         } else {
-            std::vector<double> hx((size_t)R * N);
-            HIPCHK(hipMemcpyAsync(hx.data(), dX, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, c->s));
-            HIPCHK(hipStreamSynchronize(c->s));
             for (int r = 0; r < R; ++r) {
+                std::vector<double> hx(rs.n[r]);
+                HIPCHK(hipMemcpyAsync(hx.data(), rs.ptr(kSlotX, r), sizeof(double) * hx.size(),
+                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
                 double sabs = 0.0;
-                for (int64_t i = 0; i < N; ++i) sabs += std::fabs(hx[(size_t)r * N + i]);
+                for (double v : hx) sabs += std::fabs(v);
                 (sabs < 1.0e-10 ? zero : nonzero).push_back(r);
             }
         }
         if (!zero.empty()) {
             put(dact_b, cur_b, zero);
-            k_copyb<<<gy(zero.size()), 256, 0, c->s>>>(N, dX, N, rr.p, N, dact_b.p);
+            k_copyt<<<gy(zero.size()), 256, 0, st>>>(T, R, kSlotX, kSlotR, NV, dact_b.p);
         }
         if (!nonzero.empty()) {
             put(dact_b, cur_b, nonzero);
-            matvec_block_dev(c, (int)nonzero.size(), dX, N, rr.p, N, dact_b.p);
+            rs.matvec(nonzero, dact_b.p, kSlotX, kSlotR);
         }
-        k_subb<<<gy(R), 256, 0, c->s>>>(N, dB, N, rr.p, N, rr.p, N, dact_a.p);
+        k_subt<<<gy(R), 256, 0, st>>>(T, R, kSlotB, kSlotR, kSlotR, NV, dact_a.p);
     }
     std::vector<int32_t> active = all;
     int nrsts = 0;
@@ -3030,14 +3061,14 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
         }
         put(dact_a, cur_a, active);
         const unsigned na = (unsigned)active.size();
-        if (nrsts > 0) k_copyb<<<gy(na), 256, 0, c->s>>>(N, dl.p, N, rr.p, N, dact_a.p);
-        k_fillb<<<gy(na), 256, 0, c->s>>>(N, 0.0, zz.p, N, dact_a.p);
+        if (nrsts > 0) k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotDL, kSlotR, NV, dact_a.p);
+        k_fillt<<<gy(na), 256, 0, st>>>(T, R, kSlotZ, NV, 0.0, dact_a.p);
         if (nrsts == 0) {
-            k_copyb<<<gy(na), 256, 0, c->s>>>(N, rr.p, N, wk.p, N, dact_a.p);
-            msolve_block_dev(c, (int)na, wk.p, N, rr.p, N, dact_a.p);
+            k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotR, kSlotWK, NV, dact_a.p);
+            rs.msolve(active, dact_a.p, kSlotWK, kSlotR);
         }
-        k_copyb<<<gy(na), 256, 0, c->s>>>(N, rr.p, N, vcol(0), vst, dact_a.p);
-        dot(active, dact_a.p, vcol(0), vst, vcol(0), vst, sT);
+        k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotR, V(0), NV, dact_a.p);
+        dot(active, dact_a.p, V(0), V(0), sT);
         fetch();
         std::vector<int32_t> inner;
         for (int r : active) {
@@ -3060,21 +3091,19 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
                 snormw[r] = 0.0;
             }
             put_coef();
-            k_scalb<<<gy(inner.size()), 256, 0, c->s>>>(N, coef.p, cst, 0, vcol(0), vst, dact_b.p);
+            k_scalt<<<gy(inner.size()), 256, 0, st>>>(T, R, V(0), NV, coef.p, cst, 0, dact_b.p);
         }
         for (int ll = 1; ll <= maxl && !inner.empty(); ++ll) {
             put(dact_b, cur_b, inner);
             const unsigned ni = (unsigned)inner.size();
             for (int r : inner) lgmr[r] = ll;
-            matvec_block_dev(c, (int)ni, vcol(ll - 1), vst, wk.p, N, dact_b.p);
-            msolve_block_dev(c, (int)ni, wk.p, N, vcol(ll), vst, dact_b.p);
-            dot(inner, dact_b.p, vcol(ll), vst, vcol(ll), vst, sN);
+            rs.apply(inner, dact_b.p, V(ll - 1), V(ll));
+            dot(inner, dact_b.p, V(ll), V(ll), sN);
             for (int i = 0; i < ll; ++i) {
-                dot(inner, dact_b.p, vcol(i), vst, vcol(ll), vst, i);
-                k_axpyb<<<gy(ni), 256, 0, c->s>>>(N, dval.p, dvs, i, -1.0, vcol(i), vst, vcol(ll), vst,
-                                                  dact_b.p);
+                dot(inner, dact_b.p, V(i), V(ll), i);
+                k_axpyt<<<gy(ni), 256, 0, st>>>(T, R, V(i), V(ll), NV, dval.p, dvs, i, -1.0, dact_b.p);
             }
-            dot(inner, dact_b.p, vcol(ll), vst, vcol(ll), vst, sS);
+            dot(inner, dact_b.p, V(ll), V(ll), sS);
             fetch();
             std::vector<int32_t> reo;
             for (int r : inner) {
@@ -3087,11 +3116,11 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
             if (!reo.empty()) {
                 // conditional re-orthogonalisation (DORTH), one basis vector at a time
                 std::vector<double> sumdsq(R, 0.0);
+                DBuf<int32_t> dreo, ddr;
+                dreo.upload(reo.data(), reo.size(), st);
                 for (int i = 0; i < ll; ++i) {
                     std::vector<int32_t> dr;
-                    DBuf<int32_t> dreo;
-                    dreo.upload(reo.data(), reo.size(), c->s);
-                    dot(reo, dreo.p, vcol(i), vst, vcol(ll), vst, sT);
+                    dot(reo, dreo.p, V(i), V(ll), sT);
                     fetch();
                     for (int r : reo) {
                         double *hcol = &hes[r][(size_t)(ll - 1) * ldh];
@@ -3104,11 +3133,9 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
                     }
                     if (!dr.empty()) {
                         put_coef();
-                        DBuf<int32_t> ddr;
-                        ddr.upload(dr.data(), dr.size(), c->s);
-                        k_axpyb<<<gy(dr.size()), 256, 0, c->s>>>(N, coef.p, cst, 1, 1.0, vcol(i), vst,
-                                                                vcol(ll), vst, ddr.p);
-                        HIPCHK(hipStreamSynchronize(c->s));
+                        ddr.upload(dr.data(), dr.size(), st);
+                        k_axpyt<<<gy(dr.size()), 256, 0, st>>>(T, R, V(i), V(ll), NV, coef.p, cst, 1, 1.0,
+                                                              ddr.p);
                     }
                 }
                 for (int r : reo)
@@ -3164,9 +3191,9 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
             if (!go.empty()) {
                 put_coef();
                 put(dact_b, cur_b, go);
-                k_lincombb<<<gy(go.size()), 256, 0, c->s>>>(N, ll + 1, coef.p, cst, V.p, vst, N, dl.p, N,
-                                                            dact_b.p);
-                dot(go, dact_b.p, dl.p, N, dl.p, N, sT);
+                k_lincombt<<<gy(go.size()), 256, 0, st>>>(T, R, V(0), ll + 1, NV, coef.p, cst, kSlotDL,
+                                                         dact_b.p);
+                dot(go, dact_b.p, kSlotDL, kSlotDL, sT);
                 fetch();
                 for (int r : go) {
                     err[r] = std::sqrt(hv[(size_t)r * dvs + sT]) / bnrm[r];
@@ -3180,11 +3207,12 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
                 for (int r : inner) hc[(size_t)r * cst] = 1.0 / snormw[r];
                 put_coef();
                 put(dact_b, cur_b, inner);
-                k_scalb<<<gy(inner.size()), 256, 0, c->s>>>(N, coef.p, cst, 0, vcol(ll), vst, dact_b.p);
+                k_scalt<<<gy(inner.size()), 256, 0, st>>>(T, R, V(ll), NV, coef.p, cst, 0, dact_b.p);
             }
         }
         // finish the cycle for every active run
-        std::vector<int32_t> need_z, add_z, still;
+        std::vector<int32_t> add_z, still;
+        std::map<int, std::vector<int32_t>> by_nn;
         for (int r : active) {
             if (early[r]) { iflag[r] = 0; continue; }
             if (stall[r]) { iflag[r] = 2; continue; }
@@ -3208,21 +3236,16 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
                     for (int i = 0; i < k; ++i) yls[i] += t * H[(size_t)k * ldh + i];
                 }
                 for (int k = 0; k < nn; ++k) hc[(size_t)r * cst + k] = yls[k];
-                need_z.push_back(r);
+                by_nn[nn].push_back(r);
             }
         }
-        if (!need_z.empty()) {
-            // all runs in need_z share lgmr except those that converged early in
-            // the cycle: group by basis size
+        if (!by_nn.empty()) {
             put_coef();
-            std::map<int, std::vector<int32_t>> by_nn;
-            for (int r : need_z) by_nn[lgmr[r]].push_back(r);
             for (auto &kv : by_nn) {
                 DBuf<int32_t> dg;
-                dg.upload(kv.second.data(), kv.second.size(), c->s);
-                k_lincombb<<<gy(kv.second.size()), 256, 0, c->s>>>(N, kv.first, coef.p, cst, V.p, vst,
-                                                                   N, zz.p, N, dg.p);
-                HIPCHK(hipStreamSynchronize(c->s));
+                dg.upload(kv.second.data(), kv.second.size(), st);
+                k_lincombt<<<gy(kv.second.size()), 256, 0, st>>>(T, R, V(0), kv.first, NV, coef.p, cst,
+                                                                 kSlotZ, dg.p);
             }
         }
         for (int r : active) {
@@ -3233,8 +3256,8 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
             for (int r : add_z) hc[(size_t)r * cst + maxl + 1] = 1.0;
             put_coef();
             put(dact_b, cur_b, add_z);
-            k_axpyb<<<gy(add_z.size()), 256, 0, c->s>>>(N, coef.p, cst, maxl + 1, 1.0, zz.p, N, dX, N,
-                                                       dact_b.p);
+            k_axpyt<<<gy(add_z.size()), 256, 0, st>>>(T, R, kSlotZ, kSlotX, NV, coef.p, cst, maxl + 1, 1.0,
+                                                     dact_b.p);
         }
         for (int r : active) {
             if (iflag[r] == 0) ierr[r] = 0;
@@ -3244,12 +3267,63 @@ void gmres_block_dev(tabi_ctx *c, int R, const double *dB, double *dX, double to
         active = still;
         ++nrsts;
     }
-    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipStreamSynchronize(st));
     for (int r = 0; r < R; ++r) {
         iter_out[r] = iter[r];
         err_out[r] = err[r];
         ierr_out[r] = ierr[r];
     }
+}
+
+// R right-hand sides on one surface: vectors stored with a common stride so the
+// block field and preconditioner kernels see them as block columns
+struct BlockRuns : RunSet {
+    tabi_ctx *c;
+    int64_t N;
+    DBuf<double> V, work;
+    std::vector<int64_t> stride;       // per slot: distance between runs
+    BlockRuns(tabi_ctx *c_, int R_, int maxl_, double *dB, double *dX) : c(c_)
+    {
+        R = R_;
+        maxl = maxl_;
+        s = c->s;
+        N = 2 * (int64_t)c->n;
+        n.assign(R, N);
+        const int64_t vst = (int64_t)(maxl + 1) * N;
+        V.alloc((size_t)R * vst);
+        work.alloc((size_t)R * N * 4);
+        htab.assign((size_t)nslots() * R, nullptr);
+        stride.assign(nslots(), N);
+        for (int r = 0; r < R; ++r) {
+            htab[(size_t)kSlotR * R + r] = work.p + (size_t)(0 * R + r) * N;
+            htab[(size_t)kSlotZ * R + r] = work.p + (size_t)(1 * R + r) * N;
+            htab[(size_t)kSlotWK * R + r] = work.p + (size_t)(2 * R + r) * N;
+            htab[(size_t)kSlotDL * R + r] = work.p + (size_t)(3 * R + r) * N;
+            htab[(size_t)kSlotB * R + r] = dB + (size_t)r * N;
+            htab[(size_t)kSlotX * R + r] = dX + (size_t)r * N;
+            for (int k = 0; k <= maxl; ++k)
+                htab[(size_t)(kSlotV0 + k) * R + r] = V.p + (size_t)r * vst + (size_t)k * N;
+        }
+        for (int k = 0; k <= maxl; ++k) stride[kSlotV0 + k] = vst;
+        upload_tables();
+    }
+    void matvec(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout) override
+    {
+        matvec_block_dev(c, (int)cols.size(), ptr(sin, 0), stride[sin], ptr(sout, 0), stride[sout], dcols);
+    }
+    void msolve(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout) override
+    {
+        msolve_block_dev(c, (int)cols.size(), ptr(sin, 0), stride[sin], ptr(sout, 0), stride[sout], dcols);
+    }
+};
+
+void gmres_block_dev(tabi_ctx *c, int R, double *dB, double *dX, double tol, int maxl, int nrmax,
+                     bool x0_zero, int *iter_out, double *err_out, int *ierr_out)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    if (maxl > N) maxl = (int)N;
+    BlockRuns rs(c, R, maxl, dB, dX);
+    gmres_runs(rs, c->blk_red, tol, nrmax, x0_zero, iter_out, err_out, ierr_out);
 }
 
 void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p, double *db)
@@ -3262,6 +3336,121 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
 }
+
+// Many surfaces, one context each (own tree, lists and factors), one GMRES run
+// per surface, all in lockstep: the batched vector operations run on a common
+// stream, and each step's matvec + preconditioner solve of every active surface
+// is replayed from one captured graph whose branches (one per surface, on the
+// surface's own stream) run concurrently, so many small surfaces fill the GPU
+// together.  The graph is captured again when the set of active surfaces changes.
+struct MultiRuns : RunSet {
+    std::vector<tabi_ctx *> cs;
+    std::vector<std::unique_ptr<DBuf<double>>> store;
+    bool use_graph = true;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    std::map<std::vector<int32_t>, hipGraphExec_t> graphs;
+    int64_t graph_launches = 0, captures = 0;
+
+    MultiRuns(std::vector<tabi_ctx *> cs_, int maxl_, hipStream_t main, bool graph,
+              const std::vector<double *> &dB, const std::vector<double *> &dX)
+        : cs(std::move(cs_)), use_graph(graph)
+    {
+        R = (int)cs.size();
+        s = main;
+        maxl = maxl_;
+        for (auto *c : cs) maxl = std::min<int64_t>(maxl, 2 * (int64_t)c->n);
+        n.resize(R);
+        htab.assign((size_t)nslots() * R, nullptr);
+        for (int r = 0; r < R; ++r) {
+            tabi_ctx *c = cs[r];
+            const int64_t N = 2 * (int64_t)c->n;
+            n[r] = N;
+            auto buf = std::make_unique<DBuf<double>>();
+            buf->alloc((size_t)N * (5 + maxl + 1));
+            double *b = buf->p;
+            htab[(size_t)kSlotR * R + r] = b;
+            htab[(size_t)kSlotZ * R + r] = b + N;
+            htab[(size_t)kSlotWK * R + r] = b + 2 * N;
+            htab[(size_t)kSlotDL * R + r] = b + 3 * N;
+            htab[(size_t)kSlotT2 * R + r] = b + 4 * N;
+            htab[(size_t)kSlotT1 * R + r] = c->vx.p;
+            htab[(size_t)kSlotB * R + r] = dB[r];
+            htab[(size_t)kSlotX * R + r] = dX[r];
+            for (int k = 0; k <= maxl; ++k) htab[(size_t)(kSlotV0 + k) * R + r] = b + (size_t)(5 + k) * N;
+            store.push_back(std::move(buf));
+        }
+        upload_tables();
+        HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        join.resize(R, nullptr);
+        for (auto &e : join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ~MultiRuns() override
+    {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (fork) (void)hipEventDestroy(fork);
+        for (auto e : join)
+            if (e) (void)hipEventDestroy(e);
+    }
+    // eager single-surface operations on the surfaces' own streams, ordered
+    // after the common stream's work and before its next work
+    void matvec(const std::vector<int32_t> &cols, const int32_t *, int sin, int sout) override
+    {
+        HIPCHK(hipStreamSynchronize(s));
+        for (int r : cols) matvec_dev(cs[r], ptr(sin, r), ptr(sout, r));
+        for (int r : cols) HIPCHK(hipStreamSynchronize(cs[r]->s));
+    }
+    void msolve(const std::vector<int32_t> &cols, const int32_t *, int sin, int sout) override
+    {
+        HIPCHK(hipStreamSynchronize(s));
+        for (int r : cols) msolve_dev(cs[r], ptr(sin, r), ptr(sout, r));
+        for (int r : cols) HIPCHK(hipStreamSynchronize(cs[r]->s));
+    }
+    // the step's work of one surface: vy = A vx, stage = M^-1 vy
+    void step(int r) { matvec_dev(cs[r], cs[r]->vx.p, cs[r]->vy.p); msolve_dev(cs[r], cs[r]->vy.p, ptr(kSlotT2, r)); }
+    hipGraphExec_t graph_for(const std::vector<int32_t> &cols)
+    {
+        auto it = graphs.find(cols);
+        if (it != graphs.end()) return it->second;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipEventRecord(fork, s));
+        for (int r : cols) {
+            HIPCHK(hipStreamWaitEvent(cs[r]->s, fork, 0));
+            step(r);
+            HIPCHK(hipEventRecord(join[r], cs[r]->s));
+            HIPCHK(hipStreamWaitEvent(s, join[r], 0));
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        ++captures;
+        graphs[cols] = ex;
+        return ex;
+    }
+    void apply(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout) override
+    {
+        int64_t nmax = 0;
+        for (int r : cols) nmax = std::max(nmax, n[r]);
+        const dim3 grid(nblk(nmax, 256), (unsigned)cols.size());
+        k_copyt<<<grid, 256, 0, s>>>(tab.p, R, sin, kSlotT1, dn.p, dcols);
+        if (use_graph) {
+            HIPCHK(hipGraphLaunch(graph_for(cols), s));
+            ++graph_launches;
+        } else {
+            HIPCHK(hipEventRecord(fork, s));
+            for (int r : cols) {
+                HIPCHK(hipStreamWaitEvent(cs[r]->s, fork, 0));
+                step(r);
+                HIPCHK(hipEventRecord(join[r], cs[r]->s));
+                HIPCHK(hipStreamWaitEvent(s, join[r], 0));
+            }
+        }
+        k_copyt<<<grid, 256, 0, s>>>(tab.p, R, kSlotT2, sout, dn.p, dcols);
+    }
+};
 
 // Right-hand sides of R charge sets (q[R][nq], host) at common positions
 // into dB[R][2n].
@@ -3534,6 +3723,7 @@ void tabi_ctx_destroy(tabi_ctx *c)
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     hipStream_t s = c->s;
+    if (s) (void)hipStreamSynchronize(s);  // pending work may still use the buffers
     delete c;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -3639,14 +3829,10 @@ int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qp
         if (nq < 1) throw Error("need at least one charge");
         const int64_t N = 2 * (int64_t)c->n;
         DBuf<double> dB, dX;
-        DBuf<int32_t> dall;
-        std::vector<int32_t> all(nrhs);
-        for (int k = 0; k < nrhs; ++k) all[k] = k;
-        dall.upload(all.data(), nrhs, c->s);
         dB.alloc((size_t)nrhs * N);
         dX.alloc((size_t)nrhs * N);
         rhs_block_dev(c, nrhs, nq, qpos, q, eps_p, dB.p);
-        k_fillb<<<dim3(nblk(N, 256), nrhs), 256, 0, c->s>>>(N, 0.0, dX.p, N, dall.p);
+        HIPCHK(hipMemsetAsync(dX.p, 0, sizeof(double) * nrhs * N, c->s));
         std::vector<int> it(nrhs), ie(nrhs);
         gmres_block_dev(c, nrhs, dB.p, dX.p, tol, 10, 10, true, it.data(), err, ie.data());
         energy_block_dev(c, nrhs, nq, qpos, q, eps_p, dX.p, soleng, couleng);
@@ -3656,6 +3842,106 @@ int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qp
         }
         for (int k = 0; k < nrhs; ++k) { iter[k] = it[k]; ierr[k] = ie[k]; }
     });
+}
+
+int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
+                    const double *const *xyz, const double *const *nrm,
+                    const double *const *area, const int32_t *nq, const double *const *qpos,
+                    const double *const *q, double eps_p, double tol, int32_t options,
+                    int32_t *iter, double *err, int32_t *ierr, double *soleng, double *couleng)
+{
+    return guarded([&] {
+        if (nsurf < 1) throw Error("need at least one surface");
+        for (int k = 0; k < nsurf; ++k)
+            if (nq[k] < 1) throw Error("surface " + std::to_string(k) + " has no charges");
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::unique_ptr<tabi_ctx>> ctx(nsurf);
+        std::vector<std::unique_ptr<DBuf<double>>> bx(nsurf);
+        std::vector<double *> dB(nsurf), dX(nsurf);
+        for (int k = 0; k < nsurf; ++k) {
+            ctx[k] = std::make_unique<tabi_ctx>();
+            bx[k] = std::make_unique<DBuf<double>>();
+        }
+        auto cleanup = [&]() {
+            for (auto &c : ctx)
+                if (c && c->s) {
+                    (void)hipStreamSynchronize(c->s);
+                    hipStream_t st = c->s;
+                    c->s = nullptr;
+                    (void)hipStreamDestroy(st);
+                }
+        };
+        // set-up and right-hand sides, several surfaces at a time from host threads
+        {
+            const int nth = std::max(1, std::min<int>(nsurf, 8));
+            std::vector<std::thread> th;
+            std::vector<std::string> errs(nth);
+            for (int t = 0; t < nth; ++t)
+                th.emplace_back([&, t] {
+                    try {
+                        for (int k = t; k < nsurf; k += nth) {
+                            tabi_ctx *c = ctx[k].get();
+                            ctx_setup(c, prm, nface[k], xyz[k], nrm[k], area[k], nullptr, nullptr);
+                            const int64_t N = 2 * (int64_t)nface[k];
+                            bx[k]->alloc((size_t)2 * N);
+                            rhs_dev(c, nq[k], qpos[k], q[k], eps_p, bx[k]->p);
+                            HIPCHK(hipMemsetAsync(bx[k]->p + N, 0, sizeof(double) * N, c->s));
+                            HIPCHK(hipStreamSynchronize(c->s));
+                        }
+                    } catch (const std::exception &e) {
+                        errs[t] = e.what();
+                    }
+                });
+            for (auto &x : th) x.join();
+            for (auto &e : errs)
+                if (!e.empty()) { cleanup(); throw Error(e); }
+        }
+        for (int k = 0; k < nsurf; ++k) {
+            dB[k] = bx[k]->p;
+            dX[k] = bx[k]->p + 2 * (int64_t)nface[k];
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        hipStream_t main = nullptr;
+        HIPCHK(hipStreamCreate(&main));
+        std::vector<int> it(nsurf), ie(nsurf);
+        try {
+            std::vector<tabi_ctx *> cp(nsurf);
+            for (int k = 0; k < nsurf; ++k) cp[k] = ctx[k].get();
+            DBuf<double> red;
+            {
+                MultiRuns rs(cp, 10, main, (options & 1) == 0, dB, dX);
+                gmres_runs(rs, red, tol, 10, true, it.data(), err, ie.data());
+                g_many[0] = (double)rs.captures;
+                g_many[1] = (double)rs.graph_launches;
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(main);
+            (void)hipStreamDestroy(main);
+            cleanup();
+            throw;
+        }
+        HIPCHK(hipStreamSynchronize(main));
+        (void)hipStreamDestroy(main);
+        auto t2 = std::chrono::steady_clock::now();
+        for (int k = 0; k < nsurf; ++k) {
+            energy_dev(ctx[k].get(), nq[k], qpos[k], q[k], eps_p, dX[k], soleng + k, couleng + k);
+            iter[k] = it[k];
+            ierr[k] = ie[k];
+        }
+        auto t3 = std::chrono::steady_clock::now();
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        g_many[2] = ms(t0, t1);
+        g_many[3] = ms(t1, t2);
+        g_many[4] = ms(t2, t3);
+        g_many[5] = ms(t0, t3);
+        cleanup();
+    });
+}
+
+int tabi_last_solve_many_stats(double *out6)
+{
+    for (int i = 0; i < 6; ++i) out6[i] = g_many[i];
+    return 0;
 }
 
 int tabi_release_memory(void)

@@ -25,7 +25,8 @@ EXPORTS = (
     "tabi_ws_param_count", "tabi_ws_create", "tabi_ws_destroy", "tabi_ws_get_params",
     "tabi_ctx_ws_features", "tabi_ws_predict", "tabi_ws_grad", "tabi_ws_train",
     "tabi_ctx_gmres_ws", "tabi_solve_ws", "tabi_ctx_matvec_block", "tabi_ctx_msolve_block",
-    "tabi_ctx_gmres_block", "tabi_ctx_solve_block",
+    "tabi_ctx_gmres_block", "tabi_ctx_solve_block", "tabi_solve_many",
+    "tabi_last_solve_many_stats",
 )
 
 SOLVE_PHASES = ("tree", "lists", "precond", "rhs", "gmres", "energy", "total", "warmstart")
@@ -453,3 +454,38 @@ def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4, model: WsModel 
     return dict(x=x, orderarr=order, iter=it.value, err=err.value, ierr=ierr.value,
                 soleng=sol.value, couleng=cou.value, ws_used=used.value,
                 times_ms=dict(zip(SOLVE_PHASES, t)))
+
+
+def solve_many(prm: Params, surfaces, eps_p, tol=1e-4, graph=True):
+    """Solves several surfaces together (tabi_solve_many).  surfaces: sequence
+    of (xyz, nrm, area, qpos, q) in original face order.  Returns a dict of
+    per-surface arrays (iter, err, ierr, soleng, couleng) and stats."""
+    lib = load()
+    k = len(surfaces)
+    keep = []
+    ptr = {name: (C.c_void_p * k)() for name in ("xyz", "nrm", "area", "qpos", "q")}
+    nface = np.zeros(k, dtype=np.int32)
+    nq = np.zeros(k, dtype=np.int32)
+    for i, (xyz, nrm, area, qpos, q) in enumerate(surfaces):
+        arrs = dict(xyz=_d(xyz, (-1, 3)), nrm=_d(nrm, (-1, 3)), area=_d(area),
+                    qpos=_d(qpos, (-1, 3)), q=_d(q))
+        if arrs["nrm"].shape[0] != arrs["xyz"].shape[0] or arrs["area"].shape[0] != arrs["xyz"].shape[0]:
+            raise TabiError(f"surface {i}: xyz, nrm and area disagree in length")
+        if arrs["q"].shape[0] != arrs["qpos"].shape[0]:
+            raise TabiError(f"surface {i}: qpos and q disagree in length")
+        keep.append(arrs)
+        nface[i] = arrs["xyz"].shape[0]
+        nq[i] = arrs["q"].shape[0]
+        for name, a in arrs.items():
+            ptr[name][i] = a.ctypes.data
+    it, ierr = np.zeros(k, dtype=np.int32), np.zeros(k, dtype=np.int32)
+    err, sol, cou = np.zeros(k), np.zeros(k), np.zeros(k)
+    _check(lib.tabi_solve_many(C.byref(prm), C.c_int32(k), _p(nface), ptr["xyz"], ptr["nrm"],
+                               ptr["area"], _p(nq), ptr["qpos"], ptr["q"], C.c_double(eps_p),
+                               C.c_double(tol), C.c_int32(0 if graph else 1), _p(it), _p(err),
+                               _p(ierr), _p(sol), _p(cou)))
+    st = np.zeros(6)
+    lib.tabi_last_solve_many_stats(_p(st))
+    return dict(iter=it, err=err, ierr=ierr, soleng=sol, couleng=cou,
+                stats=dict(zip(("captures", "graph_launches", "setup_ms", "gmres_ms", "energy_ms",
+                                "total_ms"), st)))
