@@ -126,11 +126,9 @@ def cpu_baseline(prob, iters, budget_s=12.0):
                 sample=sample, t_matvec_s=t_mv, t_msolve_s=t_ms, t_solve_s=t_solve)
 
 
-def batched_charge_sets(nsets=64, natom=1500, precision=1):
-    """Many charge sets on one surface: the 1500-atom synthetic molecule (seed 1)
-    with nsets charge vectors at the same atom positions (N(0, 0.35) per atom,
-    shifted to net -3 e), solved as one block (one tree pass per GMRES step for
-    all sets) and as nsets single solves on the same set-up surface."""
+def charge_set_surface(natom=1500, nsets=64):
+    """The 1500-atom synthetic molecule (seed 1) and nsets charge vectors at its
+    atom positions (N(0, 0.35) per atom, shifted to net -3 e); CPU only."""
     from tabipb import surface
     xyz, q, r = surface.synth_molecule(natom, seed=1)
     v, nv, f = surface.ses_mesh(xyz, r, probe=1.4, h=0.5)
@@ -142,9 +140,15 @@ def batched_charge_sets(nsets=64, natom=1500, precision=1):
     Q -= (Q.sum(1, keepdims=True) + 3.0) / natom
     Q = Q.astype(np.float32).astype(np.float64)
     usr = meshio.UsrData(maxparnode=500, order=3, mac=0.8)
-    prm = hip.make_params(usr.epsw / usr.epsp, driver.kappa_of(usr.bulk_strength, usr.epsw),
-                          usr.mac, usr.order, usr.maxparnode, precision)
-    S = hip.Surface(prm, fx, fn, fa)
+    return driver.Problem(usr, fx, fn, fa, qpos, Q[0]), Q
+
+
+def batched_charge_sets(prob, Q, precision=1):
+    """Many charge sets on one surface, solved as one block (one tree pass per
+    GMRES step for all sets) and as single solves on the same set-up surface."""
+    usr, nsets = prob.usr, Q.shape[0]
+    S = hip.Surface(prob.params(precision), prob.xyz, prob.nrm, prob.area)
+    qpos = prob.qpos
     S.solve_block(qpos, Q[:8], usr.epsp, want_x=False)       # warm-up
     t0 = time.perf_counter()
     blk = S.solve_block(qpos, Q, usr.epsp, want_x=False)
@@ -159,14 +163,59 @@ def batched_charge_sets(nsets=64, natom=1500, precision=1):
     t_seq = time.perf_counter() - t0
     S.close()
     sols = np.array(sols)
-    return dict(workload=f"synthetic {natom}-atom molecule (seed 1), {len(f)} faces, "
-                         f"{nsets} charge sets", faces=int(len(f)), sets=nsets,
+    n = prob.xyz.shape[0]
+    return dict(workload=f"synthetic {prob.qpos.shape[0]}-atom molecule (seed 1), {n} faces, "
+                         f"{nsets} charge sets", faces=int(n), sets=nsets,
                 precision="mixed" if precision == 1 else "fp64",
                 block_ms=t_blk * 1e3, sequential_ms=t_seq * 1e3, speedup=t_seq / t_blk,
                 sets_per_s_block=nsets / t_blk, sets_per_s_sequential=nsets / t_seq,
                 iters_equal=bool(np.array_equal(blk["iter"], its)),
                 max_dG_rel_diff=float(np.max(np.abs(blk["soleng"] - sols) / np.abs(sols))),
                 mean_iters=float(np.mean(its)))
+
+
+def many_molecules(nmol=48, lo=200, hi=4000, workers=12):
+    """nmol seeded synthetic molecules of lo..hi atoms (log-uniform), meshed in
+    worker processes; CPU only."""
+    seeds = list(range(700000, 700000 + nmol))
+    sizes = warmstart.molecule_sizes(seeds, lo=lo, hi=hi, salt=2)
+    return list(warmstart.molecules(seeds, sizes, workers=workers))
+
+
+def packed_surfaces(mols, precision=1):
+    """The molecules solved together (tabi_solve_many: threaded set-up, lockstep
+    GMRES with one captured graph per step) and one after another."""
+    prm = warmstart.params_for(precision)
+    surfs = [(m["xyz"], m["nrm"], m["area"], m["qpos"], m["q"]) for m in mols]
+    hip.solve_many(prm, surfs[:4], warmstart.EPS_P)             # warm-up
+    t0 = time.perf_counter()
+    res = hip.solve_many(prm, surfs, warmstart.EPS_P, tol=1e-4, graph=True)
+    t_many = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    single = [hip.solve(prm, *s, warmstart.EPS_P, tol=1e-4) for s in surfs]
+    t_seq = time.perf_counter() - t0
+    sol1 = np.array([r["soleng"] for r in single])
+    faces = [int(m["xyz"].shape[0]) for m in mols]
+    return dict(workload=f"{len(mols)} synthetic molecules of {min(m['natom'] for m in mols)}-"
+                         f"{max(m['natom'] for m in mols)} atoms, {min(faces)}-{max(faces)} faces "
+                         f"({sum(faces)} in total)",
+                precision="mixed" if precision == 1 else "fp64",
+                together_ms=t_many * 1e3, sequential_ms=t_seq * 1e3, speedup=t_seq / t_many,
+                solves_per_hour_together=3600.0 * len(mols) / t_many,
+                iters_equal=bool(np.array_equal(res["iter"], [r["iter"] for r in single])),
+                max_dG_rel_diff=float(np.max(np.abs(res["soleng"] - sol1) / np.abs(sol1))),
+                graph=res["stats"])
+
+
+def autotuned(prob):
+    """The autotuner's sweep (leaf 100-1000, p 2-6, two task granularities) on
+    the charge-set surface, mixed precision, dG within 1e-3 of FP64."""
+    from tabipb import autotune
+    r = autotune.autotune(prob, task_exponents=(0, 4), repeats=1)
+    default = [t for t in r["table"] if t["leaf"] == 500 and t["order"] == 3 and t["task_exp"] == 0][0]
+    return dict(best=r["best"], default=default, speedup_vs_default=default["ms"] / r["best"]["ms"],
+                candidates=len(r["table"]), rejected=sum(not t["ok"] for t in r["table"]),
+                err_target=r["err_target"])
 
 
 def main():
@@ -184,6 +233,11 @@ def main():
               "(the solve would shard as one all-gather of x per matvec).", file=sys.stderr)
         sys.exit(2)
     prob, desc = workload(a.workload)
+    # CPU preparation of the secondary workloads, before the GPU is used
+    cs_prob = cs_Q = mols = None
+    if not a.no_batch:
+        cs_prob, cs_Q = charge_set_surface()
+        mols = many_molecules()
     p = prob.params().order
     print(f"[bench] workload {desc['workload']} n={prob.xyz.shape[0]}", file=sys.stderr, flush=True)
 
@@ -254,10 +308,14 @@ def main():
     print(f"[bench] matvec {mv_ms:.3f} ms (fp64 {mv64_ms:.3f} ms); kernel avg ms {avg}; "
           f"msolve {msolve_ms:.3f} ms; stats {st}", file=sys.stderr, flush=True)
 
-    batch = None
+    batch = packed = tuned = None
     if not a.no_batch:
-        batch = batched_charge_sets()
+        batch = batched_charge_sets(cs_prob, cs_Q)
         print(f"[bench] batched charge sets {batch}", file=sys.stderr, flush=True)
+        packed = packed_surfaces(mols)
+        print(f"[bench] many surfaces {packed}", file=sys.stderr, flush=True)
+        tuned = autotuned(cs_prob)
+        print(f"[bench] autotune {tuned}", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu:
@@ -290,7 +348,7 @@ def main():
                        matvec_ms=mv_ms, fp64_matvec_ms=mv64_ms, msolve_ms=msolve_ms,
                        matvecs_per_s=1e3 / mv_ms,
                        phases_ms={k: float(v) for k, v in res["times_ms"].items()},
-                       batched_charge_sets=batch),
+                       batched_charge_sets=batch, many_surfaces=packed, autotune=tuned),
         "roofline": {"bound": "valu", "kernel": ("k_" + dom + ("32" if prec == 1 else "")),
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,

@@ -36,7 +36,9 @@ typedef struct {
     int32_t flags;       /* bit 0: preconditioner with matvec-consistent L2/L3
                             placement instead of the reference's (tc.f:1481-1488);
                             bit 1: build the tree with the sequential host
-                            partition instead of the GPU one (same result) */
+                            partition instead of the GPU one (same result);
+                            bits 8-11 (t): field kernels cut the work into about
+                            2^(10+t) tasks per launch (t = 0: 2^16) */
 } tabi_params;
 
 typedef struct tabi_ctx tabi_ctx;

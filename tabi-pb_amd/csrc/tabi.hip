@@ -2462,7 +2462,11 @@ void build_lists(tabi_ctx *c)
     c->p_near -= c->n;  // self pairs are excluded
 
     // task granularity: enough waves to fill the device several times over
-    const int64_t target = 65536;
+    // work per task: the near source runs and far item runs are cut so that
+    // each launch has about `target` tasks (flags bits 8-11 override the
+    // default of 2^16 with 2^(10+t), t = 1..15)
+    const int tt = (c->prm.flags >> 8) & 15;
+    const int64_t target = tt ? (int64_t)1 << (10 + tt) : 65536;
     int S = 64;
     while (S < 2048 && near_src_total / S > target) S *= 2;
     int F = 1;
