@@ -217,7 +217,7 @@ struct tabi_ctx {
     std::vector<int64_t> h_blk_off;
     std::vector<int32_t> h_blk_m;
     DBuf<int64_t> blk_off;
-    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info;
+    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info, lorder;
     DBuf<double> lu;
     // vectors
     DBuf<double> vx, vy, red;
@@ -1078,7 +1078,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
 // of one active block: the touched rows (the panel rows and the pivot rows
 // below them) are staged in LDS with coalesced loads, the interchanges and the
 // unit-lower solve run there, and the rows are written back.
-__global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__restrict__ lorder,
+__global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     const int32_t *__restrict__ piv_off, double *__restrict__ lu, const int32_t *__restrict__ piv)
 {
@@ -1087,11 +1087,11 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
     __shared__ int rows[2 * kLuNB];
     __shared__ int slot_of_piv[kLuNB];
     __shared__ int s_nr;
-    const int l = lorder[blockIdx.y];
+    const int2 tl = tiles[blockIdx.x];           // {block, column tile}
+    const int l = tl.x;
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
-    const int j0 = blockIdx.x * 64;
-    if (j0 >= m) return;
+    const int j0 = tl.y * 64;
     double *A = lu + blk_off[l];
     const int32_t *pv = piv + piv_off[l];
     const int t = threadIdx.x;
@@ -1123,9 +1123,12 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
             slot_of_piv[t] = sl;
         }
     }
-    for (int e = t; e < nbe * nbe; e += 256) {
-        const int r = e % nbe, c = e / nbe;
-        L11[r][c] = A[(size_t)(k0 + c) * m + k0 + r];
+    const bool right = j0 + 63 >= k0 + nbe;    // the tile holds columns right of the panel
+    if (right) {
+        for (int e = t; e < nbe * nbe; e += 256) {
+            const int r = e % nbe, c = e / nbe;
+            L11[r][c] = A[(size_t)(k0 + c) * m + k0 + r];
+        }
     }
     __syncthreads();
     const int nr = s_nr;
@@ -1134,7 +1137,9 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
         const bool col_ok = j < m && (j < k0 || j >= k0 + nbe);
         T[q][jj] = col_ok ? A[(size_t)j * m + rows[q]] : 0.0;
     }
-    __syncthreads();
+    __syncthreads();    // every staged row is in LDS before the column sweeps read it
+    // interchanges, then U12 = L11^-1 A12 for the columns right of the panel
+    // (column sweeps); tiles left of the panel only take the interchanges
     if (t < 64) {
         const int j = j0 + t;
         if (j < m && (j < k0 || j >= k0 + nbe)) {
@@ -1146,12 +1151,12 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int32_t *__r
                     T[sl][t] = tmp;
                 }
             }
-            if (j >= k0 + nbe) {
-                for (int c = 1; c < nbe; ++c) {
-                    double sacc = T[c][t];
-                    for (int c2 = 0; c2 < c; ++c2) sacc = fma(-L11[c][c2], T[c2][t], sacc);
-                    T[c][t] = sacc;
-                }
+        }
+        if (right && j < m && j >= k0 + nbe) {
+            for (int c = 1; c < nbe; ++c) {
+                double sacc = T[c][t];
+                for (int c2 = 0; c2 < c; ++c2) sacc = fma(-L11[c][c2], T[c2][t], sacc);
+                T[c][t] = sacc;
             }
         }
     }
@@ -1184,13 +1189,27 @@ __global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int4 *__restrict_
         sU[rr][cc] = (c0 + cc < m) ? A[(size_t)(c0 + cc) * m + k0 + rr] : 0.0;
     }
     __syncthreads();
-    const int tr = (t % 16) * 4, tc = (t / 16) * 4;
+    // thread (tr, tc) owns rows tr + 16a and columns tc + 16b: consecutive
+    // threads touch consecutive rows of a column
+    const int tr = t % 16, tc = t / 16;
+    // the trailing entries are loaded before the product so their latency
+    // overlaps the arithmetic
+    double cv[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int col = c0 + tc + 16 * b;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int row = r0 + tr + 16 * a;
+            cv[a][b] = (row < m && col < m) ? A[(size_t)col * m + row] : 0.0;
+        }
+    }
     double acc[4][4] = {};
 #pragma unroll 4
     for (int c = 0; c < nbe; ++c) {
         double lv[4], uv[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { lv[q] = sL[c][tr + q]; uv[q] = sU[c][tc + q]; }
+        for (int q = 0; q < 4; ++q) { lv[q] = sL[c][tr + 16 * q]; uv[q] = sU[c][tc + 16 * q]; }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1198,12 +1217,12 @@ __global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int4 *__restrict_
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const int col = c0 + tc + b;
+        const int col = c0 + tc + 16 * b;
         double *Ac = A + (size_t)col * m;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const int row = r0 + tr + a;
-            if (row < m && col < m) Ac[row] -= acc[a][b];
+            const int row = r0 + tr + 16 * a;
+            if (row < m && col < m) Ac[row] = cv[a][b] - acc[a][b];
         }
     }
 }
@@ -1228,11 +1247,11 @@ __global__ void k_lu_perm(int nleaf, const int32_t *__restrict__ blk_m,
 }
 
 // z = M^-1 r for every leaf block with the cached factors: permuted gather, then
-// blocked forward (unit lower) and back (upper) substitution.  One wavefront per
-// leaf: each 64x64 diagonal tile is held row-per-lane in registers and solved
-// with shuffles; the rows outside the tile are updated 64 at a time, each lane
-// issuing the tile's 64 column loads back to back so the factor stream keeps
-// many loads in flight.  z lives in LDS.
+// blocked forward (unit lower) and back (upper) substitution.  Each 64x64
+// diagonal tile is held row-per-lane in registers and solved with shuffles; the
+// rows outside the tile are updated 64 at a time, each lane issuing the tile's
+// 64 column loads back to back so the factor stream keeps many loads in
+// flight.  z lives in LDS.
 constexpr int kSolveB = 64;
 constexpr int kHalf = 32;
 
@@ -1255,70 +1274,79 @@ __device__ __forceinline__ double gemv_row(const double *__restrict__ Ac, int m,
     return s + s1;
 }
 
-__global__ void __launch_bounds__(64) k_lu_solve(int n, const int32_t *__restrict__ blk_ibeg,
-    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    const int32_t *__restrict__ piv_off, const double *__restrict__ lu,
-    const int32_t *__restrict__ perm, const double *__restrict__ r, double *__restrict__ z)
+// NW waves per leaf: wave 0 solves each 64x64 diagonal tile, and the row
+// updates outside the tile (the bulk of the factor stream) are shared by all NW
+// waves, so a large leaf is streamed NW times faster than by one wave.  Leaves
+// are visited in the order lorder (largest first) so the longest solves start
+// first.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__restrict__ lorder,
+    const int32_t *__restrict__ blk_ibeg, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
+    const double *__restrict__ lu, const int32_t *__restrict__ perm, const double *__restrict__ r,
+    double *__restrict__ z)
 {
     extern __shared__ double zs[];
-    const int l = blockIdx.x, lane = threadIdx.x;
+    const int l = lorder[blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
     const double *A = lu + blk_off[l];
     const int32_t *pm = perm + piv_off[l];
-    for (int k = lane; k < m; k += kWave) {
+    for (int k = t; k < m; k += 64 * NW) {
         const int src = pm[k];
         zs[k] = src < nl ? r[ib + src] : r[n + ib + src - nl];
     }
     __syncthreads();
     const int nb = (m + kSolveB - 1) / kSolveB;
-    // forward substitution, L unit lower
     for (int bi = 0; bi < nb; ++bi) {
         const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
-        const bool in = lane < bs;
-        double zl = in ? zs[kb + lane] : 0.0;
+        if (w == 0) {
+            const bool in = lane < bs;
+            double zl = in ? zs[kb + lane] : 0.0;
 #pragma unroll 1
-        for (int h = 0; h < kSolveB; h += kHalf) {
-            double tr[kHalf];
+            for (int h = 0; h < kSolveB; h += kHalf) {
+                double tr[kHalf];
 #pragma unroll
-            for (int c = 0; c < kHalf; ++c)
-                tr[c] = (in && h + c < lane) ? A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                for (int c = 0; c < kHalf; ++c)
+                    tr[c] = (in && h + c < lane) ? A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
 #pragma unroll
-            for (int c = 0; c < kHalf; ++c) zl -= tr[c] * __shfl(zl, h + c);
+                for (int c = 0; c < kHalf; ++c) zl -= tr[c] * __shfl(zl, h + c);
+            }
+            if (in) zs[kb + lane] = zl;
         }
-        if (in) zs[kb + lane] = zl;
         __syncthreads();
-        for (int r0 = kb + bs; r0 < m; r0 += kWave) {
+        for (int r0 = kb + bs + 64 * w; r0 < m; r0 += 64 * NW) {
             const int row = r0 + lane;
             if (row < m) zs[row] = gemv_row(A + (size_t)kb * m + row, m, bs, zs + kb, zs[row]);
         }
         __syncthreads();
     }
-    // back substitution, U upper with diagonal
     for (int bi = nb - 1; bi >= 0; --bi) {
         const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
-        const bool in = lane < bs;
-        double zl = in ? zs[kb + lane] : 0.0;
+        if (w == 0) {
+            const bool in = lane < bs;
+            double zl = in ? zs[kb + lane] : 0.0;
 #pragma unroll 1
-        for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
-            double tr[kHalf];
+            for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
+                double tr[kHalf];
 #pragma unroll
-            for (int c = 0; c < kHalf; ++c)
-                tr[c] = (in && h + c >= lane && h + c < bs) ? A[(size_t)(kb + h + c) * m + kb + lane]
-                                                           : 0.0;
+                for (int c = 0; c < kHalf; ++c)
+                    tr[c] = (in && h + c >= lane && h + c < bs) ? A[(size_t)(kb + h + c) * m + kb + lane]
+                                                               : 0.0;
 #pragma unroll
-            for (int c = kHalf - 1; c >= 0; --c) {
-                if (lane == h + c && in) zl = zl / tr[c];
-                const double zc = __shfl(zl, h + c);
-                if (lane < h + c) zl -= tr[c] * zc;
+                for (int c = kHalf - 1; c >= 0; --c) {
+                    if (lane == h + c && in) zl = zl / tr[c];
+                    const double zc = __shfl(zl, h + c);
+                    if (lane < h + c) zl -= tr[c] * zc;
+                }
             }
+            if (in) zs[kb + lane] = zl;
         }
-        if (in) zs[kb + lane] = zl;
         __syncthreads();
-        for (int r0 = 0; r0 < kb; r0 += kWave)
+        for (int r0 = 64 * w; r0 < kb; r0 += 64 * NW)
             zs[r0 + lane] = gemv_row(A + (size_t)kb * m + r0 + lane, m, bs, zs + kb, zs[r0 + lane]);
         __syncthreads();
     }
-    for (int q = lane; q < nl; q += kWave) {
+    for (int q = t; q < nl; q += 64 * NW) {
         z[ib + q] = zs[q];
         z[n + ib + q] = zs[nl + q];
     }
@@ -2260,8 +2288,8 @@ void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
 {
     const size_t shm = sizeof(double) * (size_t)c->mmax;
     KTimer kt(c, 4);
-    k_lu_solve<<<c->nleaf, kWave, shm, c->s>>>(c->n, c->blk_ibeg.p, c->blk_m.p, c->blk_off.p,
-        c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
+    k_lu_solve<4><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
+        c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
     HIPCHK(hipGetLastError());
 }
 
@@ -2580,10 +2608,15 @@ void build_precond(tabi_ctx *c)
     std::vector<int> nact;
     std::vector<int4> tl;
     std::vector<int64_t> tl_off(1, 0);
+    std::vector<int2> sw;                       // {block, column tile} of the interchange pass
+    std::vector<int64_t> sw_off(1, 0);
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB) {
         int na = 0;
         while (na < nl && bm[lorder[na]] > k0) ++na;
         nact.push_back(na);
+        for (int q = 0; q < na; ++q)
+            for (int ct = 0; ct * 64 < bm[lorder[q]]; ++ct) sw.push_back(make_int2(lorder[q], ct));
+        sw_off.push_back((int64_t)sw.size());
         for (int q = 0; q < na; ++q) {
             const int rem = bm[lorder[q]] - k0 - kLuNB;
             if (rem <= 0) continue;
@@ -2593,9 +2626,11 @@ void build_precond(tabi_ctx *c)
         }
         tl_off.push_back((int64_t)tl.size());
     }
-    DBuf<int32_t> dlorder;
     DBuf<int4> dtl;
-    dlorder.upload(lorder.data(), nl, c->s);
+    DBuf<int2> dsw;
+    dsw.upload(sw.data(), std::max<size_t>(sw.size(), 1), c->s);
+    c->lorder.upload(lorder.data(), nl, c->s);
+    const int32_t *dlorder_p = c->lorder.p;
     if (!tl.empty()) dtl.upload(tl.data(), tl.size(), c->s);
     int step = 0;
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB, ++step) {
@@ -2611,21 +2646,22 @@ void build_precond(tabi_ctx *c)
             while (e < na && bm[lorder[e]] - k0 > lo) ++e;
             const int cnt = e - q;
             if (cls == 2048)
-                k_lu_panel<2, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                k_lu_panel<2, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
                     c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
             else if (cls == 1024)
-                k_lu_panel<1, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                k_lu_panel<1, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
                     c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
             else if (cls == 512)
-                k_lu_panel<1, 512><<<cnt, 512, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                k_lu_panel<1, 512><<<cnt, 512, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
                     c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
             else
-                k_lu_panel<1, 256><<<cnt, 256, 0, c->s>>>(k0, q, dlorder.p, c->blk_m.p,
+                k_lu_panel<1, 256><<<cnt, 256, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
                     c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
             q = e;
         }
-        k_lu_swap_trsm<<<dim3(nblk(bm[lorder[0]], 64), na), 256, 0, c->s>>>(k0, dlorder.p,
-            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
+        const int64_t nsw = sw_off[step + 1] - sw_off[step];
+        k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
+            c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
         if (nt > 0)
             k_lu_gemm<<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,

@@ -246,6 +246,26 @@ def test_gpu_tree_build_matches_host_build_and_oracle(oracle_lib):
         Sh.close()
 
 
+def test_leaf_factorisation_is_deterministic():
+    """Two contexts of the same surface factor every leaf block identically (the
+    batched LU runs many workgroups per block and step; a missing barrier or an
+    unordered update would show up as differing preconditioner results), and a
+    repeated preconditioner solve is bit-identical."""
+    from tabipb import meshio
+    v, nv, f = meshio.icosphere(6, 2.0)
+    xyz, nrm, area = driver.face_geometry(v, nv, f)
+    prm = hip.make_params(80.0, 0.1, 0.8, 3, 500)
+    x = np.random.default_rng(1).standard_normal(2 * xyz.shape[0])
+    S1 = hip.Surface(prm, xyz, nrm, area)
+    z1 = S1.msolve(x)
+    assert np.array_equal(S1.msolve(x), z1)
+    for _ in range(2):
+        S2 = hip.Surface(prm, xyz, nrm, area)
+        assert np.array_equal(S2.msolve(x), z1)
+        S2.close()
+    S1.close()
+
+
 # ---------------------------------------------------------------------------
 # mixed precision (precision=1): FP32 near/far field with compensated sums,
 # FP64 GMRES, leaf LU, self term and energy.  Tolerances: matvec rel. L2 <= 1e-5
