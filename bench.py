@@ -37,6 +37,13 @@ FP32_PEAK_TFLOPS = 157.3
 METRIC = "PB solves/hour at ~500k faces, ΔG err<1e-3 vs FP64; GMRES iters learned vs 0"
 
 
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs of `bench.py --kernels-only 5` on the default workload,
+# each counter scaled by what it reads for the matvec's 7,737,152-byte copy of
+# x (FETCH_SIZE x 1.994, WRITE_SIZE x 1.000): {(kernel, faces): bytes}
+TRAFFIC = {("k_near32", 483572): 503.4e6 + 139.8e6}
+
+
 def w_far(p):
     """Algorithmic flops per accepted (target, cluster) pair."""
     return 20 * comb(p + 5, 3) + 64 * comb(p + 3, 3) + 40
@@ -227,12 +234,25 @@ def main():
     ap.add_argument("--matvecs", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--kernels-only", type=int, default=0, metavar="K",
+                    help="only K matvecs + preconditioner solves on the workload "
+                         "(mixed precision), for counter collection")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the solve would shard as one all-gather of x per matvec).", file=sys.stderr)
         sys.exit(2)
     prob, desc = workload(a.workload)
+    if a.kernels_only:
+        S = hip.Surface(prob.params(precision=1), prob.xyz, prob.nrm, prob.area)
+        dx = hip.DeviceArray.from_host(np.random.default_rng(0).standard_normal(2 * S.n))
+        dy = hip.DeviceArray(2 * S.n)
+        for _ in range(a.kernels_only):
+            S.matvec_dev(dx.ptr, dy.ptr)
+            S.msolve_dev(dy.ptr, dx.ptr)
+        S.sync()
+        print(f"[bench] {a.kernels_only} matvec + msolve on {S.n} faces", file=sys.stderr, flush=True)
+        return
     # CPU preparation of the secondary workloads, before the GPU is used
     cs_prob = cs_Q = mols = None
     if not a.no_batch:
@@ -351,7 +371,9 @@ def main():
                        batched_charge_sets=batch, many_surfaces=packed, autotune=tuned),
         "roofline": {"bound": "valu", "kernel": ("k_" + dom + ("32" if prec == 1 else "")),
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": TRAFFIC.get(("k_" + dom + ("32" if prec == 1 else ""),
+                                             int(prob.xyz.shape[0]))),
                      "alg_flops_per_launch": flops[dom], "avg_launch_ms": avg[dom]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
