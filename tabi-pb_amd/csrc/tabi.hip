@@ -170,6 +170,22 @@ void device_free(void *p) { DevicePool::get().free(p); }
 
 namespace {
 
+// An input array as a device pointer: device inputs are used in place, host
+// inputs are uploaded on the given stream.
+struct DevIn {
+    DBuf<double> buf;
+    const double *p = nullptr;
+    DevIn(const double *src, size_t n, bool dev, hipStream_t s)
+    {
+        if (dev) {
+            p = src;
+        } else {
+            buf.upload(src, std::max<size_t>(n, 1), s);
+            p = buf.p;
+        }
+    }
+};
+
 struct NearTask {
     int32_t g, s0, s1, leaf;  // target group, source range [s0,s1), leaf node
     u64 mask;
@@ -2708,7 +2724,8 @@ __global__ void k_gather_geom(int n, const int32_t *__restrict__ order,
 }
 
 void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz,
-               const double *nrm, const double *area, int32_t *orderarr_out, double *t_phase)
+               const double *nrm, const double *area, int32_t *orderarr_out, double *t_phase,
+               bool dev = false)
 {
     if (!prm) throw Error("null parameters");
     if (n < 1) throw Error("need at least one face");
@@ -2723,14 +2740,20 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     c->ngroups = (n + kWave - 1) / kWave;
     if (!c->s) HIPCHK(hipStreamCreate(&c->s));
     auto t0 = std::chrono::steady_clock::now();
-    DBuf<double> d_xyz, d_nrm, d_area;
+    // inputs in original face order, host or device
+    DevIn d_xyz(xyz, (size_t)3 * n, dev, c->s), d_nrm(nrm, (size_t)3 * n, dev, c->s),
+        d_area(area, n, dev, c->s);
     DBuf<int32_t> d_ord;
-    d_xyz.upload(xyz, (size_t)3 * n, c->s);
-    d_nrm.upload(nrm, (size_t)3 * n, c->s);
-    d_area.upload(area, n, c->s);
     c->tx.alloc(n); c->ty.alloc(n); c->tz.alloc(n);
     d_ord.alloc(n);
     if (prm->flags & 2) {
+        std::vector<double> hxyz;
+        if (dev) {
+            hxyz.resize((size_t)3 * n);
+            HIPCHK(hipMemcpyAsync(hxyz.data(), xyz, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            xyz = hxyz.data();
+        }
         std::vector<double> x(n), y(n), z(n);
         for (int i = 0; i < n; ++i) { x[i] = xyz[3 * i]; y[i] = xyz[3 * i + 1]; z[i] = xyz[3 * i + 2]; }
         tabi::build_tree_host(n, x.data(), y.data(), z.data(), prm->maxparnode, c->tree);
@@ -3366,11 +3389,10 @@ void gmres_block_dev(tabi_ctx *c, int R, double *dB, double *dX, double tol, int
     gmres_runs(rs, c->blk_red, tol, nrmax, x0_zero, iter_out, err_out, ierr_out);
 }
 
-void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p, double *db)
+void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p, double *db,
+             bool dev = false)
 {
-    DBuf<double> dq, dqp;
-    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
-    dq.upload(q, std::max(nq, 1), c->s);
+    DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
     k_rhs<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, nq, dqp.p, dq.p, eps_p, c->tx.p, c->ty.p,
         c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, db);
     HIPCHK(hipGetLastError());
@@ -3558,13 +3580,11 @@ void energy_block_dev(tabi_ctx *c, int R, int nq, const double *qpos, const doub
 // Warm-start features of this surface for the charges (rhs db already formed);
 // returns the charge scale s = 1/rms(b_1..b_n).
 double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
-                       const double *db, float *dfeat)
+                       const double *db, float *dfeat, bool dev = false)
 {
     const double bb = dev_dot(c, c->n, db, db);
     const double scale = bb > 0.0 ? 1.0 / std::sqrt(bb / c->n) : 1.0;
-    DBuf<double> dqp, dq;
-    dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
-    dq.upload(q, std::max(nq, 1), c->s);
+    DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
     tabi::WsFeatIn in;
     in.n = c->n;
     in.tx = c->tx.p; in.ty = c->ty.p; in.tz = c->tz.p;
@@ -3582,14 +3602,13 @@ double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *q,
 }
 
 void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
-                const double *dxs, double *sol, double *cou)
+                const double *dxs, double *sol, double *cou, bool dev = false)
 {
     KTimer kt(c, 7);
     const double para = 332.0716;  // src/main.f90:64
     if (nq <= 0) { *sol = 0.0; *cou = 0.0; return; }
-    DBuf<double> dqp, dq, dptl, part, cpart;
-    dqp.upload(qpos, (size_t)3 * nq, c->s);
-    dq.upload(q, nq, c->s);
+    DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
+    DBuf<double> dptl, part, cpart;
     dptl.alloc(nq);
     const int ngq = (nq + kWave - 1) / kWave;
     const int nch = (c->n + kEnergyChunk - 1) / kEnergyChunk;
@@ -3599,16 +3618,15 @@ void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double
                                          c->prm.kappa, part.p);
     k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, nch, part.p, dptl.p);
     const int ncb = (nq + 255) / 256;
-    cpart.alloc(ncb);
+    cpart.alloc(ncb + 1);
     k_coulomb<<<ncb, 256, 0, c->s>>>(nq, dqp.p, dq.p, cpart.p);
+    k_sum_part<<<1, 256, 0, c->s>>>(ncb, cpart.p, cpart.p + ncb);
     HIPCHK(hipGetLastError());
-    std::vector<double> ptl(nq), cp(ncb);
-    HIPCHK(hipMemcpyAsync(ptl.data(), dptl.p, sizeof(double) * nq, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(cp.data(), cpart.p, sizeof(double) * ncb, hipMemcpyDeviceToHost, c->s));
+    // sum_k q_k * potential_k (src/main.f90:148,156)
+    const double s = dev_dot(c, nq, dq.p, dptl.p);
+    double cc = 0.0;
+    HIPCHK(hipMemcpyAsync(&cc, cpart.p + ncb, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-    double s = 0.0, cc = 0.0;
-    for (int k = 0; k < nq; ++k) s += q[k] * ptl[k];
-    for (int b = 0; b < ncb; ++b) cc += cp[b];
     *sol = s * 0.5 * para * 4.0 * kPi;
     *cou = cc / eps_p * para;
 }
@@ -4081,43 +4099,50 @@ int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const doubl
                          orderarr_out, iter, err, ierr, soleng, couleng, nullptr);
 }
 
-int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
-                  const double *area, int32_t nq, const double *qpos, const double *q,
-                  double eps_p, double tol, tabi_ws *model, double *x_out,
-                  int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
-                  double *soleng, double *couleng, int32_t *ws_used)
+}  // extern "C"
+
+namespace {
+
+// One full solve; with dev the geometry, the charges and x_out are device
+// pointers (inputs already resident), otherwise host pointers.
+void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+                const double *area, int32_t nq, const double *qpos, const double *q, double eps_p,
+                double tol, tabi_ws *model, double *x_out, int32_t *orderarr_out, int32_t *iter,
+                double *err, int32_t *ierr, double *soleng, double *couleng, int32_t *ws_used,
+                bool dev)
 {
-    return guarded([&] {
-        if (model && !model->m) throw Error("null warm-start model");
-        auto t0 = std::chrono::steady_clock::now();
-        tabi_ctx c;
+    if (model && !model->m) throw Error("null warm-start model");
+    if (nq < 1) throw Error("need at least one charge");
+    auto t0 = std::chrono::steady_clock::now();
+    tabi_ctx c;
+    try {
         double ph[3] = {0, 0, 0};
-        ctx_setup(&c, prm, n, xyz, nrm, area, orderarr_out, ph);
+        ctx_setup(&c, prm, n, xyz, nrm, area, orderarr_out, ph, dev);
         auto t1 = std::chrono::steady_clock::now();
         const int64_t N = 2 * (int64_t)n;
-        DBuf<double> db, dxs;
+        DBuf<double> db, dxs_own;
         db.alloc(N);
-        dxs.alloc(N);
-        rhs_dev(&c, nq, qpos, q, eps_p, db.p);
-        HIPCHK(hipMemsetAsync(dxs.p, 0, sizeof(double) * N, c.s));
+        double *dxs = (dev && x_out) ? x_out : (dxs_own.alloc(N), dxs_own.p);
+        rhs_dev(&c, nq, qpos, q, eps_p, db.p, dev);
+        HIPCHK(hipMemsetAsync(dxs, 0, sizeof(double) * N, c.s));
         auto t2 = std::chrono::steady_clock::now();
         if (model) {
             DBuf<float> feat;
             feat.alloc((size_t)n * tabi::ws::kF);
-            const double scale = ws_features_dev(&c, nq, qpos, q, eps_p, db.p, feat.p);
-            tabi::ws_predict_dev(model->m, n, feat.p, scale, dxs.p, c.s);
+            const double scale = ws_features_dev(&c, nq, qpos, q, eps_p, db.p, feat.p, dev);
+            tabi::ws_predict_dev(model->m, n, feat.p, scale, dxs, c.s);
             HIPCHK(hipStreamSynchronize(c.s));
         }
         auto t2b = std::chrono::steady_clock::now();
         int it = 0, ie = 0, used = 0;
-        gmres_dev(&c, db.p, dxs.p, tol, 10, 10, &it, err, &ie, model != nullptr, &used);
+        gmres_dev(&c, db.p, dxs, tol, 10, 10, &it, err, &ie, model != nullptr, &used);
         auto t3 = std::chrono::steady_clock::now();
-        energy_dev(&c, nq, qpos, q, eps_p, dxs.p, soleng, couleng);
+        energy_dev(&c, nq, qpos, q, eps_p, dxs, soleng, couleng, dev);
         auto t4 = std::chrono::steady_clock::now();
-        if (x_out) {
-            HIPCHK(hipMemcpyAsync(x_out, dxs.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.s));
-            HIPCHK(hipStreamSynchronize(c.s));
+        if (x_out && !dev) {
+            HIPCHK(hipMemcpyAsync(x_out, dxs, sizeof(double) * N, hipMemcpyDeviceToHost, c.s));
         }
+        HIPCHK(hipStreamSynchronize(c.s));
         *iter = it;
         *ierr = ie;
         if (ws_used) *ws_used = used;
@@ -4125,9 +4150,44 @@ int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const do
         g_times[0] = ph[0]; g_times[1] = ph[1]; g_times[2] = ph[2];
         g_times[3] = ms(t1, t2); g_times[4] = ms(t2b, t3); g_times[5] = ms(t3, t4);
         g_times[6] = ms(t0, t4); g_times[7] = ms(t2, t2b);
-        hipStream_t s = c.s;
-        c.s = nullptr;
-        (void)hipStreamDestroy(s);
+    } catch (...) {
+        if (c.s) {
+            (void)hipStreamSynchronize(c.s);
+            (void)hipStreamDestroy(c.s);
+            c.s = nullptr;
+        }
+        throw;
+    }
+    hipStream_t s = c.s;
+    c.s = nullptr;
+    (void)hipStreamDestroy(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const double *nrm,
+                  const double *area, int32_t nq, const double *qpos, const double *q,
+                  double eps_p, double tol, tabi_ws *model, double *x_out,
+                  int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
+                  double *soleng, double *couleng, int32_t *ws_used)
+{
+    return guarded([&] {
+        solve_core(prm, n, xyz, nrm, area, nq, qpos, q, eps_p, tol, model, x_out, orderarr_out,
+                   iter, err, ierr, soleng, couleng, ws_used, false);
+    });
+}
+
+int tabi_solve_dev(const tabi_params *prm, int32_t n, const double *d_xyz, const double *d_nrm,
+                   const double *d_area, int32_t nq, const double *d_qpos, const double *d_q,
+                   double eps_p, double tol, tabi_ws *model, double *d_x_out,
+                   int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
+                   double *soleng, double *couleng, int32_t *ws_used)
+{
+    return guarded([&] {
+        solve_core(prm, n, d_xyz, d_nrm, d_area, nq, d_qpos, d_q, eps_p, tol, model, d_x_out,
+                   orderarr_out, iter, err, ierr, soleng, couleng, ws_used, true);
     });
 }
 

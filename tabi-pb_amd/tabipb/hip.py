@@ -26,7 +26,7 @@ EXPORTS = (
     "tabi_ctx_ws_features", "tabi_ws_predict", "tabi_ws_grad", "tabi_ws_train",
     "tabi_ctx_gmres_ws", "tabi_solve_ws", "tabi_ctx_matvec_block", "tabi_ctx_msolve_block",
     "tabi_ctx_gmres_block", "tabi_ctx_solve_block", "tabi_solve_many",
-    "tabi_last_solve_many_stats",
+    "tabi_last_solve_many_stats", "tabi_solve_dev",
 )
 
 SOLVE_PHASES = ("tree", "lists", "precond", "rhs", "gmres", "energy", "total", "warmstart")
@@ -489,3 +489,39 @@ def solve_many(prm: Params, surfaces, eps_p, tol=1e-4, graph=True):
     return dict(iter=it, err=err, ierr=ierr, soleng=sol, couleng=cou,
                 stats=dict(zip(("captures", "graph_launches", "setup_ms", "gmres_ms", "energy_ms",
                                 "total_ms"), st)))
+
+
+class DeviceProblem:
+    """A surface and its charges held in device memory (original face order),
+    for solves whose inputs are already resident (tabi_solve_dev)."""
+
+    def __init__(self, xyz, nrm, area, qpos, q):
+        self.n = int(_d(xyz, (-1, 3)).shape[0])
+        self.nq = int(_d(q).shape[0])
+        self.xyz = DeviceArray.from_host(_d(xyz, (-1, 3)))
+        self.nrm = DeviceArray.from_host(_d(nrm, (-1, 3)))
+        self.area = DeviceArray.from_host(_d(area))
+        self.qpos = DeviceArray.from_host(_d(qpos, (-1, 3)))
+        self.q = DeviceArray.from_host(_d(q))
+        self.x = DeviceArray(2 * self.n)
+
+    def solve(self, prm: Params, eps_p, tol=1e-4, model: WsModel = None):
+        """One solve; the tree-order solution stays on the device (self.x)."""
+        lib = load()
+        it, err, ierr = C.c_int32(), C.c_double(), C.c_int32()
+        sol, cou, used = C.c_double(), C.c_double(), C.c_int32()
+        _check(lib.tabi_solve_dev(C.byref(prm), C.c_int32(self.n), C.c_void_p(self.xyz.ptr),
+                                  C.c_void_p(self.nrm.ptr), C.c_void_p(self.area.ptr),
+                                  C.c_int32(self.nq), C.c_void_p(self.qpos.ptr),
+                                  C.c_void_p(self.q.ptr), C.c_double(eps_p), C.c_double(tol),
+                                  model._h if model is not None else None,
+                                  C.c_void_p(self.x.ptr), None, C.byref(it), C.byref(err),
+                                  C.byref(ierr), C.byref(sol), C.byref(cou), C.byref(used)))
+        t = np.zeros(8)
+        lib.tabi_last_solve_times(_p(t))
+        return dict(iter=it.value, err=err.value, ierr=ierr.value, soleng=sol.value,
+                    couleng=cou.value, ws_used=used.value, times_ms=dict(zip(SOLVE_PHASES, t)))
+
+    def free(self):
+        for a in (self.xyz, self.nrm, self.area, self.qpos, self.q, self.x):
+            a.free()

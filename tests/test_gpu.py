@@ -246,6 +246,21 @@ def test_gpu_tree_build_matches_host_build_and_oracle(oracle_lib):
         Sh.close()
 
 
+def test_device_resident_solve_equals_host_solve():
+    """tabi_solve_dev (inputs already in device memory) and tabi_solve give the
+    same iterations, energies and solution, for both precisions."""
+    P = problem("sph3_l100_p3")
+    D = hip.DeviceProblem(P.xyz, P.nrm, P.area, P.qpos, P.q)
+    for precision in (0, 1):
+        prm = P.params(precision=precision)
+        rh = hip.solve(prm, P.xyz, P.nrm, P.area, P.qpos, P.q, P.usr.epsp, 1e-6)
+        rd = D.solve(prm, P.usr.epsp, 1e-6)
+        assert rd["iter"] == rh["iter"] and rd["ierr"] == rh["ierr"] == 0
+        assert rd["soleng"] == rh["soleng"] and rd["couleng"] == rh["couleng"]
+        assert np.array_equal(D.x.get(), rh["x"])
+    D.free()
+
+
 def test_leaf_factorisation_is_deterministic():
     """Two contexts of the same surface factor every leaf block identically (the
     batched LU runs many workgroups per block and step; a missing barrier or an
