@@ -184,15 +184,6 @@ int tabi_solve(const tabi_params *prm, int32_t n, const double *xyz, const doubl
                int32_t *iter, double *err, int32_t *ierr, double *soleng,
                double *couleng);
 
-/* tabi_solve_ws with every array already resident in device memory: d_xyz,
- * d_nrm, d_area, d_qpos, d_q and d_x_out (may be NULL) are device pointers
- * (original face order in, tree order out); orderarr_out stays a host array. */
-int tabi_solve_dev(const tabi_params *prm, int32_t n, const double *d_xyz, const double *d_nrm,
-                   const double *d_area, int32_t nq, const double *d_qpos, const double *d_q,
-                   double eps_p, double tol, tabi_ws *model, double *d_x_out,
-                   int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
-                   double *soleng, double *couleng, int32_t *ws_used);
-
 /* Per-phase device times (ms) of the last tabi_solve / tabi_solve_ws on this
  * thread: {tree, lists, precond_factor, rhs, gmres, energy, total, warmstart}. */
 int tabi_last_solve_times(double *out8);
@@ -248,6 +239,15 @@ int tabi_solve_ws(const tabi_params *prm, int32_t n, const double *xyz, const do
                   double eps_p, double tol, tabi_ws *model, double *x_out,
                   int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
                   double *soleng, double *couleng, int32_t *ws_used);
+
+/* tabi_solve_ws with every array already resident in device memory: d_xyz,
+ * d_nrm, d_area, d_qpos, d_q and d_x_out (may be NULL) are device pointers
+ * (original face order in, tree order out); orderarr_out stays a host array. */
+int tabi_solve_dev(const tabi_params *prm, int32_t n, const double *d_xyz, const double *d_nrm,
+                   const double *d_area, int32_t nq, const double *d_qpos, const double *d_q,
+                   double eps_p, double tol, tabi_ws *model, double *d_x_out,
+                   int32_t *orderarr_out, int32_t *iter, double *err, int32_t *ierr,
+                   double *soleng, double *couleng, int32_t *ws_used);
 
 #ifdef __cplusplus
 }

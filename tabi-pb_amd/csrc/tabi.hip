@@ -2816,187 +2816,6 @@ void ctx_stats(tabi_ctx *c, int64_t *o)
     o[9] = c->lu.n * (int64_t)sizeof(double);
 }
 
-// GMRES with left preconditioning, ITOL=2, JSCAL=0, KMP=MAXL (DGMRES/DPIGMR,
-// src/dgmres_dep.f:689-852,1652-1863): modified Gram-Schmidt with the
-// conditional re-orthogonalisation of DORTH (1376-1411), Givens QR of the
-// Hessenberg matrix (DHEQR, 1093-1125), residual vector for the stopping test
-// and restarts from the Krylov basis (DRLCAL, 1948-1977), least squares update
-// (DHELS, 925-947).  Vectors stay on the device; only scalars come back.
-void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl, int nrmax,
-               int *iter_out, double *err_out, int *ierr_out, bool fallback = false,
-               int *x0_used = nullptr)
-{
-    const int64_t N = 2 * (int64_t)c->n;
-    if (maxl > N) maxl = (int)N;
-    const int ldh = maxl + 1;
-    DBuf<double> V, r, z, wk, dl, coef;
-    V.alloc((size_t)(maxl + 1) * N);
-    r.alloc(N); z.alloc(N); wk.alloc(N); dl.alloc(N);
-    coef.alloc(maxl + 2);
-    auto vcol = [&](int k) { return V.p + (size_t)k * N; };
-    const unsigned nb = nblk(N, 256);
-    std::vector<double> hes((size_t)ldh * maxl), q(2 * maxl), yls(maxl + 1), cf(maxl + 2);
-    auto nrm2 = [&](const double *v) { return std::sqrt(dev_dot(c, N, v, v)); };
-    // is the vector numerically zero (sum |x| < 1e-10, src/main.f90:478)?
-    auto matvec_guarded = [&](const double *in, double *out) {
-        matvec_dev(c, in, out);
-    };
-    // initial residual; x0 = 0 short-circuits the matvec
-    msolve_dev(c, db, r.p);
-    const double bnrm = nrm2(r.p);
-    // sum|x0| test on the host copy is done by the caller: x0 == 0 -> r = b
-    std::vector<double> hx(N);
-    HIPCHK(hipMemcpyAsync(hx.data(), dxs, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    double sabs = 0.0;
-    for (int64_t i = 0; i < N; ++i) sabs += std::fabs(hx[i]);
-    const bool x0_zero = sabs < 1.0e-10;
-    if (x0_zero) {
-        HIPCHK(hipMemcpyAsync(r.p, dxs, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
-    } else {
-        matvec_guarded(dxs, r.p);
-    }
-    k_sub<<<nb, 256, 0, c->s>>>(N, db, r.p, r.p);
-    if (x0_used) *x0_used = x0_zero ? 0 : 1;
-    int iter = 0, ierr = 0, nrsts = 0;
-    double err = 0.0;
-    for (;;) {
-        if (nrsts > nrmax) { ierr = 1; break; }
-        if (nrsts > 0) HIPCHK(hipMemcpyAsync(r.p, dl.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
-        HIPCHK(hipMemsetAsync(z.p, 0, sizeof(double) * N, c->s));
-        int iflag = 0, lgmr = 0;
-        if (nrsts == 0) {
-            HIPCHK(hipMemcpyAsync(wk.p, r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
-            msolve_dev(c, wk.p, r.p);
-        }
-        HIPCHK(hipMemcpyAsync(vcol(0), r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
-        double r0nrm = nrm2(vcol(0));
-        if (nrsts == 0 && fallback && !x0_zero && !(r0nrm <= bnrm)) {
-            // the initial guess is worse than none: restart from x0 = 0
-            HIPCHK(hipMemsetAsync(dxs, 0, sizeof(double) * N, c->s));
-            msolve_dev(c, db, r.p);
-            HIPCHK(hipMemcpyAsync(vcol(0), r.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
-            r0nrm = bnrm;
-            if (x0_used) *x0_used = 0;
-        }
-        err = r0nrm / bnrm;
-        bool done_early = err <= tol;
-        if (!done_early) {
-            k_scal<<<nb, 256, 0, c->s>>>(N, 1.0 / r0nrm, vcol(0));
-            std::fill(hes.begin(), hes.end(), 0.0);
-            double prod = 1.0, rho = 0.0, snormw = 0.0;
-            bool converged = false, stalled = false;
-            for (int ll = 1; ll <= maxl; ++ll) {
-                lgmr = ll;
-                matvec_guarded(vcol(ll - 1), wk.p);
-                msolve_dev(c, wk.p, vcol(ll));
-                double *w = vcol(ll);
-                double *hcol = &hes[(size_t)(ll - 1) * ldh];
-                const double vnrm = nrm2(w);
-                for (int i = 0; i < ll; ++i) {
-                    hcol[i] = dev_dot(c, N, vcol(i), w);
-                    k_axpy<<<nb, 256, 0, c->s>>>(N, -hcol[i], vcol(i), w);
-                }
-                snormw = nrm2(w);
-                if (!(vnrm + 0.001 * snormw != vnrm)) {
-                    double sumdsq = 0.0;
-                    for (int i = 0; i < ll; ++i) {
-                        const double tem = -dev_dot(c, N, vcol(i), w);
-                        if (hcol[i] + 0.001 * tem == hcol[i]) continue;
-                        hcol[i] = hcol[i] - tem;
-                        k_axpy<<<nb, 256, 0, c->s>>>(N, tem, vcol(i), w);
-                        sumdsq += tem * tem;
-                    }
-                    if (sumdsq != 0.0) snormw = std::sqrt(std::max(0.0, snormw * snormw - sumdsq));
-                }
-                hcol[ll] = snormw;
-                // DHEQR update of the new column
-                for (int k = 0; k < ll - 1; ++k) {
-                    const double cc = q[2 * k], ss = q[2 * k + 1];
-                    const double t1 = hcol[k], t2 = hcol[k + 1];
-                    hcol[k] = cc * t1 - ss * t2;
-                    hcol[k + 1] = ss * t1 + cc * t2;
-                }
-                {
-                    const double t1 = hcol[ll - 1], t2 = hcol[ll];
-                    double cc, ss;
-                    if (t2 == 0.0) { cc = 1.0; ss = 0.0; }
-                    else if (std::fabs(t2) >= std::fabs(t1)) {
-                        const double t = t1 / t2;
-                        ss = -1.0 / std::sqrt(1.0 + t * t);
-                        cc = -ss * t;
-                    } else {
-                        const double t = t2 / t1;
-                        cc = 1.0 / std::sqrt(1.0 + t * t);
-                        ss = -cc * t;
-                    }
-                    q[2 * (ll - 1)] = cc;
-                    q[2 * (ll - 1) + 1] = ss;
-                    hcol[ll - 1] = cc * t1 - ss * t2;
-                    if (hcol[ll - 1] == 0.0) { stalled = true; break; }
-                }
-                prod = prod * q[2 * ll - 1];
-                rho = std::fabs(prod * r0nrm);
-                // DRLCAL: rl = r0nrm*prod*(S..S v1 + ... + C_ll/snormw v_{ll+1})
-                {
-                    for (int k = 0; k <= ll; ++k) cf[k] = 0.0;
-                    cf[0] = 1.0;
-                    for (int i = 1; i <= ll - 1; ++i) {
-                        const double ss = q[2 * i - 1], cc = q[2 * i - 2];
-                        for (int k = 0; k < i; ++k) cf[k] *= ss;
-                        cf[i] = cc;
-                    }
-                    const double ss = q[2 * ll - 1], cc = q[2 * ll - 2] / snormw;
-                    for (int k = 0; k < ll; ++k) cf[k] *= ss;
-                    cf[ll] = cc;
-                    const double tem = r0nrm * prod;
-                    for (int k = 0; k <= ll; ++k) cf[k] *= tem;
-                    HIPCHK(hipMemcpyAsync(coef.p, cf.data(), sizeof(double) * (ll + 1), hipMemcpyHostToDevice, c->s));
-                    k_lincomb<<<nb, 256, 0, c->s>>>(N, ll + 1, coef.p, V.p, N, dl.p);
-                }
-                err = nrm2(dl.p) / bnrm;
-                if (err <= tol) { converged = true; break; }
-                if (ll == maxl) break;
-                k_scal<<<nb, 256, 0, c->s>>>(N, 1.0 / snormw, w);
-            }
-            if (stalled) {
-                iflag = 2;
-            } else {
-                if (!converged) iflag = (rho < r0nrm) ? 1 : 2;
-                if (iflag != 2) {
-                    // dl already holds the residual for lgmr == maxl (restart vector)
-                    const int nn = lgmr;
-                    std::fill(yls.begin(), yls.end(), 0.0);
-                    yls[0] = r0nrm;
-                    for (int k = 0; k < nn; ++k) {
-                        const double cc = q[2 * k], ss = q[2 * k + 1];
-                        const double t1 = yls[k], t2 = yls[k + 1];
-                        yls[k] = cc * t1 - ss * t2;
-                        yls[k + 1] = ss * t1 + cc * t2;
-                    }
-                    for (int kb = 0; kb < nn; ++kb) {
-                        const int k = nn - 1 - kb;
-                        yls[k] = yls[k] / hes[(size_t)k * ldh + k];
-                        const double t = -yls[k];
-                        for (int i = 0; i < k; ++i) yls[i] += t * hes[(size_t)k * ldh + i];
-                    }
-                    HIPCHK(hipMemcpyAsync(coef.p, yls.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->s));
-                    k_lincomb<<<nb, 256, 0, c->s>>>(N, nn, coef.p, V.p, N, z.p);
-                }
-            }
-        }
-        iter += lgmr;
-        if (iflag != 2) k_axpy<<<nb, 256, 0, c->s>>>(N, 1.0, z.p, dxs);
-        if (iflag == 0) { ierr = 0; break; }
-        if (iflag == 1) { ++nrsts; continue; }
-        ierr = 2;
-        break;
-    }
-    HIPCHK(hipStreamSynchronize(c->s));
-    *iter_out = iter;
-    *err_out = err;
-    *ierr_out = ierr;
-}
 
 // ---- R runs of GMRES in lockstep -----------------------------------------
 // Slots of the per-run vector table: fixed work vectors, then the Krylov basis.
@@ -3036,7 +2855,7 @@ struct RunSet {
 // Hessenberg matrix, rotations and stopping state.  The B and X slots hold the
 // right-hand sides and the initial guesses / solutions.
 void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_zero, int *iter_out,
-                double *err_out, int *ierr_out)
+                double *err_out, int *ierr_out, bool fallback = false, int *x0_used = nullptr)
 {
     const int R = rs.R, maxl = rs.maxl, ldh = maxl + 1;
     hipStream_t st = rs.s;
@@ -3085,6 +2904,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
     dot(all, dact_a.p, kSlotR, kSlotR, sT);
     fetch();
     std::vector<double> bnrm(R), r0nrm(R), err(R, 0.0), prod(R), rho(R), snormw(R), vnrm(R);
+    std::vector<char> guessed(R, 0);        // run started from a nonzero x0
     std::vector<int> iter(R, 0), ierr(R, 0), lgmr(R), iflag(R);
     std::vector<char> conv(R), stall(R), early(R);
     std::vector<std::vector<double>> hes(R, std::vector<double>((size_t)ldh * maxl)),
@@ -3113,6 +2933,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
             put(dact_b, cur_b, nonzero);
             rs.matvec(nonzero, dact_b.p, kSlotX, kSlotR);
         }
+        for (int r : nonzero) guessed[r] = 1;
         k_subt<<<gy(R), 256, 0, st>>>(T, R, kSlotB, kSlotR, kSlotR, NV, dact_a.p);
     }
     std::vector<int32_t> active = all;
@@ -3133,9 +2954,22 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
         k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotR, V(0), NV, dact_a.p);
         dot(active, dact_a.p, V(0), V(0), sT);
         fetch();
+        for (int r : active) r0nrm[r] = std::sqrt(hv[(size_t)r * dvs + sT]);
+        if (nrsts == 0 && fallback) {
+            // an initial guess worse than none: those runs restart from x0 = 0
+            std::vector<int32_t> fb;
+            for (int r : active)
+                if (guessed[r] && !(r0nrm[r] <= bnrm[r])) fb.push_back(r);
+            if (!fb.empty()) {
+                put(dact_b, cur_b, fb);
+                k_fillt<<<gy(fb.size()), 256, 0, st>>>(T, R, kSlotX, NV, 0.0, dact_b.p);
+                rs.msolve(fb, dact_b.p, kSlotB, kSlotR);
+                k_copyt<<<gy(fb.size()), 256, 0, st>>>(T, R, kSlotR, V(0), NV, dact_b.p);
+                for (int r : fb) { r0nrm[r] = bnrm[r]; guessed[r] = 0; }
+            }
+        }
         std::vector<int32_t> inner;
         for (int r : active) {
-            r0nrm[r] = std::sqrt(hv[(size_t)r * dvs + sT]);
             err[r] = r0nrm[r] / bnrm[r];
             early[r] = err[r] <= tol;
             lgmr[r] = 0;
@@ -3335,6 +3169,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
         iter_out[r] = iter[r];
         err_out[r] = err[r];
         ierr_out[r] = ierr[r];
+        if (x0_used) x0_used[r] = guessed[r];
     }
 }
 
@@ -3513,6 +3348,56 @@ struct MultiRuns : RunSet {
         k_copyt<<<grid, 256, 0, s>>>(tab.p, R, kSlotT2, sout, dn.p, dcols);
     }
 };
+
+// GMRES with left preconditioning, ITOL=2, JSCAL=0, KMP=MAXL (DGMRES/DPIGMR,
+// src/dgmres_dep.f:689-852,1652-1863): modified Gram-Schmidt with the
+// conditional re-orthogonalisation of DORTH (1376-1411), Givens QR of the
+// Hessenberg matrix (DHEQR, 1093-1125), residual vector for the stopping test
+// and restarts from the Krylov basis (DRLCAL, 1948-1977), least squares update
+// (DHELS, 925-947).  Vectors stay on the device; only scalars come back.
+struct SingleRun : RunSet {
+    tabi_ctx *c;
+    DBuf<double> store;
+    SingleRun(tabi_ctx *c_, int maxl_, const double *dB, double *dX) : c(c_)
+    {
+        R = 1;
+        maxl = maxl_;
+        s = c->s;
+        const int64_t N = 2 * (int64_t)c->n;
+        n.assign(1, N);
+        store.alloc((size_t)N * (4 + maxl + 1));
+        htab.assign(nslots(), nullptr);
+        htab[kSlotR] = store.p;
+        htab[kSlotZ] = store.p + N;
+        htab[kSlotWK] = store.p + 2 * N;
+        htab[kSlotDL] = store.p + 3 * N;
+        htab[kSlotB] = const_cast<double *>(dB);   // read only
+        htab[kSlotX] = dX;
+        for (int k = 0; k <= maxl; ++k) htab[kSlotV0 + k] = store.p + (size_t)(4 + k) * N;
+        upload_tables();
+    }
+    void matvec(const std::vector<int32_t> &, const int32_t *, int sin, int sout) override
+    {
+        matvec_dev(c, ptr(sin, 0), ptr(sout, 0));
+    }
+    void msolve(const std::vector<int32_t> &, const int32_t *, int sin, int sout) override
+    {
+        msolve_dev(c, ptr(sin, 0), ptr(sout, 0));
+    }
+};
+
+// One GMRES run on the context's stream (gmres_runs with R = 1); with fallback,
+// an initial guess whose preconditioned residual exceeds that of x0 = 0 is
+// dropped and *x0_used reports whether the guess was kept.
+void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl, int nrmax,
+               int *iter_out, double *err_out, int *ierr_out, bool fallback = false,
+               int *x0_used = nullptr)
+{
+    const int64_t N = 2 * (int64_t)c->n;
+    if (maxl > N) maxl = (int)N;
+    SingleRun rs(c, maxl, db, dxs);
+    gmres_runs(rs, c->blk_red, tol, nrmax, false, iter_out, err_out, ierr_out, fallback, x0_used);
+}
 
 // Right-hand sides of R charge sets (q[R][nq], host) at common positions
 // into dB[R][2n].

@@ -88,3 +88,18 @@ def test_library_missing_raises(monkeypatch):
     monkeypatch.setattr(hip, "LIB_PATH", "/nonexistent/libtabi_hip.so")
     with pytest.raises(hip.TabiError):
         hip.load()
+
+
+def test_header_compiles_as_c_and_cxx(tmp_path):
+    """include/tabi_hip.h is consumed by C, C++ and Fortran drivers: it must
+    stand alone in both C and C++."""
+    import shutil
+    import subprocess
+    hdr = os.path.join(ROOT, "include", "tabi_hip.h")
+    for cc, lang in (("gcc", "c"), ("g++", "c++")):
+        if shutil.which(cc) is None:
+            pytest.skip(f"{cc} not available")
+        src = tmp_path / f"t.{'c' if lang == 'c' else 'cpp'}"
+        src.write_text(f'#include "{hdr}"\nint main(void) {{ return 0; }}\n')
+        subprocess.run([cc, "-std=c11" if lang == "c" else "-std=c++17", "-Wall", "-Werror",
+                        "-fsyntax-only", str(src)], check=True)
