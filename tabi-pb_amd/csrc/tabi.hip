@@ -226,7 +226,7 @@ struct tabi_ctx {
     int32_t n_near_task = 0, n_far_task = 0;
     DBuf<NearTask> near_tasks;
     DBuf<FarTask> far_tasks;
-    DBuf<int32_t> near_task_off, far_task_off;
+    DBuf<int32_t> near_tgt, near_slot_off, near_slot, far_task_off;
     DBuf<double2> part_near, part_far;
     // preconditioner
     int32_t nleaf = 0, mmax = 0;
@@ -684,7 +684,8 @@ __device__ __forceinline__ PairL pair_l(double dx, double dy, double dz, double 
     return o;
 }
 
-__global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks, int ntask, int n,
+__global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks, 
+    const int32_t *__restrict__ near_tgt, int ntask, int n,
     const double *__restrict__ src8, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
@@ -694,9 +695,9 @@ __global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks,
     if (task >= ntask) return;
     const NearTask tk = tasks[task];
     const int lane = lane_id();
-    const int i = tk.g * kWave + lane;
-    const bool act = (tk.mask >> lane) & 1ull;
-    const bool valid = i < n;
+    const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
+    const bool act = i >= 0;
+    const bool valid = act;
     const double sx = valid ? tx[i] : 0.0, sy = valid ? ty[i] : 0.0, sz = valid ? tz[i] : 0.0;
     const double v0x = valid ? tnx[i] : 0.0, v0y = valid ? tny[i] : 0.0,
                  v0z = valid ? tnz[i] : 0.0;
@@ -766,7 +767,8 @@ __global__ void k_src32_weights(int n, const double *__restrict__ tarea,
     src32[(size_t)j * 8 + 7] = (float)(a * x[n + j]);
 }
 
-__global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ tasks, int ntask,
+__global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ tasks, 
+    const int32_t *__restrict__ near_tgt, int ntask,
     int n, const float *__restrict__ src32, const double4 *__restrict__ ngeo,
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, const double *__restrict__ tnx,
@@ -777,9 +779,9 @@ __global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ task
     if (task >= ntask) return;
     const NearTask tk = tasks[task];
     const int lane = lane_id();
-    const int i = tk.g * kWave + lane;
-    const bool act = (tk.mask >> lane) & 1ull;
-    const bool valid = i < n;
+    const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
+    const bool act = i >= 0;
+    const bool valid = act;
     const double4 lg = ngeo[tk.leaf];
     const float sx = valid ? (float)(tx[i] - lg.x) : 0.f, sy = valid ? (float)(ty[i] - lg.y) : 0.f,
                 sz = valid ? (float)(tz[i] - lg.z) : 0.f;
@@ -884,19 +886,20 @@ __global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__res
     if (i < n) b[i] = (float)a[i];
 }
 
-// y = pre*x - sum of the group's task partials, near tasks then far tasks
-// (TREE_COMPP_PB's combination, src/treecode3d_pb.f:1154-1155,1177-1178)
-__global__ void k_reduce(int n, const int32_t *__restrict__ near_task_off,
-    const int32_t *__restrict__ far_task_off, const double2 *__restrict__ part_near,
-    const double2 *__restrict__ part_far, const double *__restrict__ x, double pre1,
-    double pre2, double *__restrict__ y)
+// y = pre*x - sum of the target's partials, near-field slots (in the fixed
+// order of its CSR row) then the far tasks of its group (TREE_COMPP_PB's
+// combination, src/treecode3d_pb.f:1154-1155,1177-1178)
+__global__ void k_reduce(int n, const int32_t *__restrict__ near_slot_off,
+    const int32_t *__restrict__ near_slot, const int32_t *__restrict__ far_task_off,
+    const double2 *__restrict__ part_near, const double2 *__restrict__ part_far,
+    const double *__restrict__ x, double pre1, double pre2, double *__restrict__ y)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int g = i / kWave, lane = i % kWave;
     double p1 = 0.0, p2 = 0.0;
-    for (int t = near_task_off[g]; t < near_task_off[g + 1]; ++t) {
-        const double2 v = part_near[(size_t)t * kWave + lane];
+    for (int s = near_slot_off[i]; s < near_slot_off[i + 1]; ++s) {
+        const double2 v = part_near[near_slot[s]];
         p1 += v.x; p2 += v.y;
     }
     for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
@@ -1396,10 +1399,10 @@ __global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double *__rest
         }
         __syncthreads();
         const int cnt = min(256, nq - k0);
+#pragma unroll 4
         for (int c = 0; c < cnt; ++c) {
             const double dx = sq[0][c] - px, dy = sq[1][c] - py, dz = sq[2][c] - pz;
-            const double rs = sqrt(dx * dx + dy * dy + dz * dz);
-            const double inv = 1.0 / rs;
+            const double inv = rsq_nr(dx * dx + dy * dy + dz * dz);
             const double ct = (vx * dx + vy * dy + vz * dz) * inv;
             b1 += sq[3][c] * inv;
             b2 += sq[3][c] * ct * inv * inv;
@@ -1583,7 +1586,8 @@ __global__ void k_src_weightsb(int n, int R, int Rp, const double *__restrict__ 
 }
 
 template <int RC>
-__global__ void __launch_bounds__(64) k_nearb(const NearTask *__restrict__ tasks, int ntask, int n,
+__global__ void __launch_bounds__(64) k_nearb(const NearTask *__restrict__ tasks, 
+    const int32_t *__restrict__ near_tgt, int ntask, int n,
     int R, int Rp, const double *__restrict__ src8, const double *__restrict__ wb,
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
@@ -1594,9 +1598,9 @@ __global__ void __launch_bounds__(64) k_nearb(const NearTask *__restrict__ tasks
     const int r0 = blockIdx.y * RC;
     const NearTask tk = tasks[task];
     const int lane = lane_id();
-    const int i = tk.g * kWave + lane;
-    const bool act = (tk.mask >> lane) & 1ull;
-    const bool valid = i < n;
+    const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
+    const bool act = i >= 0;
+    const bool valid = act;
     const double sx = valid ? tx[i] : 0.0, sy = valid ? ty[i] : 0.0, sz = valid ? tz[i] : 0.0;
     const double v0x = valid ? tnx[i] : 0.0, v0y = valid ? tny[i] : 0.0,
                  v0z = valid ? tnz[i] : 0.0;
@@ -1629,7 +1633,8 @@ __global__ void __launch_bounds__(64) k_nearb(const NearTask *__restrict__ tasks
 }
 
 template <int RC>
-__global__ void __launch_bounds__(64) k_near32b(const NearTask *__restrict__ tasks, int ntask,
+__global__ void __launch_bounds__(64) k_near32b(const NearTask *__restrict__ tasks, 
+    const int32_t *__restrict__ near_tgt, int ntask,
     int n, int R, int Rp, const float *__restrict__ src32, const float *__restrict__ wb,
     const double4 *__restrict__ ngeo, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
@@ -1641,9 +1646,9 @@ __global__ void __launch_bounds__(64) k_near32b(const NearTask *__restrict__ tas
     const int r0 = blockIdx.y * RC;
     const NearTask tk = tasks[task];
     const int lane = lane_id();
-    const int i = tk.g * kWave + lane;
-    const bool act = (tk.mask >> lane) & 1ull;
-    const bool valid = i < n;
+    const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
+    const bool act = i >= 0;
+    const bool valid = act;
     const double4 lg = ngeo[tk.leaf];
     const float sx = valid ? (float)(tx[i] - lg.x) : 0.f, sy = valid ? (float)(ty[i] - lg.y) : 0.f,
                 sz = valid ? (float)(tz[i] - lg.z) : 0.f;
@@ -1844,19 +1849,21 @@ __global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks
     }
 }
 
-// y = pre*x - sum of the group's task partials, for block column blockIdx.y
-__global__ void k_reduceb(int n, int R, const int32_t *__restrict__ near_task_off,
-    const int32_t *__restrict__ far_task_off, const double2 *__restrict__ part_near,
-    const double2 *__restrict__ part_far, const double *__restrict__ x, int64_t xs,
-    double *__restrict__ y, int64_t ys, const int32_t *__restrict__ act, double pre1, double pre2)
+// y = pre*x - sum of the target's partials, for block column blockIdx.y
+__global__ void k_reduceb(int n, int R, const int32_t *__restrict__ near_slot_off,
+    const int32_t *__restrict__ near_slot, const int32_t *__restrict__ far_task_off,
+    const double2 *__restrict__ part_near, const double2 *__restrict__ part_far,
+    const double *__restrict__ x, int64_t xs, double *__restrict__ y, int64_t ys,
+    const int32_t *__restrict__ act, double pre1, double pre2)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (i >= n) return;
     const int g = i / kWave, lane = i % kWave;
     double p1 = 0.0, p2 = 0.0;
-    for (int t = near_task_off[g]; t < near_task_off[g + 1]; ++t) {
-        const double2 v = part_near[((size_t)t * R + r) * kWave + lane];
+    for (int s = near_slot_off[i]; s < near_slot_off[i + 1]; ++s) {
+        const int slot = near_slot[s];
+        const double2 v = part_near[((size_t)(slot / kWave) * R + r) * kWave + slot % kWave];
         p1 += v.x; p2 += v.y;
     }
     for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
@@ -2033,8 +2040,7 @@ __global__ void __launch_bounds__(256) k_rhsb(int n, int nq, int R, const double
         const int cnt = min(256, nq - k0);
         for (int c = 0; c < cnt; ++c) {
             const double dx = sq[0][c] - px, dy = sq[1][c] - py, dz = sq[2][c] - pz;
-            const double rs = sqrt(dx * dx + dy * dy + dz * dz);
-            const double inv = 1.0 / rs;
+            const double inv = rsq_nr(dx * dx + dy * dy + dz * dz);
             const double ct = (vx * dx + vy * dy + vz * dz) * inv;
 #pragma unroll
             for (int k = 0; k < RC; ++k) {
@@ -2280,12 +2286,12 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
         if (c->prm.precision == 1) {
             k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p,
                                                                c->src32.p);
-            k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n,
+            k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src32.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
                 (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p);
         } else {
             k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p, c->src8.p);
-            k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n,
+            k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src8.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
                 c->prm.eps, c->part_near.p);
         }
@@ -2294,7 +2300,7 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
     {
         KTimer kt(c, 3);
-        k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->near_task_off.p, c->far_task_off.p,
+        k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->near_slot_off.p, c->near_slot.p, c->far_task_off.p,
             c->part_near.p, c->part_far.p, c->vx.p, pre1, pre2, dy);
     }
     HIPCHK(hipGetLastError());
@@ -2368,13 +2374,13 @@ void block_fields(tabi_ctx *c, int R, const double *X, int64_t xs, const int32_t
         if (c->prm.precision == 1) {
             c->wbf.alloc((size_t)c->n * 2 * Rp);
             k_src_weightsb<float><<<gw, 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, X, xs, d_act, c->wbf.p);
-            k_near32b<RCN><<<gn, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, R, Rp,
+            k_near32b<RCN><<<gn, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n, R, Rp,
                 c->src32.p, c->wbf.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p,
                 c->tnz.p, (float)c->prm.kappa, (float)c->prm.eps, c->part_nearb.p);
         } else {
             c->wbd.alloc((size_t)c->n * 2 * Rp);
             k_src_weightsb<double><<<gw, 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, X, xs, d_act, c->wbd.p);
-            k_nearb<RCN><<<gn, 64, 0, c->s>>>(c->near_tasks.p, c->n_near_task, c->n, R, Rp,
+            k_nearb<RCN><<<gn, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n, R, Rp,
                 c->src8.p, c->wbd.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
                 c->prm.kappa, c->prm.eps, c->part_nearb.p);
         }
@@ -2414,8 +2420,8 @@ void matvec_block_dev(tabi_ctx *c, int R, const double *X, int64_t xs, double *Y
         default: throw Error("unsupported Taylor order");
         }
         KTimer kt(c, 3);
-        k_reduceb<<<dim3(nblk(c->n, 256), Rc), 256, 0, c->s>>>(c->n, Rc, c->near_task_off.p,
-            c->far_task_off.p, c->part_nearb.p, c->part_farb.p, X, xs, Y, ys, act, pre1, pre2);
+        k_reduceb<<<dim3(nblk(c->n, 256), Rc), 256, 0, c->s>>>(c->n, Rc, c->near_slot_off.p,
+            c->near_slot.p, c->far_task_off.p, c->part_nearb.p, c->part_farb.p, X, xs, Y, ys, act, pre1, pre2);
     }
     HIPCHK(hipGetLastError());
 }
@@ -2443,6 +2449,7 @@ void msolve_block_dev(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *
 
 void build_lists(tabi_ctx *c)
 {
+
     const int G = c->ngroups;
     const HostTree &T = c->tree;
     const int nn = T.nnode();
@@ -2496,14 +2503,37 @@ void build_lists(tabi_ctx *c)
 
     c->p_far = 0;
     for (int64_t k = 0; k < nf; ++k) c->p_far += __builtin_popcountll(far_mask[k]);
+    // near field, compacted per leaf: the targets that list a leaf are
+    // gathered (groups and lanes ascending) and cut into waves of 64, so no
+    // lane idles on a target that does not need the leaf
+    std::vector<int32_t> loff(nn + 1, 0);
     c->p_near = 0;
-    int64_t near_src_total = 0;
     for (int64_t k = 0; k < nnr; ++k) {
-        const int np = T.numpar[near_leaf[k]];
-        c->p_near += (int64_t)__builtin_popcountll(near_mask[k]) * np;
-        near_src_total += np;
+        const int pc = __builtin_popcountll(near_mask[k]);
+        loff[near_leaf[k] + 1] += pc;
+        c->p_near += (int64_t)pc * T.numpar[near_leaf[k]];
     }
     c->p_near -= c->n;  // self pairs are excluded
+    for (int i = 0; i < nn; ++i) loff[i + 1] += loff[i];
+    // items of each leaf in group order (counting sort), then the leaves'
+    // target lists, tasks and slot rows are built by several host threads
+    std::vector<int32_t> ioff(nn + 1, 0), items(std::max<int64_t>(nnr, 1));
+    for (int64_t k = 0; k < nnr; ++k) ++ioff[near_leaf[k] + 1];
+    for (int i = 0; i < nn; ++i) ioff[i + 1] += ioff[i];
+    std::vector<int32_t> item_group(std::max<int64_t>(nnr, 1));
+    {
+        std::vector<int32_t> pos(ioff.begin(), ioff.end() - 1);
+        for (int g = 0; g < G; ++g)
+            for (int32_t k = c->h_near_off[g]; k < c->h_near_off[g + 1]; ++k) {
+                items[pos[near_leaf[k]]++] = k;
+                item_group[k] = g;
+            }
+    }
+    int64_t near_src_total = 0;   // source sweeps of the compacted waves
+    for (int nd = 0; nd < nn; ++nd) {
+        const int cnt = loff[nd + 1] - loff[nd];
+        if (cnt) near_src_total += (int64_t)((cnt + kWave - 1) / kWave) * T.numpar[nd];
+    }
 
     // task granularity: enough waves to fill the device several times over
     // work per task: the near source runs and far item runs are cut so that
@@ -2515,27 +2545,87 @@ void build_lists(tabi_ctx *c)
     while (S < 2048 && near_src_total / S > target) S *= 2;
     int F = 1;
     while (F < 64 && nf / F > target) F *= 2;
-    std::vector<NearTask> nt;
-    std::vector<FarTask> ft;
-    std::vector<int32_t> nto(G + 1, 0), fto(G + 1, 0);
-    for (int g = 0; g < G; ++g) {
-        for (int32_t k = c->h_near_off[g]; k < c->h_near_off[g + 1]; ++k) {
-            const int nd = near_leaf[k];
-            const int b = T.ibeg[nd], e = T.iend[nd] + 1;
-            for (int s0 = b; s0 < e; s0 += S) nt.push_back({g, s0, std::min(e, s0 + S), nd, near_mask[k]});
+    // tasks per leaf: chunks of 64 targets x source runs of S
+    std::vector<int64_t> toff(nn + 1, 0);
+    for (int nd = 0; nd < nn; ++nd) {
+        const int cnt = loff[nd + 1] - loff[nd];
+        const int nsplit = (T.numpar[nd] + S - 1) / S;
+        toff[nd + 1] = toff[nd] + (cnt ? (int64_t)((cnt + kWave - 1) / kWave) * nsplit : 0);
+    }
+    const int64_t ntask = toff[nn];
+    if (ntask * kWave > INT32_MAX) throw Error("too many near-field tasks");
+    std::vector<NearTask> nt(std::max<int64_t>(ntask, 1));
+    std::vector<int32_t> tgt(std::max<int64_t>(ntask * kWave, 1));
+    std::vector<int32_t> scnt(c->n + 1, 0);
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto parallel = [&](int64_t count, auto &&body) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; ++t)
+            th.emplace_back([&, t] {
+                for (int64_t q = t; q < count; q += nth) body(q);
+            });
+        for (auto &x : th) x.join();
+    };
+    parallel(nn, [&](int64_t nd) {
+        const int cnt = loff[nd + 1] - loff[nd];
+        if (!cnt) return;
+        // the leaf's targets, groups and lanes ascending
+        std::vector<int32_t> lt;
+        lt.reserve(cnt);
+        for (int32_t q = ioff[nd]; q < ioff[nd + 1]; ++q) {
+            const int32_t k = items[q];
+            u64 m = near_mask[k];
+            while (m) {
+                lt.push_back(item_group[k] * kWave + __builtin_ctzll(m));
+                m &= m - 1;
+            }
         }
-        nto[g + 1] = (int32_t)nt.size();
+        const int b = T.ibeg[nd], e = T.iend[nd] + 1;
+        int64_t task = toff[nd];
+        for (int c0 = 0; c0 < cnt; c0 += kWave)
+            for (int s0 = b; s0 < e; s0 += S, ++task) {
+                nt[task] = {-1, s0, std::min(e, s0 + S), (int32_t)nd, 0};
+                for (int lane = 0; lane < kWave; ++lane) {
+                    const int i = c0 + lane < cnt ? lt[c0 + lane] : -1;
+                    tgt[task * kWave + lane] = i;
+                    if (i >= 0) __atomic_fetch_add(&scnt[i + 1], 1, __ATOMIC_RELAXED);
+                }
+            }
+    });
+    // per target, its near slots (task*64 + lane), sorted: task order
+    for (int i = 0; i < c->n; ++i) scnt[i + 1] += scnt[i];
+    std::vector<int32_t> slot(std::max<int32_t>(scnt[c->n], 1));
+    {
+        std::vector<int32_t> pos(scnt.begin(), scnt.end() - 1);
+        parallel(nn, [&](int64_t nd) {
+            for (int64_t q = toff[nd] * kWave; q < toff[nd + 1] * kWave; ++q) {
+                const int i = tgt[q];
+                if (i >= 0) slot[__atomic_fetch_add(&pos[i], 1, __ATOMIC_RELAXED)] = (int32_t)q;
+            }
+        });
+        parallel(c->n, [&](int64_t i) { std::sort(slot.begin() + scnt[i], slot.begin() + scnt[i + 1]); });
+    }
+    nt.resize(ntask);
+    tgt.resize(ntask * kWave);
+    std::vector<FarTask> ft;
+    std::vector<int32_t> fto(G + 1, 0);
+    for (int g = 0; g < G; ++g) {
         for (int32_t k = c->h_far_off[g]; k < c->h_far_off[g + 1]; k += F)
             ft.push_back({g, k, std::min(c->h_far_off[g + 1], k + F), 0});
         fto[g + 1] = (int32_t)ft.size();
     }
+    if ((int64_t)tgt.size() > INT32_MAX) throw Error("too many near-field tasks");
     c->n_near_task = (int32_t)nt.size();
     c->n_far_task = (int32_t)ft.size();
     c->near_tasks.alloc(std::max<size_t>(nt.size(), 1));
     c->far_tasks.alloc(std::max<size_t>(ft.size(), 1));
-    if (!nt.empty()) c->near_tasks.upload(nt.data(), nt.size(), c->s);
+    if (!nt.empty()) {
+        c->near_tasks.upload(nt.data(), nt.size(), c->s);
+        c->near_tgt.upload(tgt.data(), tgt.size(), c->s);
+    }
     if (!ft.empty()) c->far_tasks.upload(ft.data(), ft.size(), c->s);
-    c->near_task_off.upload(nto.data(), G + 1, c->s);
+    c->near_slot_off.upload(scnt.data(), c->n + 1, c->s);
+    c->near_slot.upload(slot.data(), slot.size(), c->s);
     c->far_task_off.upload(fto.data(), G + 1, c->s);
     c->part_near.alloc((size_t)std::max(c->n_near_task, 1) * kWave);
     c->part_far.alloc((size_t)std::max(c->n_far_task, 1) * kWave);
