@@ -187,8 +187,7 @@ struct DevIn {
 };
 
 struct NearTask {
-    int32_t g, s0, s1, leaf;  // target group, source range [s0,s1), leaf node
-    u64 mask;
+    int32_t s0, s1, leaf, pad;  // source range [s0,s1) of leaf node `leaf`
 };
 struct FarTask {
     int32_t g, i0, i1, pad;
@@ -226,7 +225,7 @@ struct tabi_ctx {
     int32_t n_near_task = 0, n_far_task = 0;
     DBuf<NearTask> near_tasks;
     DBuf<FarTask> far_tasks;
-    DBuf<int32_t> near_tgt, near_slot_off, near_slot, far_task_off;
+    DBuf<int32_t> near_tgt, near_ipos, node_toff, node_nsplit, far_task_off;
     DBuf<double2> part_near, part_far;
     // preconditioner
     int32_t nleaf = 0, mmax = 0;
@@ -886,21 +885,66 @@ __global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__res
     if (i < n) b[i] = (float)a[i];
 }
 
-// y = pre*x - sum of the target's partials, near-field slots (in the fixed
-// order of its CSR row) then the far tasks of its group (TREE_COMPP_PB's
-// combination, src/treecode3d_pb.f:1154-1155,1177-1178)
-__global__ void k_reduce(int n, const int32_t *__restrict__ near_slot_off,
-    const int32_t *__restrict__ near_slot, const int32_t *__restrict__ far_task_off,
+// Near-field task layout.  Item k of target group g (targets g*64 + lane)
+// lists source leaf nd for the lanes set in near_mask[k].  Leaf nd's targets
+// are gathered in item order (k ascending, so groups ascending) and, inside an
+// item, lanes ascending: item k's j-th listed target sits at position
+// p = ipos[k] + j of nd's list.  The list is cut into waves of 64 and each wave
+// into nsplit[nd] source runs, so position p is lane p % 64 of the tasks
+// toff[nd] + (p / 64) * nsplit[nd] + s, s < nsplit[nd].
+struct NearSlots {
+    const int32_t *__restrict__ off;
+    const int32_t *__restrict__ leaf;
+    const u64 *__restrict__ mask;
+    const int32_t *__restrict__ ipos;
+    const int32_t *__restrict__ toff;
+    const int32_t *__restrict__ nsplit;
+};
+
+// the target rows of the near tasks (padding lanes hold -1 from a prior fill):
+// one wave per target group, lane = the target's lane in the group
+__global__ void __launch_bounds__(64) k_near_tgt(NearSlots ns, int32_t *__restrict__ tgt)
+{
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const u64 below = (1ull << lane) - 1;
+    for (int k = ns.off[g]; k < ns.off[g + 1]; ++k) {
+        const u64 m = ns.mask[k];
+        if (!((m >> lane) & 1)) continue;
+        const int p = ns.ipos[k] + __popcll(m & below);
+        const int nd = ns.leaf[k], nsp = ns.nsplit[nd];
+        const int64_t t0 = (int64_t)ns.toff[nd] + (int64_t)(p / kWave) * nsp;
+        for (int s = 0; s < nsp; ++s) tgt[(t0 + s) * kWave + p % kWave] = g * kWave + lane;
+    }
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t v, int32_t *__restrict__ a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// y = pre*x - sum of the target's partials: its near-field tasks (items of
+// its group in list order, then source runs) and then the far tasks of its
+// group (TREE_COMPP_PB's combination, src/treecode3d_pb.f:1154-1155,1177-1178)
+__global__ void k_reduce(int n, NearSlots ns, const int32_t *__restrict__ far_task_off,
     const double2 *__restrict__ part_near, const double2 *__restrict__ part_far,
     const double *__restrict__ x, double pre1, double pre2, double *__restrict__ y)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int g = i / kWave, lane = i % kWave;
+    const u64 below = (1ull << lane) - 1;
     double p1 = 0.0, p2 = 0.0;
-    for (int s = near_slot_off[i]; s < near_slot_off[i + 1]; ++s) {
-        const double2 v = part_near[near_slot[s]];
-        p1 += v.x; p2 += v.y;
+    for (int k = ns.off[g]; k < ns.off[g + 1]; ++k) {
+        const u64 m = ns.mask[k];
+        if (!((m >> lane) & 1)) continue;
+        const int p = ns.ipos[k] + __popcll(m & below);
+        const int nd = ns.leaf[k], nsp = ns.nsplit[nd];
+        const double2 *pt = part_near + ((size_t)ns.toff[nd] + (size_t)(p / kWave) * nsp) * kWave + p % kWave;
+        for (int s = 0; s < nsp; ++s) {
+            const double2 v = pt[(size_t)s * kWave];
+            p1 += v.x; p2 += v.y;
+        }
     }
     for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
         const double2 v = part_far[(size_t)t * kWave + lane];
@@ -1850,8 +1894,7 @@ __global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks
 }
 
 // y = pre*x - sum of the target's partials, for block column blockIdx.y
-__global__ void k_reduceb(int n, int R, const int32_t *__restrict__ near_slot_off,
-    const int32_t *__restrict__ near_slot, const int32_t *__restrict__ far_task_off,
+__global__ void k_reduceb(int n, int R, NearSlots ns, const int32_t *__restrict__ far_task_off,
     const double2 *__restrict__ part_near, const double2 *__restrict__ part_far,
     const double *__restrict__ x, int64_t xs, double *__restrict__ y, int64_t ys,
     const int32_t *__restrict__ act, double pre1, double pre2)
@@ -1860,11 +1903,18 @@ __global__ void k_reduceb(int n, int R, const int32_t *__restrict__ near_slot_of
     const int r = blockIdx.y;
     if (i >= n) return;
     const int g = i / kWave, lane = i % kWave;
+    const u64 below = (1ull << lane) - 1;
     double p1 = 0.0, p2 = 0.0;
-    for (int s = near_slot_off[i]; s < near_slot_off[i + 1]; ++s) {
-        const int slot = near_slot[s];
-        const double2 v = part_near[((size_t)(slot / kWave) * R + r) * kWave + slot % kWave];
-        p1 += v.x; p2 += v.y;
+    for (int k = ns.off[g]; k < ns.off[g + 1]; ++k) {
+        const u64 m = ns.mask[k];
+        if (!((m >> lane) & 1)) continue;
+        const int p = ns.ipos[k] + __popcll(m & below);
+        const int nd = ns.leaf[k], nsp = ns.nsplit[nd];
+        const size_t t0 = (size_t)ns.toff[nd] + (size_t)(p / kWave) * nsp;
+        for (int s = 0; s < nsp; ++s) {
+            const double2 v = part_near[((t0 + s) * R + r) * kWave + p % kWave];
+            p1 += v.x; p2 += v.y;
+        }
     }
     for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
         const double2 v = part_far[((size_t)t * R + r) * kWave + lane];
@@ -2275,6 +2325,12 @@ void dispatch_far(tabi_ctx *c)
 }
 
 // y = A x on device vectors c->vx -> c->vy
+NearSlots near_slots(const tabi_ctx *c)
+{
+    return {c->near_off.p, c->near_leaf.p, c->near_mask.p, c->near_ipos.p, c->node_toff.p,
+            c->node_nsplit.p};
+}
+
 void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
 {
     // the moment and field kernels read x from c->vx
@@ -2300,7 +2356,7 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
     {
         KTimer kt(c, 3);
-        k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->near_slot_off.p, c->near_slot.p, c->far_task_off.p,
+        k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, near_slots(c), c->far_task_off.p,
             c->part_near.p, c->part_far.p, c->vx.p, pre1, pre2, dy);
     }
     HIPCHK(hipGetLastError());
@@ -2420,8 +2476,8 @@ void matvec_block_dev(tabi_ctx *c, int R, const double *X, int64_t xs, double *Y
         default: throw Error("unsupported Taylor order");
         }
         KTimer kt(c, 3);
-        k_reduceb<<<dim3(nblk(c->n, 256), Rc), 256, 0, c->s>>>(c->n, Rc, c->near_slot_off.p,
-            c->near_slot.p, c->far_task_off.p, c->part_nearb.p, c->part_farb.p, X, xs, Y, ys, act, pre1, pre2);
+        k_reduceb<<<dim3(nblk(c->n, 256), Rc), 256, 0, c->s>>>(c->n, Rc, near_slots(c),
+            c->far_task_off.p, c->part_nearb.p, c->part_farb.p, X, xs, Y, ys, act, pre1, pre2);
     }
     HIPCHK(hipGetLastError());
 }
@@ -2503,37 +2559,22 @@ void build_lists(tabi_ctx *c)
 
     c->p_far = 0;
     for (int64_t k = 0; k < nf; ++k) c->p_far += __builtin_popcountll(far_mask[k]);
-    // near field, compacted per leaf: the targets that list a leaf are
-    // gathered (groups and lanes ascending) and cut into waves of 64, so no
-    // lane idles on a target that does not need the leaf
-    std::vector<int32_t> loff(nn + 1, 0);
+    // near field, compacted per leaf (the layout NearSlots describes): the
+    // targets that list a leaf are cut into waves of 64, so no lane idles on a
+    // target that does not need the leaf
+    std::vector<int32_t> lcnt(nn, 0), ipos(std::max<int64_t>(nnr, 1));
     c->p_near = 0;
     for (int64_t k = 0; k < nnr; ++k) {
+        const int nd = near_leaf[k];
         const int pc = __builtin_popcountll(near_mask[k]);
-        loff[near_leaf[k] + 1] += pc;
-        c->p_near += (int64_t)pc * T.numpar[near_leaf[k]];
+        ipos[k] = lcnt[nd];
+        lcnt[nd] += pc;
+        c->p_near += (int64_t)pc * T.numpar[nd];
     }
     c->p_near -= c->n;  // self pairs are excluded
-    for (int i = 0; i < nn; ++i) loff[i + 1] += loff[i];
-    // items of each leaf in group order (counting sort), then the leaves'
-    // target lists, tasks and slot rows are built by several host threads
-    std::vector<int32_t> ioff(nn + 1, 0), items(std::max<int64_t>(nnr, 1));
-    for (int64_t k = 0; k < nnr; ++k) ++ioff[near_leaf[k] + 1];
-    for (int i = 0; i < nn; ++i) ioff[i + 1] += ioff[i];
-    std::vector<int32_t> item_group(std::max<int64_t>(nnr, 1));
-    {
-        std::vector<int32_t> pos(ioff.begin(), ioff.end() - 1);
-        for (int g = 0; g < G; ++g)
-            for (int32_t k = c->h_near_off[g]; k < c->h_near_off[g + 1]; ++k) {
-                items[pos[near_leaf[k]]++] = k;
-                item_group[k] = g;
-            }
-    }
     int64_t near_src_total = 0;   // source sweeps of the compacted waves
-    for (int nd = 0; nd < nn; ++nd) {
-        const int cnt = loff[nd + 1] - loff[nd];
-        if (cnt) near_src_total += (int64_t)((cnt + kWave - 1) / kWave) * T.numpar[nd];
-    }
+    for (int nd = 0; nd < nn; ++nd)
+        if (lcnt[nd]) near_src_total += (int64_t)((lcnt[nd] + kWave - 1) / kWave) * T.numpar[nd];
 
     // task granularity: enough waves to fill the device several times over
     // work per task: the near source runs and far item runs are cut so that
@@ -2545,68 +2586,19 @@ void build_lists(tabi_ctx *c)
     while (S < 2048 && near_src_total / S > target) S *= 2;
     int F = 1;
     while (F < 64 && nf / F > target) F *= 2;
-    // tasks per leaf: chunks of 64 targets x source runs of S
-    std::vector<int64_t> toff(nn + 1, 0);
+    // tasks per leaf: waves of 64 targets x source runs of S
+    std::vector<int32_t> toff(nn), nsplit(nn);
+    std::vector<NearTask> nt;
     for (int nd = 0; nd < nn; ++nd) {
-        const int cnt = loff[nd + 1] - loff[nd];
-        const int nsplit = (T.numpar[nd] + S - 1) / S;
-        toff[nd + 1] = toff[nd] + (cnt ? (int64_t)((cnt + kWave - 1) / kWave) * nsplit : 0);
-    }
-    const int64_t ntask = toff[nn];
-    if (ntask * kWave > INT32_MAX) throw Error("too many near-field tasks");
-    std::vector<NearTask> nt(std::max<int64_t>(ntask, 1));
-    std::vector<int32_t> tgt(std::max<int64_t>(ntask * kWave, 1));
-    std::vector<int32_t> scnt(c->n + 1, 0);
-    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    auto parallel = [&](int64_t count, auto &&body) {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nth; ++t)
-            th.emplace_back([&, t] {
-                for (int64_t q = t; q < count; q += nth) body(q);
-            });
-        for (auto &x : th) x.join();
-    };
-    parallel(nn, [&](int64_t nd) {
-        const int cnt = loff[nd + 1] - loff[nd];
-        if (!cnt) return;
-        // the leaf's targets, groups and lanes ascending
-        std::vector<int32_t> lt;
-        lt.reserve(cnt);
-        for (int32_t q = ioff[nd]; q < ioff[nd + 1]; ++q) {
-            const int32_t k = items[q];
-            u64 m = near_mask[k];
-            while (m) {
-                lt.push_back(item_group[k] * kWave + __builtin_ctzll(m));
-                m &= m - 1;
-            }
-        }
+        if ((int64_t)nt.size() * kWave > INT32_MAX) throw Error("too many near-field tasks");
+        toff[nd] = (int32_t)nt.size();
+        nsplit[nd] = (T.numpar[nd] + S - 1) / S;
         const int b = T.ibeg[nd], e = T.iend[nd] + 1;
-        int64_t task = toff[nd];
-        for (int c0 = 0; c0 < cnt; c0 += kWave)
-            for (int s0 = b; s0 < e; s0 += S, ++task) {
-                nt[task] = {-1, s0, std::min(e, s0 + S), (int32_t)nd, 0};
-                for (int lane = 0; lane < kWave; ++lane) {
-                    const int i = c0 + lane < cnt ? lt[c0 + lane] : -1;
-                    tgt[task * kWave + lane] = i;
-                    if (i >= 0) __atomic_fetch_add(&scnt[i + 1], 1, __ATOMIC_RELAXED);
-                }
-            }
-    });
-    // per target, its near slots (task*64 + lane), sorted: task order
-    for (int i = 0; i < c->n; ++i) scnt[i + 1] += scnt[i];
-    std::vector<int32_t> slot(std::max<int32_t>(scnt[c->n], 1));
-    {
-        std::vector<int32_t> pos(scnt.begin(), scnt.end() - 1);
-        parallel(nn, [&](int64_t nd) {
-            for (int64_t q = toff[nd] * kWave; q < toff[nd + 1] * kWave; ++q) {
-                const int i = tgt[q];
-                if (i >= 0) slot[__atomic_fetch_add(&pos[i], 1, __ATOMIC_RELAXED)] = (int32_t)q;
-            }
-        });
-        parallel(c->n, [&](int64_t i) { std::sort(slot.begin() + scnt[i], slot.begin() + scnt[i + 1]); });
+        for (int c0 = 0; c0 < lcnt[nd]; c0 += kWave)
+            for (int s0 = b; s0 < e; s0 += S) nt.push_back({s0, std::min(e, s0 + S), nd, 0});
     }
-    nt.resize(ntask);
-    tgt.resize(ntask * kWave);
+    const int64_t ntask = (int64_t)nt.size();
+    if (ntask * kWave > INT32_MAX) throw Error("too many near-field tasks");
     std::vector<FarTask> ft;
     std::vector<int32_t> fto(G + 1, 0);
     for (int g = 0; g < G; ++g) {
@@ -2614,18 +2606,21 @@ void build_lists(tabi_ctx *c)
             ft.push_back({g, k, std::min(c->h_far_off[g + 1], k + F), 0});
         fto[g + 1] = (int32_t)ft.size();
     }
-    if ((int64_t)tgt.size() > INT32_MAX) throw Error("too many near-field tasks");
-    c->n_near_task = (int32_t)nt.size();
+    c->n_near_task = (int32_t)ntask;
     c->n_far_task = (int32_t)ft.size();
-    c->near_tasks.alloc(std::max<size_t>(nt.size(), 1));
+    c->near_tasks.alloc(std::max<int64_t>(ntask, 1));
     c->far_tasks.alloc(std::max<size_t>(ft.size(), 1));
-    if (!nt.empty()) {
-        c->near_tasks.upload(nt.data(), nt.size(), c->s);
-        c->near_tgt.upload(tgt.data(), tgt.size(), c->s);
+    c->near_ipos.upload(ipos.data(), ipos.size(), c->s);
+    c->node_toff.upload(toff.data(), nn, c->s);
+    c->node_nsplit.upload(nsplit.data(), nn, c->s);
+    if (ntask) {
+        c->near_tasks.upload(nt.data(), ntask, c->s);
+        c->near_tgt.alloc(ntask * kWave);
+        k_fill_i32<<<nblk(ntask * kWave, 256), 256, 0, c->s>>>(ntask * kWave, -1, c->near_tgt.p);
+        k_near_tgt<<<G, kWave, 0, c->s>>>(near_slots(c), c->near_tgt.p);
+        HIPCHK(hipGetLastError());
     }
     if (!ft.empty()) c->far_tasks.upload(ft.data(), ft.size(), c->s);
-    c->near_slot_off.upload(scnt.data(), c->n + 1, c->s);
-    c->near_slot.upload(slot.data(), slot.size(), c->s);
     c->far_task_off.upload(fto.data(), G + 1, c->s);
     c->part_near.alloc((size_t)std::max(c->n_near_task, 1) * kWave);
     c->part_far.alloc((size_t)std::max(c->n_far_task, 1) * kWave);
