@@ -28,6 +28,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <array>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -232,7 +233,7 @@ struct tabi_ctx {
     std::vector<int64_t> h_blk_off;
     std::vector<int32_t> h_blk_m;
     DBuf<int64_t> blk_off;
-    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info, lorder;
+    DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info, lorder, lu_swrec;
     DBuf<double> lu;
     // vectors
     DBuf<double> vx, vy, red;
@@ -766,6 +767,42 @@ __global__ void k_src32_weights(int n, const double *__restrict__ tarea,
     src32[(size_t)j * 8 + 7] = (float)(a * x[n + j]);
 }
 
+// FP32 near-field terms of one (target, source) pair (the pair kernel of
+// COMPP_DIRECT_PB, src/treecode3d_pb.f:965-1001, as in the FP64 kernel): source
+// record ra = {x, y, z, nx}, rb = {ny, nz, w1, w2}; target at (sx, sy, sz)
+// with normal v0.  For the self pair (use false) 1/r is set to zero, which
+// zeroes every term.
+struct Near32 {
+    float sx, sy, sz, v0x, v0y, v0z, kappa, kappa2, eps, inv_eps;
+    __device__ __forceinline__ void terms(const float4 ra, const float4 rb, bool use,
+                                          float &l1, float &l2, float &l3, float &l4) const
+    {
+        const float c4 = (float)kOneOver4Pi;
+        const float dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float inv = use ? __builtin_amdgcn_rsqf(r2) : 0.f;
+        const float rs = r2 * inv;
+        const float krs = kappa * rs;
+        const float ek = __expf(-krs);
+        const float g0 = c4 * inv;
+        const float gk = ek * g0;
+        const float ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
+        const float ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
+        const float tp1 = g0 * inv;
+        const float tp2 = fmaf(krs, ek, ek);
+        const float g1 = ct * tp1, g10 = ct0 * tp1;
+        const float g3 = (v0x * ra.w + v0y * rb.x + v0z * rb.y - 3.f * ct0 * ct) * inv * tp1;
+        l1 = g1 * fmaf(-eps, tp2, 1.f);
+        l2 = g0 - gk;
+        l3 = fmaf(g3, tp2 - 1.f, -kappa2 * ct0 * ct * gk);
+        l4 = g10 * fmaf(-inv_eps, tp2, 1.f);
+    }
+};
+
+// Sources are taken kNearBlk at a time: their contributions are summed with
+// FMAs into a block total, which then enters the compensated (Kahan) sum.
+constexpr int kNearBlk = 4;
+
 __global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ tasks, 
     const int32_t *__restrict__ near_tgt, int ntask,
     int n, const float *__restrict__ src32, const double4 *__restrict__ ngeo,
@@ -780,43 +817,33 @@ __global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ task
     const int lane = lane_id();
     const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
     const bool act = i >= 0;
-    const bool valid = act;
     const double4 lg = ngeo[tk.leaf];
-    const float sx = valid ? (float)(tx[i] - lg.x) : 0.f, sy = valid ? (float)(ty[i] - lg.y) : 0.f,
-                sz = valid ? (float)(tz[i] - lg.z) : 0.f;
-    const float v0x = valid ? (float)tnx[i] : 0.f, v0y = valid ? (float)tny[i] : 0.f,
-                v0z = valid ? (float)tnz[i] : 0.f;
-    const float kappa2 = kappa * kappa, inv_eps = 1.f / eps;
-    const float c4 = (float)kOneOver4Pi;
+    Near32 nk;
+    nk.sx = act ? (float)(tx[i] - lg.x) : 0.f;
+    nk.sy = act ? (float)(ty[i] - lg.y) : 0.f;
+    nk.sz = act ? (float)(tz[i] - lg.z) : 0.f;
+    nk.v0x = act ? (float)tnx[i] : 0.f;
+    nk.v0y = act ? (float)tny[i] : 0.f;
+    nk.v0z = act ? (float)tnz[i] : 0.f;
+    nk.kappa = kappa; nk.kappa2 = kappa * kappa; nk.eps = eps; nk.inv_eps = 1.f / eps;
     Kahan p1, p2;
     if (act) {
-#pragma unroll 4
-        for (int j = tk.s0; j < tk.s1; ++j) {
-            const float4 ra = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8);
-            const float4 rb = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8 + 4);
-            const bool use = (j != i);
-            const float dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            r2 = use ? r2 : 1.f;
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float rs = r2 * inv;
-            const float krs = kappa * rs;
-            const float ek = __expf(-krs);
-            const float g0 = c4 * inv;
-            const float gk = ek * g0;
-            const float ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
-            const float ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
-            const float tp1 = g0 * inv;
-            const float tp2 = fmaf(krs, ek, ek);
-            const float g1 = ct * tp1, g10 = ct0 * tp1;
-            const float g3 = (v0x * ra.w + v0y * rb.x + v0z * rb.y - 3.f * ct0 * ct) * inv * tp1;
-            const float l1 = g1 * fmaf(-eps, tp2, 1.f);
-            const float l2 = g0 - gk;
-            const float l3 = fmaf(g3, tp2 - 1.f, -kappa2 * ct0 * ct * gk);
-            const float l4 = g10 * fmaf(-inv_eps, tp2, 1.f);
-            const float w1 = use ? rb.z : 0.f, w2 = use ? rb.w : 0.f;
-            p1.add(fmaf(l1, w1, l2 * w2));
-            p2.add(fmaf(l3, w1, l4 * w2));
+        for (int j0 = tk.s0; j0 < tk.s1; j0 += kNearBlk) {
+            float b1 = 0.f, b2 = 0.f;
+#pragma unroll
+            for (int u = 0; u < kNearBlk; ++u) {
+                const int j = j0 + u;
+                if (j < tk.s1) {
+                    const float4 ra = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8);
+                    const float4 rb = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8 + 4);
+                    float l1, l2, l3, l4;
+                    nk.terms(ra, rb, j != i, l1, l2, l3, l4);
+                    b1 = fmaf(l1, rb.z, fmaf(l2, rb.w, b1));
+                    b2 = fmaf(l3, rb.z, fmaf(l4, rb.w, b2));
+                }
+            }
+            p1.add(b1);
+            p2.add(b2);
         }
     }
     part[(size_t)task * kWave + lane] = make_double2((double)p1.s, (double)p2.s);
@@ -1018,6 +1045,71 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
 // Batched LU: leaf blocks are processed largest first (lorder[] sorts leaves by
 // block size, descending), so the blocks still active at panel step k0 are a
 // prefix of lorder and every launch covers only active blocks.
+//
+// Two-level blocking.  Columns are factored in strips of kLuW (four panels of
+// kLuNB).  Inside a strip the panels are right-looking over the strip's own
+// columns only.  The columns beyond the strip take the row interchanges of
+// every panel as they come, but none of the strip's updates until its end:
+// their rows of panel k0 are brought up to date left-looking (the panel rows
+// minus L[panel rows][K0..k0) U[K0..k0)[col]) just before the panel's unit
+// lower solve, and everything below the strip is updated once, by a rank-kLuW
+// product, when the strip is done.  Rows below the panel therefore always
+// hold either all or none of a given panel's update, so the interchanges move
+// consistent data.  The factors are DGETRF's (the unblocked steps of DGETF2
+// within each panel, src/dgetrf_dep.f:757-914); only the order in which
+// the trailing products are accumulated differs.
+constexpr int kLuW = 4 * kLuNB;   // strip width
+
+// Interchange record of one panel (pivot rows piv[c] of the rows k0 + c,
+// c < nb, applied in order): the touched rows dst[q], q < nr -- the panel
+// rows, then each distinct pivot row below the panel in order of first use --
+// and src[q], the row whose contents end up in dst[q].  Built by one wave
+// with cross-lane exchanges (lane c holds pivot c; lane q tracks slot q);
+// writes dst and src to the arrays given and returns nr in every lane.
+constexpr int kSwRec = 4 * kLuNB + 4;   // ints per block: dst[64], src[64], nr
+__device__ __forceinline__ int lu_swap_record(int lane, int k0, int nbe, int pr, int *dst,
+                                              int *src)
+{
+    // first occurrence of this lane's pivot row
+    int first = lane;
+    for (int c2 = 0; c2 < kLuNB; ++c2) {
+        const int o = __shfl(pr, c2);
+        if (c2 < first && c2 < lane && o == pr) first = c2;
+    }
+    const bool isnew = lane < nbe && pr >= k0 + nbe && first == lane;
+    const u64 newmask = __ballot(isnew);
+    const int nr = nbe + __popcll(newmask);
+    // slot of this lane's pivot row in the touched-row list
+    int slot = lane;
+    if (lane < nbe)
+        slot = pr < k0 + nbe ? pr - k0 : nbe + __popcll(newmask & ((1ull << first) - 1));
+    // the list itself, held one entry per lane: panel rows, then new extras
+    int row = lane < nbe ? k0 + lane : 0;
+    {
+        // the extra with rank r sits in the lane of the r-th set bit of newmask
+        const int r = lane - nbe;
+        int idx = 0;
+        if (r >= 0 && lane < nr) {
+            u64 mm = newmask;
+            for (int k = 0; k < r; ++k) mm &= mm - 1;
+            idx = __ffsll((long long)mm) - 1;
+        }
+        const int o = __shfl(pr, idx);
+        if (r >= 0 && lane < nr) row = o;
+    }
+    // position tracking: after the swaps (c, slot[c]) in order, slot q holds
+    // the original contents of slot p
+    int p = lane;
+    for (int c = 0; c < nbe; ++c) {
+        const int sl = __shfl(slot, c);
+        const int pc = __shfl(p, c), ps = __shfl(p, sl);
+        if (lane == c) p = ps;
+        else if (lane == sl) p = pc;
+    }
+    const int srow = __shfl(row, p);
+    if (lane < nr) { dst[lane] = row; src[lane] = srow; }
+    return nr;
+}
 
 // Panel factorisation of columns k0..k0+nb-1: partial pivoting with the
 // first-maximum rule (IDAMAX), reciprocal scaling and rank-1 updates restricted
@@ -1025,17 +1117,21 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
 // src/dgetrf_dep.f:757-914).  One workgroup of kPanelThreads per block holds
 // the panel in registers, RPT rows per thread (row k0 + t + j*kPanelThreads);
 // each column step is a block-wide arg-max, a row exchange through LDS and a
-// broadcast of the pivot row.
+// broadcast of the pivot row.  Afterwards the panel's interchanges are applied
+// to the strip's earlier columns K0..k0-1 (staged in LDS).
 template <int RPT, int kPanelThreads>
 __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
-    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info)
+    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info,
+    int32_t *__restrict__ swrec)
 {
     __shared__ double s_best[kPanelThreads / kWave];
     __shared__ int s_bi[kPanelThreads / kWave];
     __shared__ double s_row[2][kLuNB];
     __shared__ int s_prow;
+    __shared__ int32_t s_piv[kLuNB];
+    __shared__ int s_dst[2 * kLuNB], s_src[2 * kLuNB];
     const int l = lorder[first + blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
@@ -1081,7 +1177,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
                 const int oi = __shfl_xor(bi, off);
                 if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
             }
-            if (lane == 0) { s_prow = bi; pv[col] = bi; }
+            if (lane == 0) { s_prow = bi; s_piv[c] = bi; pv[col] = bi; }
         }
         __syncthreads();
         const int prow = s_prow;
@@ -1134,160 +1230,253 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
         }
     }
     if (bad && t == 0) atomicCAS(info + l, 0, bad);
+    // interchange record for the other columns (read by k_lu_swap_trsm)
+    int32_t *rec = swrec + (size_t)l * kSwRec;
+    if (w == 0) {
+        const int nr = lu_swap_record(lane, k0, nbe, lane < nbe ? s_piv[lane] : -1, s_dst, s_src);
+        if (lane < nr) { rec[lane] = s_dst[lane]; rec[2 * kLuNB + lane] = s_src[lane]; }
+        if (lane == 0) { rec[4 * kLuNB] = nr; s_prow = nr; }
+    }
+    // the panel's interchanges on the strip's earlier columns K0..k0-1: every
+    // touched row is read, then written to its destination
+    const int K0 = k0 - k0 % kLuW, ncl = k0 - K0;
+    if (ncl == 0) return;
+    __syncthreads();
+    const int nr = s_prow;
+    // element e = t + kPanelThreads i: row slot e % 64, column e / 64
+    constexpr int kEs = (2 * kLuNB * (kLuW - kLuNB) + kPanelThreads - 1) / kPanelThreads;
+    double v[kEs];
+#pragma unroll
+    for (int i = 0; i < kEs; ++i) {
+        const int e = t + kPanelThreads * i, q = e % 64, jj = min(e / 64, ncl - 1);
+        v[i] = A[(size_t)(K0 + jj) * m + s_src[min(q, nr - 1)]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kEs; ++i) {
+        const int e = t + kPanelThreads * i, q = e % 64, jj = e / 64;
+        if (q < nr && jj < ncl) A[(size_t)(K0 + jj) * m + s_dst[q]] = v[i];
+    }
 }
 
-// Apply the panel's row interchanges to the columns outside the panel and form
-// U12 = L11^-1 A12 for the columns to its right.  A workgroup takes 64 columns
-// of one active block: the touched rows (the panel rows and the pivot rows
-// below them) are staged in LDS with coalesced loads, the interchanges and the
-// unit-lower solve run there, and the rows are written back.
+// Row interchanges of panel k0 on the columns outside the strip's factored
+// part, and U12 = L11^-1 A12 for the columns to the panel's right.  A
+// workgroup takes 64 columns of one active block; the touched rows (the panel
+// rows and the pivot rows below them) are staged in LDS with coalesced loads.
+//   columns left of the strip          interchanges only;
+//   strip columns K0..k0+nb-1          untouched (the panel kernel's part);
+//   strip columns right of the panel   interchanges, unit lower solve (their
+//                                      rows are up to date: in-strip updates);
+//   columns beyond the strip           interchanges, left-looking update of
+//                                      the panel rows, unit lower solve.
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    const int32_t *__restrict__ piv_off, double *__restrict__ lu, const int32_t *__restrict__ piv)
+    double *__restrict__ lu, const int32_t *__restrict__ swrec)
 {
     __shared__ double L11[kLuNB][kLuNB + 1];
     __shared__ double T[2 * kLuNB][64 + 1];
-    __shared__ int rows[2 * kLuNB];
-    __shared__ int slot_of_piv[kLuNB];
+    __shared__ double LL[kLuNB][kLuW - kLuNB + 1];   // L[k0..k0+nb)[K0..k0)
+    __shared__ int rows[2 * kLuNB], src[2 * kLuNB];
     __shared__ int s_nr;
     const int2 tl = tiles[blockIdx.x];           // {block, column tile}
     const int l = tl.x;
+    if (l < 0) return;                         // padding of the XCD-grouped list
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
+    const int K0 = k0 - k0 % kLuW, kl = k0 - K0, send = min(m, K0 + kLuW);
     const int j0 = tl.y * 64;
     double *A = lu + blk_off[l];
-    const int32_t *pv = piv + piv_off[l];
+    const int32_t *rec = swrec + (size_t)l * kSwRec;
     const int t = threadIdx.x;
-    // touched rows: the panel rows, then each distinct pivot row below the
-    // panel in order of first use (built by 32 threads in parallel)
-    if (t < kLuNB) {
-        if (t < nbe) rows[t] = k0 + t;
-        int pr = -1, isnew = 0;
-        if (t < nbe) {
-            pr = pv[k0 + t];
-            isnew = pr >= k0 + nbe;
-            for (int c2 = 0; c2 < t && isnew; ++c2)
-                if (pv[k0 + c2] == pr) isnew = 0;
+    if (t < 2 * kLuNB) { rows[t] = rec[t]; src[t] = rec[2 * kLuNB + t]; }
+    if (t == 0) s_nr = rec[4 * kLuNB];
+    const bool right = j0 + 63 >= k0 + nbe;    // the tile holds columns right of the panel
+    const bool beyond = j0 + 63 >= send;       // ... and columns beyond the strip
+    // staging loops have fixed trip counts and clamped addresses, so their
+    // loads are issued together
+    if (right) {
+        double v[kLuNB * kLuNB / 256];
+#pragma unroll
+        for (int i = 0; i < kLuNB * kLuNB / 256; ++i) {
+            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
+            v[i] = A[(size_t)(k0 + min(c, nbe - 1)) * m + k0 + min(r, nbe - 1)];
         }
-        // rank among new extras = number of new extras before t
-        const u64 newmask = __ballot(isnew) & ((1ull << kLuNB) - 1);
-        const int rank = __popcll(newmask & ((1ull << t) - 1));
-        if (isnew) rows[nbe + rank] = pr;
-        if (t == 0) s_nr = nbe + __popcll(newmask);
-        if (t < nbe) {
-            int sl = pr - k0;
-            if (pr >= k0 + nbe) {
-                // slot of the first occurrence of pr
-                int first = t;
-                for (int c2 = 0; c2 < t; ++c2)
-                    if (pv[k0 + c2] == pr) { first = c2; break; }
-                sl = nbe + __popcll(newmask & ((1ull << first) - 1));
-            }
-            slot_of_piv[t] = sl;
+#pragma unroll
+        for (int i = 0; i < kLuNB * kLuNB / 256; ++i) {
+            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
+            L11[r][c] = v[i];
         }
     }
-    const bool right = j0 + 63 >= k0 + nbe;    // the tile holds columns right of the panel
-    if (right) {
-        for (int e = t; e < nbe * nbe; e += 256) {
-            const int r = e % nbe, c = e / nbe;
-            L11[r][c] = A[(size_t)(k0 + c) * m + k0 + r];
+    if (beyond && kl > 0) {
+        // a strip only extends beyond itself once its panels are full (nb = kLuNB)
+        constexpr int kN = kLuNB * (kLuW - kLuNB) / 256;
+        double v[kN];
+#pragma unroll
+        for (int i = 0; i < kN; ++i) {
+            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
+            v[i] = A[(size_t)(K0 + min(c, kl - 1)) * m + k0 + r];
+        }
+#pragma unroll
+        for (int i = 0; i < kN; ++i) {
+            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
+            if (c < kl) LL[r][c] = v[i];
         }
     }
     __syncthreads();
     const int nr = s_nr;
-    for (int e = t; e < nr * 64; e += 256) {
-        const int jj = e / nr, q = e % nr, j = j0 + jj;
-        const bool col_ok = j < m && (j < k0 || j >= k0 + nbe);
-        T[q][jj] = col_ok ? A[(size_t)j * m + rows[q]] : 0.0;
+    auto col_ok = [&](int j) { return j < m && (j < K0 || j >= k0 + nbe); };
+    // touched rows: element e = t + 256 i is row slot e % 64, column e / 64
+    constexpr int kTs = 2 * kLuNB * 64 / 256;
+    {
+        double v[kTs];
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + 256 * i, q = e % 64, j = min(j0 + e / 64, m - 1);
+            v[i] = A[(size_t)j * m + src[min(q, nr - 1)]];
+        }
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + 256 * i, q = e % 64, jj = e / 64;
+            if (q < nr) T[q][jj] = col_ok(j0 + jj) ? v[i] : 0.0;
+        }
     }
-    __syncthreads();    // every staged row is in LDS before the column sweeps read it
-    // interchanges, then U12 = L11^-1 A12 for the columns right of the panel
-    // (column sweeps); tiles left of the panel only take the interchanges
+    __syncthreads();    // the rows are staged in their post-interchange order
+    if (beyond && kl > 0) {
+        // panel rows of the columns beyond the strip: minus L[rows][K0..k0) U[K0..k0)[j];
+        // thread (column t%64, row group t/64)
+        // (rows rg + 4i, i < 8: a strip always has full panels before this one)
+        const int jc = t % 64, rg = t / 64, j = j0 + jc;
+        if (j < m && j >= send) {
+            const double *U = A + (size_t)j * m + K0;
+            double s[kLuNB / 4];
+#pragma unroll
+            for (int i = 0; i < kLuNB / 4; ++i) s[i] = T[rg + 4 * i][jc];
+#pragma unroll 4
+            for (int c = 0; c < kl; ++c) {
+                const double u = U[c];
+#pragma unroll
+                for (int i = 0; i < kLuNB / 4; ++i) s[i] = fma(-LL[rg + 4 * i][c], u, s[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < kLuNB / 4; ++i) T[rg + 4 * i][jc] = s[i];
+        }
+        __syncthreads();
+    }
+    // U12 = L11^-1 A12 for the columns right of the panel, eliminated column
+    // by column: the updates for one pivot are independent of each other
     if (t < 64) {
         const int j = j0 + t;
-        if (j < m && (j < k0 || j >= k0 + nbe)) {
-            for (int c = 0; c < nbe; ++c) {
-                const int sl = slot_of_piv[c];
-                if (sl != c) {
-                    const double tmp = T[c][t];
-                    T[c][t] = T[sl][t];
-                    T[sl][t] = tmp;
-                }
-            }
-        }
         if (right && j < m && j >= k0 + nbe) {
-            for (int c = 1; c < nbe; ++c) {
-                double sacc = T[c][t];
-                for (int c2 = 0; c2 < c; ++c2) sacc = fma(-L11[c][c2], T[c2][t], sacc);
-                T[c][t] = sacc;
+            for (int c2 = 0; c2 < nbe - 1; ++c2) {
+                const double x = T[c2][t];
+                for (int c = c2 + 1; c < nbe; ++c) T[c][t] = fma(-L11[c][c2], x, T[c][t]);
             }
         }
     }
     __syncthreads();
-    for (int e = t; e < nr * 64; e += 256) {
-        const int jj = e / nr, q = e % nr, j = j0 + jj;
-        if (j < m && (j < k0 || j >= k0 + nbe)) A[(size_t)j * m + rows[q]] = T[q][jj];
+#pragma unroll
+    for (int i = 0; i < kTs; ++i) {
+        const int e = t + 256 * i, q = e % 64, j = j0 + e / 64;
+        if (q < nr && col_ok(j)) A[(size_t)j * m + rows[q]] = T[q][e / 64];
     }
 }
 
 // Trailing update A22 -= L21 U12 on 64x64 tiles from a precomputed tile list
-// {block, tile row, tile col}; 256 threads x (4x4) outputs.
-__global__ void __launch_bounds__(256) k_lu_gemm(int k0, const int4 *__restrict__ tiles,
+// {block, tile row, tile col, column end} on the FP64 matrix cores.  L21 holds
+// the columns kb..kb+KD-1 (KD = kLuNB for the in-strip update after a panel,
+// kLuW for the update below a finished strip), taken kLuNB at a time through
+// LDS; rows and columns start at kb + KD; columns stop at the tile's column
+// end.  Each of the four waves owns a 32x32 quadrant as 2x2 blocks of
+// v_mfma_f64_16x16x4f64: A operand L21[row = lane%16][k = lane/16], B operand
+// U12[k = lane/16][col = lane%16], result register v = D[lane/16 + 4v][lane%16].
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int KD>
+__global__ void __launch_bounds__(256) k_lu_gemm(int kb, const int4 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     double *__restrict__ lu)
 {
-    __shared__ double sL[kLuNB][64 + 1];
-    __shared__ double sU[kLuNB][64 + 1];
+    __shared__ double sL[kLuNB][64 + 1];   // [k][row]
+    __shared__ double sU[kLuNB][64 + 1];   // [k][column]
     const int4 tl = tiles[blockIdx.x];
     const int l = tl.x;
+    if (l < 0) return;                     // padding of the XCD-grouped list
     const int m = blk_m[l];
-    const int nbe = kLuNB;
-    const int r0 = k0 + nbe + tl.y * 64, c0 = k0 + nbe + tl.z * 64;
+    const int cend = tl.w;
+    const int r0 = kb + KD + tl.y * 64, c0 = kb + KD + tl.z * 64;
     double *A = lu + blk_off[l];
-    const int t = threadIdx.x;
-    for (int e = t; e < nbe * 64; e += 256) {
-        const int c = e / 64, r = e % 64;
-        sL[c][r] = (r0 + r < m) ? A[(size_t)(k0 + c) * m + r0 + r] : 0.0;
-        const int cc = e / nbe, rr = e % nbe;  // U12 row rr, column c0+cc
-        sU[rr][cc] = (c0 + cc < m) ? A[(size_t)(c0 + cc) * m + k0 + rr] : 0.0;
-    }
-    __syncthreads();
-    // thread (tr, tc) owns rows tr + 16a and columns tc + 16b: consecutive
-    // threads touch consecutive rows of a column
-    const int tr = t % 16, tc = t / 16;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int rw = (w & 1) * 32, cw = (w >> 1) * 32, li = lane & 15, lk = lane >> 4;
     // the trailing entries are loaded before the product so their latency
     // overlaps the arithmetic
-    double cv[4][4];
+    double cv[2][2][4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int col = c0 + tc + 16 * b;
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int row = r0 + tr + 16 * a;
-            cv[a][b] = (row < m && col < m) ? A[(size_t)col * m + row] : 0.0;
+        for (int bj = 0; bj < 2; ++bj) {
+            const int col = c0 + cw + 16 * bj + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = r0 + rw + 16 * bi + lk + 4 * v;
+                cv[bi][bj][v] = (row < m && col < cend) ? A[(size_t)col * m + row] : 0.0;
+            }
+        }
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+    // chunk staging: element e = t + 256 i of L21 (column e / 64, row e % 64)
+    // and of U12 (column e / kLuNB, row e % kLuNB); rows and columns past the
+    // tile's end are clamped (their products land in outputs that are not
+    // stored), so every load is unconditional and a chunk's 16 loads are in
+    // flight together -- the next chunk's while this one is multiplied
+    constexpr int kSt = kLuNB * 64 / 256;
+    double lst[kSt], ust[kSt];
+    auto fetch = [&](int k1) {
+#pragma unroll
+        for (int i = 0; i < kSt; ++i) {
+            const int e = t + 256 * i;
+            const int row = min(r0 + (e & 63), m - 1);
+            lst[i] = A[(size_t)(k1 + (e >> 6)) * m + row];
+            const int col = min(c0 + e / kLuNB, cend - 1);
+            ust[i] = A[(size_t)col * m + k1 + e % kLuNB];
+        }
+    };
+    fetch(kb);
+#pragma unroll 1
+    for (int k1 = kb; k1 < kb + KD; k1 += kLuNB) {
+        if (k1 > kb) __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSt; ++i) {
+            const int e = t + 256 * i;
+            sL[e >> 6][e & 63] = lst[i];
+            sU[e % kLuNB][e / kLuNB] = ust[i];
+        }
+        __syncthreads();
+        if (k1 + kLuNB < kb + KD) fetch(k1 + kLuNB);
+#pragma unroll
+        for (int kk = 0; kk < kLuNB; kk += 4) {
+            const double a0 = sL[kk + lk][rw + li], a1 = sL[kk + lk][rw + 16 + li];
+            const double b0 = sU[kk + lk][cw + li], b1 = sU[kk + lk][cw + 16 + li];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
     }
-    double acc[4][4] = {};
-#pragma unroll 4
-    for (int c = 0; c < nbe; ++c) {
-        double lv[4], uv[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { lv[q] = sL[c][tr + 16 * q]; uv[q] = sU[c][tc + 16 * q]; }
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int bj = 0; bj < 2; ++bj) {
+            const int col = c0 + cw + 16 * bj + li;
+            double *Ac = A + (size_t)col * m;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(lv[a], uv[b], acc[a][b]);
-    }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int col = c0 + tc + 16 * b;
-        double *Ac = A + (size_t)col * m;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int row = r0 + tr + 16 * a;
-            if (row < m && col < m) Ac[row] = cv[a][b] - acc[a][b];
+            for (int v = 0; v < 4; ++v) {
+                const int row = r0 + rw + 16 * bi + lk + 4 * v;
+                if (row < m && col < cend) Ac[row] = cv[bi][bj][v] - acc[bi][bj][v];
+            }
         }
-    }
 }
 
 // Composite row permutation of each leaf's pivot sequence: (P r)[k] = r[perm[k]]
@@ -1692,45 +1881,41 @@ __global__ void __launch_bounds__(64) k_near32b(const NearTask *__restrict__ tas
     const int lane = lane_id();
     const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
     const bool act = i >= 0;
-    const bool valid = act;
     const double4 lg = ngeo[tk.leaf];
-    const float sx = valid ? (float)(tx[i] - lg.x) : 0.f, sy = valid ? (float)(ty[i] - lg.y) : 0.f,
-                sz = valid ? (float)(tz[i] - lg.z) : 0.f;
-    const float v0x = valid ? (float)tnx[i] : 0.f, v0y = valid ? (float)tny[i] : 0.f,
-                v0z = valid ? (float)tnz[i] : 0.f;
-    const float kappa2 = kappa * kappa, inv_eps = 1.f / eps;
-    const float c4 = (float)kOneOver4Pi;
+    Near32 nk;
+    nk.sx = act ? (float)(tx[i] - lg.x) : 0.f;
+    nk.sy = act ? (float)(ty[i] - lg.y) : 0.f;
+    nk.sz = act ? (float)(tz[i] - lg.z) : 0.f;
+    nk.v0x = act ? (float)tnx[i] : 0.f;
+    nk.v0y = act ? (float)tny[i] : 0.f;
+    nk.v0z = act ? (float)tnz[i] : 0.f;
+    nk.kappa = kappa; nk.kappa2 = kappa * kappa; nk.eps = eps; nk.inv_eps = 1.f / eps;
     Kahan p1[RC], p2[RC];
     if (act) {
-        for (int j = tk.s0; j < tk.s1; ++j) {
-            const float4 ra = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8);
-            const float4 rb = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8 + 4);
-            const bool use = (j != i);
-            const float dx = ra.x - sx, dy = ra.y - sy, dz = ra.z - sz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            r2 = use ? r2 : 1.f;
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float rs = r2 * inv;
-            const float krs = kappa * rs;
-            const float ek = __expf(-krs);
-            const float g0 = c4 * inv;
-            const float gk = ek * g0;
-            const float ct = (ra.w * dx + rb.x * dy + rb.y * dz) * inv;
-            const float ct0 = (v0x * dx + v0y * dy + v0z * dz) * inv;
-            const float tp1 = g0 * inv;
-            const float tp2 = fmaf(krs, ek, ek);
-            const float g1 = ct * tp1, g10 = ct0 * tp1;
-            const float g3 = (v0x * ra.w + v0y * rb.x + v0z * rb.y - 3.f * ct0 * ct) * inv * tp1;
-            const float l1 = g1 * fmaf(-eps, tp2, 1.f);
-            const float l2 = g0 - gk;
-            const float l3 = fmaf(g3, tp2 - 1.f, -kappa2 * ct0 * ct * gk);
-            const float l4 = g10 * fmaf(-inv_eps, tp2, 1.f);
-            const float *w = wb + (size_t)j * 2 * Rp + 2 * r0;
+        for (int j0 = tk.s0; j0 < tk.s1; j0 += kNearBlk) {
+            float b1[RC], b2[RC];
+#pragma unroll
+            for (int k = 0; k < RC; ++k) b1[k] = b2[k] = 0.f;
+#pragma unroll
+            for (int u = 0; u < kNearBlk; ++u) {
+                const int j = j0 + u;
+                if (j < tk.s1) {
+                    const float4 ra = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8);
+                    const float4 rb = *reinterpret_cast<const float4 *>(src32 + (size_t)j * 8 + 4);
+                    float l1, l2, l3, l4;
+                    nk.terms(ra, rb, j != i, l1, l2, l3, l4);
+                    const float *w = wb + (size_t)j * 2 * Rp + 2 * r0;
+#pragma unroll
+                    for (int k = 0; k < RC; ++k) {
+                        b1[k] = fmaf(l1, w[2 * k], fmaf(l2, w[2 * k + 1], b1[k]));
+                        b2[k] = fmaf(l3, w[2 * k], fmaf(l4, w[2 * k + 1], b2[k]));
+                    }
+                }
+            }
 #pragma unroll
             for (int k = 0; k < RC; ++k) {
-                const float w1 = use ? w[2 * k] : 0.f, w2 = use ? w[2 * k + 1] : 0.f;
-                p1[k].add(fmaf(l1, w1, l2 * w2));
-                p2[k].add(fmaf(l3, w1, l4 * w2));
+                p1[k].add(b1[k]);
+                p2[k].add(b2[k]);
             }
         }
     }
@@ -2667,6 +2852,39 @@ void build_lists(tabi_ctx *c)
     HIPCHK(hipStreamSynchronize(c->s));
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin in dispatch order.  A tile
+// list is laid out so that XCD x receives the contiguous run
+// [x*per, (x+1)*per) of the logical order (per = ceil(count/8)): tiles of
+// one leaf block, which share their L and U panels, then meet in one L2.
+// Padding entries carry block -1 and exit at once.
+constexpr int kXcd = 8;
+
+// Side streams (default priority) for launches that may run beside the
+// library's stream, one set per device, created on first use and kept.
+hipStream_t side_stream(int k)
+{
+    static std::mutex mu;
+    static std::map<int, std::array<hipStream_t, 3>> pool;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto &a = pool[dev];
+    if (!a[k]) HIPCHK(hipStreamCreate(&a[k]));
+    return a[k];
+}
+
+template <class T>
+void xcd_group(std::vector<T> &v, size_t beg, T pad)
+{
+    const size_t cnt = v.size() - beg;
+    if (cnt <= (size_t)kXcd) return;
+    const size_t per = (cnt + kXcd - 1) / kXcd;
+    std::vector<T> out(per * kXcd, pad);
+    for (size_t q = 0; q < cnt; ++q) out[(q % per) * kXcd + q / per] = v[beg + q];
+    v.resize(beg);
+    v.insert(v.end(), out.begin(), out.end());
+}
+
 void build_precond(tabi_ctx *c)
 {
     const HostTree &T = c->tree;
@@ -2697,6 +2915,7 @@ void build_precond(tabi_ctx *c)
     c->piv.alloc(pof);
     c->lu.alloc(off);
     c->lu_info.alloc(nl);
+    c->lu_swrec.alloc((size_t)nl * kSwRec);
     HIPCHK(hipMemsetAsync(c->lu_info.p, 0, sizeof(int32_t) * nl, c->s));
     k_lu_assemble<<<c->n, 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
         c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
@@ -2707,38 +2926,62 @@ void build_precond(tabi_ctx *c)
     for (int l = 0; l < nl; ++l) lorder[l] = l;
     std::stable_sort(lorder.begin(), lorder.end(), [&](int a, int b) { return bm[a] > bm[b]; });
     std::vector<int> nact;
-    std::vector<int4> tl;
-    std::vector<int64_t> tl_off(1, 0);
+    std::vector<int4> tl, tlo;                  // in-strip / below-strip update tiles
+    std::vector<int64_t> tl_off(1, 0), tlo_off(1, 0);
     std::vector<int2> sw;                       // {block, column tile} of the interchange pass
     std::vector<int64_t> sw_off(1, 0);
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB) {
+        const int K0 = k0 - k0 % kLuW;
         int na = 0;
         while (na < nl && bm[lorder[na]] > k0) ++na;
         nact.push_back(na);
+        const size_t sw0 = sw.size(), tl0 = tl.size(), tlo0 = tlo.size();
         for (int q = 0; q < na; ++q)
             for (int ct = 0; ct * 64 < bm[lorder[q]]; ++ct) sw.push_back(make_int2(lorder[q], ct));
+        xcd_group(sw, sw0, make_int2(-1, 0));
         sw_off.push_back((int64_t)sw.size());
+        // after the panel: rows below it, strip columns right of it
         for (int q = 0; q < na; ++q) {
-            const int rem = bm[lorder[q]] - k0 - kLuNB;
-            if (rem <= 0) continue;
-            const int tt = (rem + 63) / 64;
-            for (int y = 0; y < tt; ++y)
-                for (int x = 0; x < tt; ++x) tl.push_back(make_int4(lorder[q], y, x, 0));
+            const int mq = bm[lorder[q]], send = std::min(mq, K0 + kLuW);
+            const int rem = mq - k0 - kLuNB, cols = send - k0 - kLuNB;
+            if (rem <= 0 || cols <= 0) continue;
+            for (int y = 0; y * 64 < rem; ++y)
+                for (int x = 0; x * 64 < cols; ++x) tl.push_back(make_int4(lorder[q], y, x, send));
         }
+        xcd_group(tl, tl0, make_int4(-1, 0, 0, 0));
         tl_off.push_back((int64_t)tl.size());
+        // after the strip's last panel: everything below and right of the strip
+        if (k0 + kLuNB == K0 + kLuW) {
+            for (int q = 0; q < na; ++q) {
+                const int mq = bm[lorder[q]], rem = mq - K0 - kLuW;
+                if (rem <= 0) continue;
+                for (int y = 0; y * 64 < rem; ++y)
+                    for (int x = 0; x * 64 < rem; ++x) tlo.push_back(make_int4(lorder[q], y, x, mq));
+            }
+        }
+        xcd_group(tlo, tlo0, make_int4(-1, 0, 0, 0));
+        tlo_off.push_back((int64_t)tlo.size());
     }
-    DBuf<int4> dtl;
+    DBuf<int4> dtl, dtlo;
     DBuf<int2> dsw;
     dsw.upload(sw.data(), std::max<size_t>(sw.size(), 1), c->s);
     c->lorder.upload(lorder.data(), nl, c->s);
     const int32_t *dlorder_p = c->lorder.p;
     if (!tl.empty()) dtl.upload(tl.data(), tl.size(), c->s);
+    if (!tlo.empty()) dtlo.upload(tlo.data(), tlo.size(), c->s);
+    // the panel launches of one step (one per size class) run side by side:
+    // the first on the library's stream, the others on side streams that
+    // fork from and join back into it
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    for (auto &e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     int step = 0;
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB, ++step) {
         const int na = nact[step];
         // one launch per size class of the remaining rows (lorder is sorted, so
         // each class is a contiguous range of active blocks)
-        int q = 0;
+        int q = 0, ncls = 0;
+        HIPCHK(hipEventRecord(ev_fork, c->s));
         while (q < na) {
             const int rem = bm[lorder[q]] - k0;
             int cls = rem > 1024 ? 2048 : rem > 512 ? 1024 : rem > 256 ? 512 : 256;
@@ -2746,27 +2989,39 @@ void build_precond(tabi_ctx *c)
             const int lo = cls == 256 ? 0 : cls / 2;
             while (e < na && bm[lorder[e]] - k0 > lo) ++e;
             const int cnt = e - q;
+            hipStream_t ps = c->s;
+            if (ncls > 0) {
+                ps = side_stream(ncls - 1);
+                HIPCHK(hipStreamWaitEvent(ps, ev_fork, 0));
+            }
             if (cls == 2048)
-                k_lu_panel<2, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+                k_lu_panel<2, 1024><<<cnt, 1024, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
             else if (cls == 1024)
-                k_lu_panel<1, 1024><<<cnt, 1024, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+                k_lu_panel<1, 1024><<<cnt, 1024, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
             else if (cls == 512)
-                k_lu_panel<1, 512><<<cnt, 512, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+                k_lu_panel<1, 512><<<cnt, 512, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
             else
-                k_lu_panel<1, 256><<<cnt, 256, 0, c->s>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p);
+                k_lu_panel<1, 256><<<cnt, 256, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
+            if (ncls > 0) HIPCHK(hipEventRecord(ev_join[ncls - 1], ps));
+            ++ncls;
             q = e;
         }
+        for (int k = 1; k < ncls; ++k) HIPCHK(hipStreamWaitEvent(c->s, ev_join[k - 1], 0));
         const int64_t nsw = sw_off[step + 1] - sw_off[step];
         k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
-            c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p);
+            c->blk_off.p, c->lu.p, c->lu_swrec.p);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
         if (nt > 0)
-            k_lu_gemm<<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
-                                                      c->blk_off.p, c->lu.p);
+            k_lu_gemm<kLuNB><<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
+                                                             c->blk_off.p, c->lu.p);
+        const int64_t nto = tlo_off[step + 1] - tlo_off[step];
+        if (nto > 0)
+            k_lu_gemm<kLuW><<<(unsigned)nto, 256, 0, c->s>>>(k0 + kLuNB - kLuW, dtlo.p + tlo_off[step],
+                                                             c->blk_m.p, c->blk_off.p, c->lu.p);
     }
     HIPCHK(hipGetLastError());
     c->perm.alloc(pof);
@@ -2775,6 +3030,8 @@ void build_precond(tabi_ctx *c)
     std::vector<int32_t> info(nl);
     HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
+    (void)hipEventDestroy(ev_fork);
+    for (auto &e : ev_join) (void)hipEventDestroy(e);
     for (int l = 0; l < nl; ++l)
         if (info[l]) throw Error("singular leaf block " + std::to_string(l + 1) + " (column " +
                                  std::to_string(info[l]) + ")");
