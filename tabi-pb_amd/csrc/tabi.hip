@@ -234,6 +234,7 @@ struct tabi_ctx {
     std::vector<int32_t> h_blk_m;
     DBuf<int64_t> blk_off;
     DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info, lorder, lu_swrec;
+    DBuf<double> lu_linv;
     DBuf<double> lu;
     // vectors
     DBuf<double> vx, vy, red;
@@ -1126,9 +1127,12 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info,
     int32_t *__restrict__ swrec)
 {
-    __shared__ double s_best[kPanelThreads / kWave];
-    __shared__ int s_bi[kPanelThreads / kWave];
-    __shared__ double s_row[2][kLuNB];
+    constexpr int NW = kPanelThreads / kWave;
+    // per column: the waves' maxima and rows, and the two exchanged rows;
+    // double-buffered by column parity, so a column step needs two barriers
+    __shared__ double s_best[2][NW];
+    __shared__ int s_bi[2][NW];
+    __shared__ double s_row[2][2][kLuNB];
     __shared__ int s_prow;
     __shared__ int32_t s_piv[kLuNB];
     __shared__ int s_dst[2 * kLuNB], s_src[2 * kLuNB];
@@ -1149,7 +1153,8 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
 #pragma clang loop unroll(full)
     for (int c = 0; c < kLuNB; ++c) {
         if (c < nbe) {  // nbe is uniform over the workgroup
-        const int col = k0 + c;
+        const int col = k0 + c, pb = c & 1;
+        // arg-max of |a[row][col]| over rows col..m-1, first maximum (IDAMAX)
         double best = -1.0;
         int bi = 0x7fffffff;
 #pragma unroll
@@ -1160,47 +1165,51 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
                 if (v > best) { best = v; bi = row; }
             }
         }
+        double wmax = best;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        if (lane == 0) { s_best[w] = best; s_bi[w] = bi; }
-        __syncthreads();
-        if (w == 0) {
-            best = lane < kPanelThreads / kWave ? s_best[lane] : -1.0;
-            bi = lane < kPanelThreads / kWave ? s_bi[lane] : 0x7fffffff;
+        for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
+        int wbi;
+        if (RPT == 1) {
+            // rows grow with the lane: the first lane holding the maximum
+            const u64 hit = __ballot(best == wmax);
+            wbi = __shfl(bi, __ffsll((long long)hit) - 1);
+        } else {
+            wbi = best == wmax ? bi : 0x7fffffff;
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-            }
-            if (lane == 0) { s_prow = bi; s_piv[c] = bi; pv[col] = bi; }
+            for (int off = 32; off > 0; off >>= 1) wbi = min(wbi, __shfl_xor(wbi, off));
         }
+        if (lane == 0) { s_best[pb][w] = wmax; s_bi[pb][w] = wbi; }
         __syncthreads();
-        const int prow = s_prow;
-        // exchange rows col and prow: both owners publish, then take the other
+        // every thread combines the waves' results (same order everywhere)
+        double gmax = s_best[pb][0];
+        int prow = s_bi[pb][0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+            const double v = s_best[pb][q];
+            const int r = s_bi[pb][q];
+            if (v > gmax || (v == gmax && r < prow)) { gmax = v; prow = r; }
+        }
+        if (t == 0) { s_piv[c] = prow; pv[col] = prow; }
+        // rows col and prow publish themselves, then trade
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int row = k0 + t + j * kPanelThreads;
             if (row == col || row == prow) {
 #pragma unroll
-                for (int cc = 0; cc < kLuNB; ++cc) s_row[row == col ? 0 : 1][cc] = p[j][cc];
+                for (int cc = 0; cc < kLuNB; ++cc) s_row[pb][row == col ? 0 : 1][cc] = p[j][cc];
             }
         }
         __syncthreads();
+        const int pr = prow == col ? 0 : 1;   // s_row[pb][pr] is the pivot row
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int row = k0 + t + j * kPanelThreads;
             if (prow != col && (row == col || row == prow)) {
 #pragma unroll
-                for (int cc = 0; cc < kLuNB; ++cc) p[j][cc] = s_row[row == col ? 1 : 0][cc];
+                for (int cc = 0; cc < kLuNB; ++cc) p[j][cc] = s_row[pb][row == col ? 1 : 0][cc];
             }
         }
-        // s_row[1] now holds the pivot row (old row prow) for everyone
-        const double pivot = s_row[prow == col ? 0 : 1][c];
+        const double pivot = s_row[pb][pr][c];
         if (pivot == 0.0) {
             bad = col + 1;
         } else {
@@ -1213,13 +1222,13 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
                     const double lv = p[j][c];
 #pragma unroll
                     for (int cc = c + 1; cc < kLuNB; ++cc)
-                        p[j][cc] = fma(-lv, s_row[prow == col ? 0 : 1][cc], p[j][cc]);
+                        p[j][cc] = fma(-lv, s_row[pb][pr][cc], p[j][cc]);
                 }
             }
         }
-        __syncthreads();
         }
     }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         const int row = k0 + t + j * kPanelThreads;
@@ -1259,6 +1268,49 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     }
 }
 
+// Inverse of the unit lower triangle L11 of panel k0 (rows and columns
+// k0..k0+nb-1, zero-padded to kLuNB) for every active block, so that
+// U12 = L11^-1 A12 becomes an independent product per entry.  One wave per
+// block; lane c < kLuNB forms column c by forward substitution.
+__global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
+    const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const double *__restrict__ lu,
+    double *__restrict__ linv)
+{
+    __shared__ double Ls[kLuNB][kLuNB + 1];
+    __shared__ double Xs[kLuNB][kLuNB + 1];
+    const int l = lorder[first + blockIdx.x], t = threadIdx.x;
+    const int m = blk_m[l];
+    const int nbe = min(kLuNB, m - k0);
+    const double *A = lu + blk_off[l];
+    double v[kLuNB * kLuNB / 64];
+#pragma unroll
+    for (int i = 0; i < kLuNB * kLuNB / 64; ++i) {
+        const int e = t + 64 * i, r = e % kLuNB, c = e / kLuNB;
+        v[i] = A[(size_t)(k0 + min(c, nbe - 1)) * m + k0 + min(r, nbe - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < kLuNB * kLuNB / 64; ++i) {
+        const int e = t + 64 * i, r = e % kLuNB, c = e / kLuNB;
+        Ls[r][c] = (c < r && r < nbe) ? v[i] : 0.0;
+        Xs[r][c] = r == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (t < kLuNB) {
+        for (int k = 0; k < nbe - 1; ++k) {
+            const double xk = Xs[k][t];
+            for (int r = k + 1; r < nbe; ++r) Xs[r][t] = fma(-Ls[r][k], xk, Xs[r][t]);
+        }
+    }
+    __syncthreads();
+    double *Li = linv + (size_t)l * kLuNB * kLuNB;
+#pragma unroll
+    for (int i = 0; i < kLuNB * kLuNB / 64; ++i) {
+        const int e = t + 64 * i;
+        Li[e] = Xs[e / kLuNB][e % kLuNB];
+    }
+}
+
 // Row interchanges of panel k0 on the columns outside the strip's factored
 // part, and U12 = L11^-1 A12 for the columns to the panel's right.  A
 // workgroup takes 64 columns of one active block; the touched rows (the panel
@@ -1271,9 +1323,9 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
 //                                      the panel rows, unit lower solve.
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    double *__restrict__ lu, const int32_t *__restrict__ swrec)
+    double *__restrict__ lu, const int32_t *__restrict__ swrec, const double *__restrict__ linv)
 {
-    __shared__ double L11[kLuNB][kLuNB + 1];
+    __shared__ double Li[kLuNB][kLuNB + 1];          // L11^-1 (row-major)
     __shared__ double T[2 * kLuNB][64 + 1];
     __shared__ double LL[kLuNB][kLuW - kLuNB + 1];   // L[k0..k0+nb)[K0..k0)
     __shared__ int rows[2 * kLuNB], src[2 * kLuNB];
@@ -1295,16 +1347,14 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
     // staging loops have fixed trip counts and clamped addresses, so their
     // loads are issued together
     if (right) {
+        const double *Lg = linv + (size_t)l * kLuNB * kLuNB;
         double v[kLuNB * kLuNB / 256];
 #pragma unroll
-        for (int i = 0; i < kLuNB * kLuNB / 256; ++i) {
-            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
-            v[i] = A[(size_t)(k0 + min(c, nbe - 1)) * m + k0 + min(r, nbe - 1)];
-        }
+        for (int i = 0; i < kLuNB * kLuNB / 256; ++i) v[i] = Lg[t + 256 * i];
 #pragma unroll
         for (int i = 0; i < kLuNB * kLuNB / 256; ++i) {
-            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
-            L11[r][c] = v[i];
+            const int e = t + 256 * i;
+            Li[e / kLuNB][e % kLuNB] = v[i];
         }
     }
     if (beyond && kl > 0) {
@@ -1362,15 +1412,27 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
         }
         __syncthreads();
     }
-    // U12 = L11^-1 A12 for the columns right of the panel, eliminated column
-    // by column: the updates for one pivot are independent of each other
-    if (t < 64) {
-        const int j = j0 + t;
-        if (right && j < m && j >= k0 + nbe) {
-            for (int c2 = 0; c2 < nbe - 1; ++c2) {
-                const double x = T[c2][t];
-                for (int c = c2 + 1; c < nbe; ++c) T[c][t] = fma(-L11[c][c2], x, T[c][t]);
+    // U12 = L11^-1 A12 for the columns right of the panel: thread (column
+    // t % 64, row group t / 64) forms rows rg + 4i of its column
+    if (right) {
+        const int jc = t % 64, rg = t / 64, j = j0 + jc;
+        const bool mine = j < m && j >= k0 + nbe;
+        double acc[kLuNB / 4];
+#pragma unroll
+        for (int i = 0; i < kLuNB / 4; ++i) acc[i] = 0.0;
+        if (mine) {
+#pragma unroll 4
+            for (int c = 0; c < nbe; ++c) {
+                const double tc = T[c][jc];
+#pragma unroll
+                for (int i = 0; i < kLuNB / 4; ++i) acc[i] = fma(Li[rg + 4 * i][c], tc, acc[i]);
             }
+        }
+        __syncthreads();
+        if (mine) {
+#pragma unroll
+            for (int i = 0; i < kLuNB / 4; ++i)
+                if (rg + 4 * i < nbe) T[rg + 4 * i][jc] = acc[i];
         }
     }
     __syncthreads();
@@ -2916,6 +2978,7 @@ void build_precond(tabi_ctx *c)
     c->lu.alloc(off);
     c->lu_info.alloc(nl);
     c->lu_swrec.alloc((size_t)nl * kSwRec);
+    c->lu_linv.alloc((size_t)nl * kLuNB * kLuNB);
     HIPCHK(hipMemsetAsync(c->lu_info.p, 0, sizeof(int32_t) * nl, c->s));
     k_lu_assemble<<<c->n, 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
         c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
@@ -3012,8 +3075,10 @@ void build_precond(tabi_ctx *c)
         }
         for (int k = 1; k < ncls; ++k) HIPCHK(hipStreamWaitEvent(c->s, ev_join[k - 1], 0));
         const int64_t nsw = sw_off[step + 1] - sw_off[step];
+        k_lu_linv<<<na, 64, 0, c->s>>>(k0, 0, dlorder_p, c->blk_m.p, c->blk_off.p, c->lu.p,
+                                       c->lu_linv.p);
         k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
-            c->blk_off.p, c->lu.p, c->lu_swrec.p);
+            c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
         if (nt > 0)
             k_lu_gemm<kLuNB><<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
