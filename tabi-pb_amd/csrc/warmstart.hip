@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict
     const int ii = valid ? i : n - 1;
     const float cx = (float)tx[ii], cy = (float)ty[ii], cz = (float)tz[ii];
     const float nx = (float)tnx[ii], ny = (float)tny[ii], nz = (float)tnz[ii];
-    const float i1 = 1.0f / (kR1 * kR1), i2 = 1.0f / (kR2 * kR2);
+    static_assert(kR2 == 2.0f * kR1, "the narrow window is derived from the wide one");
+    const float i2 = 1.0f / (kR2 * kR2);
     float d1 = INFINITY, d2 = INFINITY;          // squared distances, ascending
     float ax = 0.f, ay = 0.f, az = 0.f;          // nearest charge
     float p1 = 0.f, f1 = 0.f, p2 = 0.f, f2 = 0.f, bur = 0.f;
@@ -167,17 +168,17 @@ __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict
         for (int k = 0; k < na; ++k) {
             const float dx = sx[k] - cx, dy = sy[k] - cy, dz = sz[k] - cz;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < d2) {
-                if (r2 < d1) {
-                    d2 = d1; d1 = r2;
-                    ax = sx[k]; ay = sy[k]; az = sz[k];
-                } else {
-                    d2 = r2;
-                }
-            }
+            // the two nearest (squared) distances, first nearest charge kept
+            const bool closer = r2 < d1;
+            d2 = fminf(d2, fmaxf(d1, r2));
+            d1 = fminf(d1, r2);
+            ax = closer ? sx[k] : ax;
+            ay = closer ? sy[k] : ay;
+            az = closer ? sz[k] : az;
             if (r2 < 9.0f * kR2 * kR2 && r2 > 0.f) {
-                const float ir = 1.0f / sqrtf(r2);
-                const float w1 = __expf(-r2 * i1), w2 = __expf(-r2 * i2);
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                // kR1 = kR2 / 2: exp(-r2/kR1^2) = exp(-r2/kR2^2)^4
+                const float w2 = __expf(-r2 * i2), w2s = w2 * w2, w1 = w2s * w2s;
                 const float pot = sq[k] * ir;
                 const float fld = pot * (nx * dx + ny * dy + nz * dz) * ir * ir;
                 p1 += pot * w1; f1 += fld * w1;
