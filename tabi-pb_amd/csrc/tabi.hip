@@ -1820,36 +1820,6 @@ __global__ void __launch_bounds__(256) k_sum_part(int np, const double *__restri
     if (threadIdx.x == 0) *out = red[0];
 }
 
-__global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] += a * x[i];
-}
-
-__global__ void k_scal(int64_t n, double a, double *__restrict__ x)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] *= a;
-}
-
-// out = sum_k c_k v_k (k < nv), vectors at stride ld
-__global__ void k_lincomb(int64_t n, int nv, const double *__restrict__ coef,
-    const double *__restrict__ v, int64_t ld, double *__restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (int k = 0; k < nv; ++k) s += coef[k] * v[(int64_t)k * ld + i];
-    out[i] = s;
-}
-
-__global__ void k_sub(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
-    double *__restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = a[i] - b[i];
-}
-
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
