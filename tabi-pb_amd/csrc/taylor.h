@@ -15,6 +15,10 @@
 // With KAPPA == false the kernel is 1/|x-y| and every b_k other than b_000 = 1
 // vanishes, so those terms are dropped.
 //
+// In FP32 the reciprocal and square root are the hardware instructions (about
+// 1 ulp) and 1/|x-y| is formed as |x-y| * (1/|x-y|^2); FP64 keeps exact
+// division and square root.
+//
 // Multi-indices are stored compactly: degree d = i+j+k, s = j+k,
 // index = d(d+1)(d+2)/6 + s(s+1)/2 + k.  All loops have compile-time bounds and
 // are fully unrolled so that a[] and b[] live in registers.
@@ -48,13 +52,21 @@ __device__ __forceinline__ void taylor(T dx, T dy, T dz, T kappa, T (&a)[TaylorS
 #define B3(i, j, k) b[tidx(i, j, k)]
     const T ddx = T(2) * dx, ddy = T(2) * dy, ddz = T(2) * dz;
     const T kx = kappa * dx, ky = kappa * dy, kz = kappa * dz;
-    T dist = dx * dx + dy * dy + dz * dz;
-    const T fac = T(1) / dist;
-    dist = sqrt(dist);
+    const T dist2 = dx * dx + dy * dy + dz * dz;
+    T fac, dist, rdist;
+    if constexpr (sizeof(T) == 4) {
+        fac = __builtin_amdgcn_rcpf(dist2);
+        dist = __builtin_amdgcn_sqrtf(dist2);
+        rdist = dist * fac;
+    } else {
+        fac = T(1) / dist2;
+        dist = sqrt(dist2);
+        rdist = T(1) / dist;
+    }
 
     if constexpr (KAPPA) {
         B3(0, 0, 0) = exp(-kappa * dist);
-        A3(0, 0, 0) = B3(0, 0, 0) / dist;
+        A3(0, 0, 0) = sizeof(T) == 4 ? B3(0, 0, 0) * rdist : B3(0, 0, 0) / dist;
         B3(1, 0, 0) = kx * A3(0, 0, 0);
         B3(0, 1, 0) = ky * A3(0, 0, 0);
         B3(0, 0, 1) = kz * A3(0, 0, 0);
@@ -63,7 +75,7 @@ __device__ __forceinline__ void taylor(T dx, T dy, T dz, T kappa, T (&a)[TaylorS
         A3(0, 1, 0) = fac * dy * s0;
         A3(0, 0, 1) = fac * dz * s0;
     } else {
-        A3(0, 0, 0) = T(1) / dist;
+        A3(0, 0, 0) = rdist;
         A3(1, 0, 0) = fac * dx * A3(0, 0, 0);
         A3(0, 1, 0) = fac * dy * A3(0, 0, 0);
         A3(0, 0, 1) = fac * dz * A3(0, 0, 0);
