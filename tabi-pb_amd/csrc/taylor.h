@@ -15,9 +15,9 @@
 // With KAPPA == false the kernel is 1/|x-y| and every b_k other than b_000 = 1
 // vanishes, so those terms are dropped.
 //
-// In FP32 the reciprocal and square root are the hardware instructions (about
-// 1 ulp) and 1/|x-y| is formed as |x-y| * (1/|x-y|^2); FP64 keeps exact
-// division and square root.
+// In FP32 the reciprocal, square root and exponential are the hardware
+// instructions (a few ulp) and 1/|x-y| is formed as |x-y| * (1/|x-y|^2); FP64
+// keeps exact division, square root and exp.
 //
 // Multi-indices are stored compactly: degree d = i+j+k, s = j+k,
 // index = d(d+1)(d+2)/6 + s(s+1)/2 + k.  All loops have compile-time bounds and
@@ -65,8 +65,13 @@ __device__ __forceinline__ void taylor(T dx, T dy, T dz, T kappa, T (&a)[TaylorS
     }
 
     if constexpr (KAPPA) {
-        B3(0, 0, 0) = exp(-kappa * dist);
-        A3(0, 0, 0) = sizeof(T) == 4 ? B3(0, 0, 0) * rdist : B3(0, 0, 0) / dist;
+        if constexpr (sizeof(T) == 4) {
+            B3(0, 0, 0) = __expf(-kappa * dist);
+            A3(0, 0, 0) = B3(0, 0, 0) * rdist;
+        } else {
+            B3(0, 0, 0) = exp(-kappa * dist);
+            A3(0, 0, 0) = B3(0, 0, 0) / dist;
+        }
         B3(1, 0, 0) = kx * A3(0, 0, 0);
         B3(0, 1, 0) = ky * A3(0, 0, 0);
         B3(0, 0, 1) = kz * A3(0, 0, 0);
