@@ -281,7 +281,8 @@ def test_leaf_lu_with_heavy_pivoting(n, box, amax, leaf, oracle_lib):
     pivots far below the panel and pivot rows repeated within a panel: the
     preconditioner solve against the oracle's partial-pivot LU and against a
     dense solve of each block (numpy), within 1e-8 relative L2 (the blocks'
-    condition numbers reach ~1e7), and bit-identical between two set-ups."""
+    condition numbers reach ~1e7), bit-identical between two set-ups, and the
+    block solve equal to single solves per column."""
     import scipy.linalg as sl
     O = oracle_lib
     print(f"[gpu] leaf LU pivoting n={n} leaf={leaf}", flush=True)
@@ -317,6 +318,11 @@ def test_leaf_lu_with_heavy_pivoting(n, box, amax, leaf, oracle_lib):
     S2 = hip.Surface(hip.make_params(eps, kappa, 0.8, 3, leaf), xyz, nrm, area)
     assert np.array_equal(S2.msolve(r), z)
     S2.close()
+    # the block solve applies the same permuted factors to each column
+    R = np.stack([r, np.random.default_rng(4).standard_normal(2 * n)])
+    Z = S.msolve_block(R)
+    for k in range(2):
+        assert rel_l2(Z[k], S.msolve(R[k])) <= 1e-14
     S.close()
 
 
