@@ -1112,35 +1112,21 @@ __device__ __forceinline__ int lu_swap_record(int lane, int k0, int nbe, int pr,
     return nr;
 }
 
-// Panel factorisation of columns k0..k0+nb-1: partial pivoting with the
-// first-maximum rule (IDAMAX), reciprocal scaling and rank-1 updates restricted
-// to the panel (the unblocked step of DGETRF2's semantics,
-// src/dgetrf_dep.f:757-914).  One workgroup of kPanelThreads per block holds
-// the panel in registers, RPT rows per thread (row k0 + t + j*kPanelThreads);
-// each column step is a block-wide arg-max, a row exchange through LDS and a
-// broadcast of the pivot row.  Afterwards the panel's interchanges are applied
-// to the strip's earlier columns K0..k0-1 (staged in LDS).
+// Panel factorisation with partial pivoting, for a panel in which the
+// optimistic pass met a column whose diagonal entry is not its first maximum
+// in magnitude: the panel is reloaded and factored with the first-maximum rule
+// (IDAMAX): per column a block-wide arg-max (wave shuffles, a ballot for the
+// first maximum, the waves' results combined by every thread), a row exchange
+// through LDS and the rank-1 update; then stored.  Out of line, so the common
+// pass keeps its registers.
 template <int RPT, int kPanelThreads>
-__global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
-    const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
-    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
-    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info,
-    int32_t *__restrict__ swrec)
+__device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nbe,
+    double *__restrict__ A, int32_t *__restrict__ pv, int32_t *__restrict__ info_l,
+    double (*s_best)[kPanelThreads / kWave], int (*s_bi)[kPanelThreads / kWave],
+    double (*s_row)[2][kLuNB], int32_t *s_piv)
 {
     constexpr int NW = kPanelThreads / kWave;
-    // per column: the waves' maxima and rows, and the two exchanged rows;
-    // double-buffered by column parity, so a column step needs two barriers
-    __shared__ double s_best[2][NW];
-    __shared__ int s_bi[2][NW];
-    __shared__ double s_row[2][2][kLuNB];
-    __shared__ int s_prow;
-    __shared__ int32_t s_piv[kLuNB];
-    __shared__ int s_dst[2 * kLuNB], s_src[2 * kLuNB];
-    const int l = lorder[first + blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int m = blk_m[l];
-    const int nbe = min(kLuNB, m - k0);
-    double *A = lu + blk_off[l];
-    int32_t *pv = piv + piv_off[l];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     double p[RPT][kLuNB];
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
@@ -1228,6 +1214,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
         }
         }
     }
+    if (bad && t == 0) atomicCAS(info_l, 0, bad);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
@@ -1238,18 +1225,111 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
                 if (c < nbe) A[(size_t)(k0 + c) * m + row] = p[j][c];
         }
     }
-    if (bad && t == 0) atomicCAS(info + l, 0, bad);
+}
+
+// Panel factorisation of columns k0..k0+nb-1: partial pivoting with the
+// first-maximum rule (IDAMAX), reciprocal scaling and rank-1 updates restricted
+// to the panel (the unblocked step of DGETRF2's semantics,
+// src/dgetrf_dep.f:757-914).  One workgroup of kPanelThreads per block holds
+// the panel in registers, RPT rows per thread (row k0 + t + j*kPanelThreads);
+// each column step is a block-wide arg-max, a row exchange through LDS and a
+// broadcast of the pivot row.  Afterwards the panel's interchanges are applied
+// to the strip's earlier columns K0..k0-1 (staged in LDS).
+template <int RPT, int kPanelThreads>
+__global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
+    const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
+    const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
+    double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info,
+    int32_t *__restrict__ swrec)
+{
+    constexpr int NW = kPanelThreads / kWave;
+    // per column: the waves' maxima and rows, and the two exchanged rows;
+    // double-buffered by column parity, so a column step needs two barriers
+    __shared__ double s_best[2][NW];
+    __shared__ int s_bi[2][NW];
+    __shared__ double s_row[2][2][kLuNB];
+    __shared__ int s_prow;
+    __shared__ int32_t s_piv[kLuNB];
+    __shared__ int s_dst[2 * kLuNB], s_src[2 * kLuNB];
+    const int l = lorder[first + blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m = blk_m[l];
+    const int nbe = min(kLuNB, m - k0);
+    double *A = lu + blk_off[l];
+    int32_t *pv = piv + piv_off[l];
+    double p[RPT][kLuNB];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int row = k0 + t + j * kPanelThreads;
+#pragma unroll
+        for (int c = 0; c < kLuNB; ++c)
+            p[j][c] = (row < m && c < nbe) ? A[(size_t)(k0 + c) * m + row] : 0.0;
+    }
+    // Optimistic pass: each column's diagonal entry as its pivot, one barrier
+    // per column.  A thread notes when one of its rows below the diagonal is
+    // strictly larger in magnitude (IDAMAX takes the first maximum, so a tie
+    // keeps the diagonal) or the pivot is zero.  If no thread did, partial
+    // pivoting would have chosen the diagonal in every column and the factors
+    // are exactly those of the pivoted pass below (the same operations);
+    // otherwise the panel is reloaded and factored with partial pivoting.
+    int need = 0;
+#pragma clang loop unroll(full)
+    for (int c = 0; c < kLuNB; ++c) {
+        if (c < nbe) {
+        const int col = k0 + c, pb = c & 1;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int row = k0 + t + j * kPanelThreads;
+            if (row == col) {
+#pragma unroll
+                for (int cc = 0; cc < kLuNB; ++cc) s_row[pb][0][cc] = p[j][cc];
+            }
+        }
+        __syncthreads();
+        const double pivot = s_row[pb][0][c];
+        const double ap = fabs(pivot), rinv = 1.0 / pivot;
+        need |= pivot == 0.0;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int row = k0 + t + j * kPanelThreads;
+            if (row > col && row < m) {
+                need |= fabs(p[j][c]) > ap;
+                p[j][c] *= rinv;
+                const double lv = p[j][c];
+#pragma unroll
+                for (int cc = c + 1; cc < kLuNB; ++cc)
+                    p[j][cc] = fma(-lv, s_row[pb][0][cc], p[j][cc]);
+            }
+        }
+        }
+    }
+    const bool fast = !__syncthreads_or(need);
+    if (fast) {
+        if (t < nbe) { s_piv[t] = k0 + t; pv[k0 + t] = k0 + t; }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int row = k0 + t + j * kPanelThreads;
+        if (row < m) {
+#pragma unroll
+            for (int c = 0; c < kLuNB; ++c)
+                if (c < nbe) A[(size_t)(k0 + c) * m + row] = p[j][c];
+        }
+    }
+    } else {
+        lu_panel_pivoted<RPT, kPanelThreads>(k0, m, nbe, A, pv, info + l, s_best, s_bi, s_row,
+                                             s_piv);
+    }
+    __syncthreads();
     // interchange record for the other columns (read by k_lu_swap_trsm)
     int32_t *rec = swrec + (size_t)l * kSwRec;
     if (w == 0) {
         const int nr = lu_swap_record(lane, k0, nbe, lane < nbe ? s_piv[lane] : -1, s_dst, s_src);
         if (lane < nr) { rec[lane] = s_dst[lane]; rec[2 * kLuNB + lane] = s_src[lane]; }
-        if (lane == 0) { rec[4 * kLuNB] = nr; s_prow = nr; }
+        if (lane == 0) { rec[4 * kLuNB] = nr; rec[4 * kLuNB + 1] = fast; s_prow = nr; }
     }
     // the panel's interchanges on the strip's earlier columns K0..k0-1: every
     // touched row is read, then written to its destination
     const int K0 = k0 - k0 % kLuW, ncl = k0 - K0;
-    if (ncl == 0) return;
+    if (ncl == 0 || fast) return;
     __syncthreads();
     const int nr = s_prow;
     // element e = t + kPanelThreads i: row slot e % 64, column e / 64
@@ -1344,6 +1424,7 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
     if (t == 0) s_nr = rec[4 * kLuNB];
     const bool right = j0 + 63 >= k0 + nbe;    // the tile holds columns right of the panel
     const bool beyond = j0 + 63 >= send;       // ... and columns beyond the strip
+    if (!right && rec[4 * kLuNB + 1]) return;  // interchanges only, and the panel made none
     // staging loops have fixed trip counts and clamped addresses, so their
     // loads are issued together
     if (right) {
