@@ -16,8 +16,9 @@
 //   k_lu_solve               blocked forward/back substitution with the leaf
 //                            factors computed once at set-up
 // Set-up: k_lists (per-target MAC traversal with lane masks), k_lu_assemble,
-//   k_lu_panel / k_lu_swap_trsm / k_lu_gemm (batched right-looking LU with
-//   partial pivoting over all leaf blocks at once).
+//   k_lu_panel / k_lu_linv / k_lu_swap_trsm / k_lu_strip_trsm / k_lu_gemm
+//   (batched two-level blocked LU with partial pivoting over all leaf blocks
+//   at once).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -986,8 +987,9 @@ __global__ void k_reduce(int n, NearSlots ns, const int32_t *__restrict__ far_ta
 // leaf-block preconditioner
 // ---------------------------------------------------------------------------
 // Assemble every leaf block, column-major with leading dimension m = 2*nl, in
-// the reference's layout (src/treecode3d_pb.f:1439-1528): one workgroup per
-// source face j, threads over the targets i of j's leaf.  flags bit 0 selects
+// the reference's layout (src/treecode3d_pb.f:1439-1528): one wave per source
+// face j (four per workgroup), lanes over the targets i of j's leaf, so a
+// leaf of ~150 faces keeps most lanes busy.  flags bit 0 selects
 // the matvec-consistent placement of the L2/L3 couplings.
 __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__restrict__ leaf_of,
     const int32_t *__restrict__ blk_ibeg, const int32_t *__restrict__ blk_m,
@@ -997,16 +999,18 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
     const double *__restrict__ tnz, const double *__restrict__ tarea, double kappa,
     double eps, int flags, double *__restrict__ lu)
 {
-    const int j = blockIdx.x;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (j >= n) return;
     const int l = leaf_of[j];
     const int ib = blk_ibeg[l], m = blk_m[l], nl = m / 2;
     const int jj = j - ib;
     double *A = lu + blk_off[l];
-    const double pre1 = 0.5 * (1.0 + eps), pre2 = 0.5 * (1.0 + 1.0 / eps);
+    const double inv_eps = 1.0 / eps;
+    const double pre1 = 0.5 * (1.0 + eps), pre2 = 0.5 * (1.0 + inv_eps);
     const double rx = tx[j], ry = ty[j], rz = tz[j];
     const double vx = tnx[j], vy = tny[j], vz = tnz[j], aj = tarea[j];
     const bool swap23 = flags & 1;
-    for (int ii = threadIdx.x; ii < nl; ii += blockDim.x) {
+    for (int ii = lane; ii < nl; ii += kWave) {
         const int i = ib + ii;
         double e1, e2, e3, e4;
         if (i == j) {
@@ -1019,22 +1023,23 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
         const double dx = rx - tx[i], dy = ry - ty[i], dz = rz - tz[i];
         const double v0x = tnx[i], v0y = tny[i], v0z = tnz[i];
         const double rs = sqrt(dx * dx + dy * dy + dz * dz);
-        const double g0 = kOneOver4Pi / rs;
+        const double irs = 1.0 / rs;
+        const double g0 = kOneOver4Pi * irs;
         const double krs = kappa * rs;
         const double ek = exp(-krs);
         const double gk = ek * g0;
-        const double ct = (vx * dx + vy * dy + vz * dz) / rs;
-        const double ct0 = (v0x * dx + v0y * dy + v0z * dz) / rs;
-        const double tp1 = g0 / rs;
+        const double ct = (vx * dx + vy * dy + vz * dz) * irs;
+        const double ct0 = (v0x * dx + v0y * dy + v0z * dz) * irs;
+        const double tp1 = g0 * irs;
         const double tp2 = (1.0 + krs) * ek;
         const double g10 = ct0 * tp1, g20 = tp2 * g10;
         const double g1 = ct * tp1, g2 = tp2 * g1;
-        const double g3 = (v0x * vx + v0y * vy + v0z * vz - 3.0 * ct0 * ct) / rs * tp1;
+        const double g3 = (v0x * vx + v0y * vy + v0z * vz - 3.0 * ct0 * ct) * irs * tp1;
         const double g4 = tp2 * g3 - kappa * kappa * ct0 * ct * gk;
         e1 = -(g1 - eps * g2) * aj;
         e2 = -(g0 - gk) * aj;
         e3 = -(g4 - g3) * aj;
-        e4 = -(g10 - g20 / eps) * aj;
+        e4 = -(g10 - g20 * inv_eps) * aj;
         if (swap23) { const double t = e2; e2 = e3; e3 = t; }
         A[(size_t)jj * m + ii] = e1;
         A[(size_t)jj * m + ii + nl] = e2;
@@ -1050,13 +1055,12 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
 // Two-level blocking.  Columns are factored in strips of kLuW (four panels of
 // kLuNB).  Inside a strip the panels are right-looking over the strip's own
 // columns only.  The columns beyond the strip take the row interchanges of
-// every panel as they come, but none of the strip's updates until its end:
-// their rows of panel k0 are brought up to date left-looking (the panel rows
-// minus L[panel rows][K0..k0) U[K0..k0)[col]) just before the panel's unit
-// lower solve, and everything below the strip is updated once, by a rank-kLuW
-// product, when the strip is done.  Rows below the panel therefore always
-// hold either all or none of a given panel's update, so the interchanges move
-// consistent data.  The factors are DGETRF's (the unblocked steps of DGETF2
+// every panel as they come, but none of the strip's arithmetic until its end,
+// so their rows always hold the original entries (interchanged) and the
+// interchanges move consistent data.  When the strip is done, its rows of
+// those columns are solved at once, U12 = L11^-1 A12 with the 128x128 unit
+// lower triangle, and everything below the strip is updated by one rank-kLuW
+// product.  The factors are DGETRF's (the unblocked steps of DGETF2
 // within each panel, src/dgetrf_dep.f:757-914); only the order in which
 // the trailing products are accumulated differs.
 constexpr int kLuW = 4 * kLuNB;   // strip width
@@ -1351,7 +1355,10 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
 // Inverse of the unit lower triangle L11 of panel k0 (rows and columns
 // k0..k0+nb-1, zero-padded to kLuNB) for every active block, so that
 // U12 = L11^-1 A12 becomes an independent product per entry.  One wave per
-// block; lane c < kLuNB forms column c by forward substitution.
+// block; lane c < kLuNB forms column c by forward substitution.  The inverse
+// is stored column-major in the block's slot for the panel's place in its
+// strip (kLuW / kLuNB slots per block), where the strip's final solve finds
+// all four.
 __global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
     const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const double *__restrict__ lu,
@@ -1383,31 +1390,26 @@ __global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
         }
     }
     __syncthreads();
-    double *Li = linv + (size_t)l * kLuNB * kLuNB;
+    double *Li = linv + ((size_t)l * (kLuW / kLuNB) + (k0 % kLuW) / kLuNB) * kLuNB * kLuNB;
 #pragma unroll
     for (int i = 0; i < kLuNB * kLuNB / 64; ++i) {
         const int e = t + 64 * i;
-        Li[e] = Xs[e / kLuNB][e % kLuNB];
+        Li[e] = Xs[e % kLuNB][e / kLuNB];
     }
 }
 
-// Row interchanges of panel k0 on the columns outside the strip's factored
-// part, and U12 = L11^-1 A12 for the columns to the panel's right.  A
-// workgroup takes 64 columns of one active block; the touched rows (the panel
-// rows and the pivot rows below them) are staged in LDS with coalesced loads.
-//   columns left of the strip          interchanges only;
-//   strip columns K0..k0+nb-1          untouched (the panel kernel's part);
-//   strip columns right of the panel   interchanges, unit lower solve (their
-//                                      rows are up to date: in-strip updates);
-//   columns beyond the strip           interchanges, left-looking update of
-//                                      the panel rows, unit lower solve.
+// Row interchanges of panel k0 on every column outside the strip's factored
+// part, and U12 = L11^-1 A12 for the strip's columns right of the panel (their
+// rows are up to date: inside the strip the factorisation is right-looking).
+// A workgroup takes 64 columns of one active block; the touched rows (the
+// panel rows and the pivot rows below them) are staged in LDS with coalesced
+// loads.  Columns left of the strip and beyond it take the interchanges only.
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     double *__restrict__ lu, const int32_t *__restrict__ swrec, const double *__restrict__ linv)
 {
     __shared__ double Li[kLuNB][kLuNB + 1];          // L11^-1 (row-major)
     __shared__ double T[2 * kLuNB][64 + 1];
-    __shared__ double LL[kLuNB][kLuW - kLuNB + 1];   // L[k0..k0+nb)[K0..k0)
     __shared__ int rows[2 * kLuNB], src[2 * kLuNB];
     __shared__ int s_nr;
     const int2 tl = tiles[blockIdx.x];           // {block, column tile}
@@ -1415,42 +1417,27 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
     if (l < 0) return;                         // padding of the XCD-grouped list
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
-    const int K0 = k0 - k0 % kLuW, kl = k0 - K0, send = min(m, K0 + kLuW);
+    const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
     const int j0 = tl.y * 64;
     double *A = lu + blk_off[l];
     const int32_t *rec = swrec + (size_t)l * kSwRec;
     const int t = threadIdx.x;
     if (t < 2 * kLuNB) { rows[t] = rec[t]; src[t] = rec[2 * kLuNB + t]; }
     if (t == 0) s_nr = rec[4 * kLuNB];
-    const bool right = j0 + 63 >= k0 + nbe;    // the tile holds columns right of the panel
-    const bool beyond = j0 + 63 >= send;       // ... and columns beyond the strip
+    // the tile holds strip columns right of the panel
+    const bool right = j0 + 63 >= k0 + nbe && j0 < send;
     if (!right && rec[4 * kLuNB + 1]) return;  // interchanges only, and the panel made none
     // staging loops have fixed trip counts and clamped addresses, so their
     // loads are issued together
     if (right) {
-        const double *Lg = linv + (size_t)l * kLuNB * kLuNB;
+        const double *Lg = linv + ((size_t)l * (kLuW / kLuNB) + (k0 - K0) / kLuNB) * kLuNB * kLuNB;
         double v[kLuNB * kLuNB / 256];
 #pragma unroll
         for (int i = 0; i < kLuNB * kLuNB / 256; ++i) v[i] = Lg[t + 256 * i];
 #pragma unroll
         for (int i = 0; i < kLuNB * kLuNB / 256; ++i) {
             const int e = t + 256 * i;
-            Li[e / kLuNB][e % kLuNB] = v[i];
-        }
-    }
-    if (beyond && kl > 0) {
-        // a strip only extends beyond itself once its panels are full (nb = kLuNB)
-        constexpr int kN = kLuNB * (kLuW - kLuNB) / 256;
-        double v[kN];
-#pragma unroll
-        for (int i = 0; i < kN; ++i) {
-            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
-            v[i] = A[(size_t)(K0 + min(c, kl - 1)) * m + k0 + r];
-        }
-#pragma unroll
-        for (int i = 0; i < kN; ++i) {
-            const int e = t + 256 * i, r = e % kLuNB, c = e / kLuNB;
-            if (c < kl) LL[r][c] = v[i];
+            Li[e % kLuNB][e / kLuNB] = v[i];
         }
     }
     __syncthreads();
@@ -1472,32 +1459,11 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
         }
     }
     __syncthreads();    // the rows are staged in their post-interchange order
-    if (beyond && kl > 0) {
-        // panel rows of the columns beyond the strip: minus L[rows][K0..k0) U[K0..k0)[j];
-        // thread (column t%64, row group t/64)
-        // (rows rg + 4i, i < 8: a strip always has full panels before this one)
-        const int jc = t % 64, rg = t / 64, j = j0 + jc;
-        if (j < m && j >= send) {
-            const double *U = A + (size_t)j * m + K0;
-            double s[kLuNB / 4];
-#pragma unroll
-            for (int i = 0; i < kLuNB / 4; ++i) s[i] = T[rg + 4 * i][jc];
-#pragma unroll 4
-            for (int c = 0; c < kl; ++c) {
-                const double u = U[c];
-#pragma unroll
-                for (int i = 0; i < kLuNB / 4; ++i) s[i] = fma(-LL[rg + 4 * i][c], u, s[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < kLuNB / 4; ++i) T[rg + 4 * i][jc] = s[i];
-        }
-        __syncthreads();
-    }
-    // U12 = L11^-1 A12 for the columns right of the panel: thread (column
-    // t % 64, row group t / 64) forms rows rg + 4i of its column
+    // U12 = L11^-1 A12 for the strip columns right of the panel: thread
+    // (column t % 64, row group t / 64) forms rows rg + 4i of its column
     if (right) {
         const int jc = t % 64, rg = t / 64, j = j0 + jc;
-        const bool mine = j < m && j >= k0 + nbe;
+        const bool mine = j < send && j >= k0 + nbe;
         double acc[kLuNB / 4];
 #pragma unroll
         for (int i = 0; i < kLuNB / 4; ++i) acc[i] = 0.0;
@@ -1524,6 +1490,104 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
     }
 }
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// U12 = L11^-1 A12 for the columns beyond a finished strip K0..K0+kLuW-1,
+// whose strip rows still hold their original (interchanged) entries.  A
+// workgroup takes 64 columns of one block, staged in LDS column-major; wave w
+// owns columns 16w..16w+15 and solves them panel by panel on the FP64 matrix
+// cores, R = A_p - L[rows of p][K0..K0+32p) X and X_p = (L11_p)^-1 R with the
+// panel inverses of k_lu_linv, so no wave reads another wave's columns.
+// Operands as in k_lu_gemm: A L[row = lane%16][k = lane/16] straight from
+// global memory (16 consecutive rows of 4 columns), B X[k = lane/16][col =
+// lane%16] from LDS; result register v = D[lane/16 + 4v][lane%16].
+constexpr int kStS = kLuW + 2;   // LDS column stride in doubles: conflict-free operand reads
+__global__ void __launch_bounds__(256, 2) k_lu_strip_trsm(int K0, const int2 *__restrict__ tiles,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    double *__restrict__ lu, const double *__restrict__ linv)
+{
+    __shared__ double T[64 * kStS];
+    const int2 tl = tiles[blockIdx.x];           // {block, column tile beyond the strip}
+    const int l = tl.x;
+    if (l < 0) return;                         // padding of the XCD-grouped list
+    const int m = blk_m[l];
+    const int j0 = K0 + kLuW + tl.y * 64;
+    double *A = lu + blk_off[l];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+    // stage rows K0..K0+kLuW-1 of the tile's columns: element e = t + 256 i is
+    // row e % kLuW of column e / kLuW (columns past m are clamped, never stored)
+    constexpr int kTs = kLuW * 64 / 256;
+    {
+        double v[kTs];
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + 256 * i;
+            v[i] = A[(size_t)min(j0 + e / kLuW, m - 1) * m + K0 + e % kLuW];
+        }
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + 256 * i;
+            T[(e / kLuW) * kStS + e % kLuW] = v[i];
+        }
+    }
+    __syncthreads();
+    const double *L = A + (size_t)K0 * m + K0;  // L11, column-major, leading dimension m
+    const double *Lv = linv + (size_t)l * (kLuW / kLuNB) * kLuNB * kLuNB;
+    double *X = T + (16 * w + li) * kStS;       // this lane's column
+    // a wave's LDS accesses complete in order, so the lanes of one wave read
+    // each other's results below without a barrier
+#pragma unroll
+    for (int p = 0; p < kLuW / kLuNB; ++p) {
+        // the panel's A operands are loaded together before its products, so
+        // their latency is paid once per panel rather than once per product
+        const double *Li = Lv + p * kLuNB * kLuNB;  // column-major 32x32
+        double al[2][(kLuW - kLuNB) / 4], ai[2][kLuNB / 4];
+#pragma unroll
+        for (int q = 0; q < kLuNB * p / 4; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                al[h][q] = -L[(size_t)(4 * q + lk) * m + kLuNB * p + 16 * h + li];
+#pragma unroll
+        for (int q = 0; q < kLuNB / 4; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) ai[h][q] = Li[(4 * q + lk) * kLuNB + 16 * h + li];
+        f64x4 acc[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[h][v] = X[kLuNB * p + 16 * h + lk + 4 * v];
+#pragma unroll
+        for (int q = 0; q < kLuNB * p / 4; ++q) {
+            const double b = X[4 * q + lk];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(al[h][q], b, acc[h], 0, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) X[kLuNB * p + 16 * h + lk + 4 * v] = acc[h][v];
+        f64x4 x[2] = {f64x4{0.0, 0.0, 0.0, 0.0}, f64x4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int q = 0; q < kLuNB / 4; ++q) {
+            const double b = X[kLuNB * p + 4 * q + lk];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                x[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[h][q], b, x[h], 0, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) X[kLuNB * p + 16 * h + lk + 4 * v] = x[h][v];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kTs; ++i) {
+        const int e = t + 256 * i, c = e / kLuW;
+        if (j0 + c < m) A[(size_t)(j0 + c) * m + K0 + e % kLuW] = T[c * kStS + e % kLuW];
+    }
+}
+
 // Trailing update A22 -= L21 U12 on 64x64 tiles from a precomputed tile list
 // {block, tile row, tile col, column end} on the FP64 matrix cores.  L21 holds
 // the columns kb..kb+KD-1 (KD = kLuNB for the in-strip update after a panel,
@@ -1532,7 +1596,6 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
 // end.  Each of the four waves owns a 32x32 quadrant as 2x2 blocks of
 // v_mfma_f64_16x16x4f64: A operand L21[row = lane%16][k = lane/16], B operand
 // U12[k = lane/16][col = lane%16], result register v = D[lane/16 + 4v][lane%16].
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <int KD>
 __global__ void __launch_bounds__(256) k_lu_gemm(int kb, const int4 *__restrict__ tiles,
@@ -2969,8 +3032,81 @@ void build_lists(tabi_ctx *c)
 // list is laid out so that XCD x receives the contiguous run
 // [x*per, (x+1)*per) of the logical order (per = ceil(count/8)): tiles of
 // one leaf block, which share their L and U panels, then meet in one L2.
-// Padding entries carry block -1 and exit at once.
+// Padding entries carry block -1 and exit at once.  Lists of at most 8 tiles
+// keep the logical order.
 constexpr int kXcd = 8;
+
+// The tile lists of the batched LU, one entry per workgroup of a launch.
+enum LuTileKind : int {
+    kTilesSwap = 0,    // {block, column tile}: interchanges / in-strip solve
+    kTilesPanel = 1,   // {block, tile row, tile col, strip end}: update after a panel
+    kTilesStrip = 2,   // {block, column tile}: solve of the columns beyond a strip
+    kTilesBelow = 3,   // {block, tile row, tile col, m}: update below a strip
+    kTileKinds = 4
+};
+
+// Tiles of one kind that block m contributes at panel step k0 (the logical
+// order within a block is row-major over its tile grid).
+__host__ __device__ inline void lu_tile_grid(int kind, int k0, int mq, int &ny, int &nx)
+{
+    const int K0 = k0 - k0 % kLuW, send = mq < K0 + kLuW ? mq : K0 + kLuW;
+    const bool strip_end = k0 + kLuNB == K0 + kLuW;
+    ny = 1; nx = 0;
+    if (kind == kTilesSwap) {
+        nx = (mq + 63) / 64;
+    } else if (kind == kTilesPanel) {
+        const int rem = mq - k0 - kLuNB, cols = send - k0 - kLuNB;
+        if (rem > 0 && cols > 0) { ny = (rem + 63) / 64; nx = (cols + 63) / 64; }
+    } else if (strip_end && mq > K0 + kLuW) {
+        nx = (mq - K0 - kLuW + 63) / 64;
+        ny = kind == kTilesBelow ? nx : 1;
+    }
+}
+
+// Expands one kind's per-step tile counts into its XCD-grouped list.  Step s
+// (k0 = s * kLuNB) owns entries [off[s], off[s+1]) and the exclusive prefix
+// pre[poff[s] + i] of the tile counts of the active blocks lorder[0..na[s]);
+// an entry's logical index q is mapped back through the XCD layout and its
+// block found by bisection.  One thread per entry.
+__global__ void k_lu_tiles(int kind, int nsteps, const int64_t *__restrict__ off,
+    const int32_t *__restrict__ cnt, const int32_t *__restrict__ poff,
+    const int32_t *__restrict__ na, const int32_t *__restrict__ pre,
+    const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m, int32_t *__restrict__ out)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= off[nsteps]) return;
+    int s = 0;
+    while (off[s + 1] <= b) ++s;
+    const int64_t r = b - off[s];
+    const int c = cnt[s];
+    int64_t q = r;
+    if (c > kXcd) {
+        const int64_t per = (c + kXcd - 1) / kXcd;
+        q = (r % kXcd) * per + r / kXcd;
+    }
+    int4 v = make_int4(-1, 0, 0, 0);
+    if (q < c) {
+        const int32_t *pr = pre + poff[s];
+        int lo = 0, hi = na[s];                // last i with pr[i] <= q
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (pr[mid] <= q) lo = mid; else hi = mid;
+        }
+        const int l = lorder[lo], idx = (int)(q - pr[lo]), mq = blk_m[l];
+        const int k0 = s * kLuNB;
+        int ny, nx;
+        lu_tile_grid(kind, k0, mq, ny, nx);
+        const int K0 = k0 - k0 % kLuW;
+        if (kind == kTilesPanel) v = make_int4(l, idx / nx, idx % nx, min(mq, K0 + kLuW));
+        else if (kind == kTilesBelow) v = make_int4(l, idx / nx, idx % nx, mq);
+        else v = make_int4(l, idx, 0, 0);
+    }
+    if (kind == kTilesPanel || kind == kTilesBelow) {
+        reinterpret_cast<int4 *>(out)[b] = v;
+    } else {
+        reinterpret_cast<int2 *>(out)[b] = make_int2(v.x, v.y);
+    }
+}
 
 // Side streams (default priority) for launches that may run beside the
 // library's stream, one set per device, created on first use and kept.
@@ -3029,9 +3165,16 @@ void build_precond(tabi_ctx *c)
     c->lu.alloc(off);
     c->lu_info.alloc(nl);
     c->lu_swrec.alloc((size_t)nl * kSwRec);
-    c->lu_linv.alloc((size_t)nl * kLuNB * kLuNB);
+    c->lu_linv.alloc((size_t)nl * (kLuW / kLuNB) * kLuNB * kLuNB);
     HIPCHK(hipMemsetAsync(c->lu_info.p, 0, sizeof(int32_t) * nl, c->s));
-    k_lu_assemble<<<c->n, 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
+    // the tile lists below are uploaded on a side stream that waits for the
+    // work queued so far but not for the assembly, so the copies overlap it
+    hipEvent_t ev_fork = nullptr, ev_up = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
+    for (auto &e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev_up, c->s));
+    k_lu_assemble<<<nblk(c->n, 4), 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
         c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
         c->prm.kappa, c->prm.eps, c->prm.flags, c->lu.p);
     HIPCHK(hipGetLastError());
@@ -3039,56 +3182,69 @@ void build_precond(tabi_ctx *c)
     std::vector<int32_t> lorder(nl);
     for (int l = 0; l < nl; ++l) lorder[l] = l;
     std::stable_sort(lorder.begin(), lorder.end(), [&](int a, int b) { return bm[a] > bm[b]; });
-    std::vector<int> nact;
-    std::vector<int4> tl, tlo;                  // in-strip / below-strip update tiles
-    std::vector<int64_t> tl_off(1, 0), tlo_off(1, 0);
-    std::vector<int2> sw;                       // {block, column tile} of the interchange pass
-    std::vector<int64_t> sw_off(1, 0);
-    for (int k0 = 0; k0 < c->mmax; k0 += kLuNB) {
-        const int K0 = k0 - k0 % kLuW;
-        int na = 0;
-        while (na < nl && bm[lorder[na]] > k0) ++na;
-        nact.push_back(na);
-        const size_t sw0 = sw.size(), tl0 = tl.size(), tlo0 = tlo.size();
-        for (int q = 0; q < na; ++q)
-            for (int ct = 0; ct * 64 < bm[lorder[q]]; ++ct) sw.push_back(make_int2(lorder[q], ct));
-        xcd_group(sw, sw0, make_int2(-1, 0));
-        sw_off.push_back((int64_t)sw.size());
-        // after the panel: rows below it, strip columns right of it
-        for (int q = 0; q < na; ++q) {
-            const int mq = bm[lorder[q]], send = std::min(mq, K0 + kLuW);
-            const int rem = mq - k0 - kLuNB, cols = send - k0 - kLuNB;
-            if (rem <= 0 || cols <= 0) continue;
-            for (int y = 0; y * 64 < rem; ++y)
-                for (int x = 0; x * 64 < cols; ++x) tl.push_back(make_int4(lorder[q], y, x, send));
-        }
-        xcd_group(tl, tl0, make_int4(-1, 0, 0, 0));
-        tl_off.push_back((int64_t)tl.size());
-        // after the strip's last panel: everything below and right of the strip
-        if (k0 + kLuNB == K0 + kLuW) {
-            for (int q = 0; q < na; ++q) {
-                const int mq = bm[lorder[q]], rem = mq - K0 - kLuW;
-                if (rem <= 0) continue;
-                for (int y = 0; y * 64 < rem; ++y)
-                    for (int x = 0; x * 64 < rem; ++x) tlo.push_back(make_int4(lorder[q], y, x, mq));
-            }
-        }
-        xcd_group(tlo, tlo0, make_int4(-1, 0, 0, 0));
-        tlo_off.push_back((int64_t)tlo.size());
+    // per-step tile counts of each list kind; the lists themselves are
+    // expanded on the GPU (k_lu_tiles) beside the assembly
+    const int nst = (c->mmax + kLuNB - 1) / kLuNB;
+    std::vector<int> nact(nst);
+    for (int st = 0; st < nst; ++st) {
+        int na = st ? nact[st - 1] : nl;
+        while (na > 0 && bm[lorder[na - 1]] <= st * kLuNB) --na;
+        nact[st] = na;
     }
+    // tab: per kind cnt[nst] and poff[nst], then na[nst], then the prefixes
+    std::vector<int32_t> tab((size_t)(2 * kTileKinds + 1) * nst);
+    std::vector<int64_t> toff((size_t)kTileKinds * (nst + 1), 0);
+    for (int st = 0; st < nst; ++st) tab[(size_t)2 * kTileKinds * nst + st] = nact[st];
+    for (int kind = 0; kind < kTileKinds; ++kind) {
+        int64_t *o = toff.data() + (size_t)kind * (nst + 1);
+        for (int st = 0; st < nst; ++st) {
+            tab[(size_t)(2 * kind + 1) * nst + st] = (int32_t)tab.size();
+            int64_t run = 0;
+            for (int q = 0; q < nact[st]; ++q) {
+                tab.push_back((int32_t)run);
+                int ny, nx;
+                lu_tile_grid(kind, st * kLuNB, bm[lorder[q]], ny, nx);
+                run += (int64_t)ny * nx;
+            }
+            if (run > INT32_MAX) throw Error("too many LU tiles in one step");
+            tab[(size_t)2 * kind * nst + st] = (int32_t)run;
+            o[st + 1] = o[st] + (run > kXcd ? (run + kXcd - 1) / kXcd * kXcd : run);
+        }
+    }
+    const int64_t *sw_off = toff.data(), *tl_off = sw_off + (nst + 1);
+    const int64_t *ts_off = tl_off + (nst + 1), *tlo_off = ts_off + (nst + 1);
     DBuf<int4> dtl, dtlo;
-    DBuf<int2> dsw;
-    dsw.upload(sw.data(), std::max<size_t>(sw.size(), 1), c->s);
-    c->lorder.upload(lorder.data(), nl, c->s);
+    DBuf<int2> dsw, dts;
+    DBuf<int32_t> dtab;
+    DBuf<int64_t> dtoff;
+    {
+        const hipStream_t up = side_stream(0);
+        HIPCHK(hipStreamWaitEvent(up, ev_up, 0));
+        c->lorder.upload(lorder.data(), nl, up);
+        dtab.upload(tab.data(), tab.size(), up);
+        dtoff.upload(toff.data(), toff.size(), up);
+        dsw.alloc(std::max<int64_t>(sw_off[nst], 1));
+        dtl.alloc(std::max<int64_t>(tl_off[nst], 1));
+        dts.alloc(std::max<int64_t>(ts_off[nst], 1));
+        dtlo.alloc(std::max<int64_t>(tlo_off[nst], 1));
+        int32_t *outs[kTileKinds] = {reinterpret_cast<int32_t *>(dsw.p), reinterpret_cast<int32_t *>(dtl.p),
+                                     reinterpret_cast<int32_t *>(dts.p), reinterpret_cast<int32_t *>(dtlo.p)};
+        for (int kind = 0; kind < kTileKinds; ++kind) {
+            const int64_t tot = toff[(size_t)kind * (nst + 1) + nst];
+            if (tot == 0) continue;
+            k_lu_tiles<<<(unsigned)((tot + 255) / 256), 256, 0, up>>>(kind, nst,
+                dtoff.p + (size_t)kind * (nst + 1), dtab.p + (size_t)2 * kind * nst,
+                dtab.p + (size_t)(2 * kind + 1) * nst, dtab.p + (size_t)2 * kTileKinds * nst, dtab.p,
+                c->lorder.p, c->blk_m.p, outs[kind]);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev_up, up));
+        HIPCHK(hipStreamWaitEvent(c->s, ev_up, 0));
+    }
     const int32_t *dlorder_p = c->lorder.p;
-    if (!tl.empty()) dtl.upload(tl.data(), tl.size(), c->s);
-    if (!tlo.empty()) dtlo.upload(tlo.data(), tlo.size(), c->s);
     // the panel launches of one step (one per size class) run side by side:
     // the first on the library's stream, the others on side streams that
     // fork from and join back into it
-    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
-    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    for (auto &e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     int step = 0;
     for (int k0 = 0; k0 < c->mmax; k0 += kLuNB, ++step) {
         const int na = nact[step];
@@ -3134,6 +3290,10 @@ void build_precond(tabi_ctx *c)
         if (nt > 0)
             k_lu_gemm<kLuNB><<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
                                                              c->blk_off.p, c->lu.p);
+        const int64_t nts = ts_off[step + 1] - ts_off[step];
+        if (nts > 0)
+            k_lu_strip_trsm<<<(unsigned)nts, 256, 0, c->s>>>(k0 + kLuNB - kLuW, dts.p + ts_off[step],
+                c->blk_m.p, c->blk_off.p, c->lu.p, c->lu_linv.p);
         const int64_t nto = tlo_off[step + 1] - tlo_off[step];
         if (nto > 0)
             k_lu_gemm<kLuW><<<(unsigned)nto, 256, 0, c->s>>>(k0 + kLuNB - kLuW, dtlo.p + tlo_off[step],
@@ -3147,6 +3307,7 @@ void build_precond(tabi_ctx *c)
     HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     (void)hipEventDestroy(ev_fork);
+    (void)hipEventDestroy(ev_up);
     for (auto &e : ev_join) (void)hipEventDestroy(e);
     for (int l = 0; l < nl; ++l)
         if (info[l]) throw Error("singular leaf block " + std::to_string(l + 1) + " (column " +
