@@ -2530,42 +2530,6 @@ __global__ void k_energy_reduceb(int nq, int nchunk, int R, const double *__rest
 // found through a pointer table, tab[slot*R + r], and have length nv[r]; the
 // column list act maps grid column blockIdx.y to a run.  Per run the arithmetic
 // is that of the single-vector kernels (same partition and summation order).
-__global__ void __launch_bounds__(256) k_dot_partt(double *const *__restrict__ tab, int R, int sa,
-    int sb, const int64_t *__restrict__ nv, const int32_t *__restrict__ act, double *__restrict__ part)
-{
-    __shared__ double red[256];
-    const int col = blockIdx.y, r = act[col];
-    const double *ar = tab[(size_t)sa * R + r], *br = tab[(size_t)sb * R + r];
-    const int64_t n = nv[r];
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        s += ar[i] * br[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[(size_t)col * gridDim.x + blockIdx.x] = red[0];
-}
-
-// out[act[col]*os + oofs] = sum of column col's partials
-__global__ void __launch_bounds__(256) k_sum_partb(int np, const double *__restrict__ part,
-    double *__restrict__ out, int64_t os, int oofs, const int32_t *__restrict__ act)
-{
-    __shared__ double red[256];
-    const int col = blockIdx.x;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += 256) s += part[(size_t)col * np + i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[(size_t)act[col] * os + oofs] = red[0];
-}
-
 // y += (sgn * coef[r*cs + cofs]) * x
 __global__ void k_axpyt(double *const *__restrict__ tab, int R, int sx, int sy,
     const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int cofs,
@@ -2602,6 +2566,173 @@ __global__ void k_lincombt(double *const *__restrict__ tab, int R, int sv0, int 
     tab[(size_t)so * R + r][i] = s;
 }
 
+// Fused forms for the GMRES inner loop, with the per-element arithmetic, the
+// partition (k_dot_part's grid-stride over gridDim.x = kRedBlocks blocks) and
+// the summation order (k_sum_part's) of the unfused sequence (axpy, dot
+// partials, sum of partials), so their results are bit-identical to it.  A dot
+// leaves its kRedBlocks partials in part[(r*dvs + slot)*kRedBlocks ...]; a
+// kernel that needs the value (the next step's axpy coefficient) sums them
+// again in every block, in the same order, so no block waits for another and
+// no cross-XCD fence is needed; k_sum_slots forms the values the host reads.
+
+// elements per load group of a thread: vectors of up to kFuseU * kRedBlocks *
+// 256 (~1M) entries take a single group
+constexpr int kFuseU = 8;
+
+// sum of np partials in k_sum_part's order (all 256 threads of the block)
+__device__ __forceinline__ double sum_partials(const double *p, int np, double *red)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += p[i];
+    __syncthreads();
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// y = slot sy.  When sx >= 0, first y -= h * slot(sx) (k_axpyt with sgn -1)
+// where h is the dot of slot hs, summed from its partials.  Then the partials of
+// slot(sa) . y into slot o1 when o1 >= 0, and of y . y into slot o2 when
+// o2 >= 0.  Grid (kRedBlocks, columns).
+__global__ void __launch_bounds__(256) k_axpy_dott(double *const *__restrict__ tab, int R, int sx,
+    int sy, int sa, const int64_t *__restrict__ nv, int hs, const int32_t *__restrict__ act,
+    double *__restrict__ part, int64_t dvs, int o1, int o2)
+{
+    __shared__ double red[2][256];
+    const int col = blockIdx.y, r = act[col], t = threadIdx.x;
+    double *y = tab[(size_t)sy * R + r];
+    const double *x = tab[(size_t)(sx >= 0 ? sx : sy) * R + r];
+    const double *a = tab[(size_t)(o1 >= 0 ? sa : sy) * R + r];
+    const int64_t n = nv[r];
+    double *pr = part + (size_t)r * dvs * gridDim.x;
+    const double c = sx >= 0 ? -1.0 * sum_partials(pr + (size_t)hs * gridDim.x, gridDim.x, red[0]) : 0.0;
+    double s1 = 0.0, s2 = 0.0;
+    // a thread's elements i0 + u*stride in order, kFuseU at a time: all loads
+    // of a group are issued before its stores, so they are in flight together
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i0 = (int64_t)blockIdx.x * 256 + t; i0 < n; i0 += stride * kFuseU) {
+        double yv[kFuseU], xv[kFuseU], av[kFuseU];
+#pragma unroll
+        for (int u = 0; u < kFuseU; ++u) {
+            const int64_t i = i0 + u * stride;
+            const bool ok = i < n;
+            yv[u] = ok ? y[i] : 0.0;
+            xv[u] = ok && sx >= 0 ? x[i] : 0.0;
+            av[u] = ok && o1 >= 0 && sa != sy ? a[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kFuseU; ++u) {
+            const int64_t i = i0 + u * stride;
+            if (i >= n) break;
+            double yi = yv[u];
+            if (sx >= 0) {
+                yi += c * xv[u];
+                y[i] = yi;
+            }
+            if (o1 >= 0) s1 += (sa == sy ? yi : av[u]) * yi;
+            if (o2 >= 0) s2 += yi * yi;
+        }
+    }
+    __syncthreads();
+    red[0][t] = s1;
+    red[1][t] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) { red[0][t] += red[0][t + w]; red[1][t] += red[1][t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (o1 >= 0) pr[(size_t)o1 * gridDim.x + blockIdx.x] = red[0][0];
+        if (o2 >= 0) pr[(size_t)o2 * gridDim.x + blockIdx.x] = red[1][0];
+    }
+}
+
+// Coefficients of one run passed by value (kernel arguments), for launches
+// over a single run: no upload precedes them.
+constexpr int kCoefArg = 32;
+struct CoefArg {
+    double v[kCoefArg];
+};
+
+// out = sum_{k < nvv} coef[r*cs + k] * slot(sv0 + k) (k_lincombt), the
+// partials of out . out into slot o, and slot(sv0 + nvv - 1) *= coef[r*cs +
+// csc] when csc >= 0 (k_scalt, after the slot was read for the combination).
+// With by_value the single run's coefficients come from ca instead.
+__global__ void __launch_bounds__(256) k_lincomb_dott(double *const *__restrict__ tab, int R, int sv0,
+    int nvv, const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int so,
+    int csc, const int32_t *__restrict__ act, double *__restrict__ part, int64_t dvs, int o,
+    bool by_value, CoefArg ca)
+{
+    __shared__ double red[256];
+    const int col = blockIdx.y, r = act[col], t = threadIdx.x;
+    const double *cr = by_value ? ca.v : coef + (size_t)r * cs;
+    double *out = tab[(size_t)so * R + r];
+    double *vl = tab[(size_t)(sv0 + nvv - 1) * R + r];
+    const double sc = csc >= 0 ? cr[csc] : 1.0;
+    const int64_t n = nv[r];
+    double s2 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i0 = (int64_t)blockIdx.x * 256 + t; i0 < n; i0 += stride * kFuseU) {
+        double sv[kFuseU], lv[kFuseU];
+#pragma unroll
+        for (int u = 0; u < kFuseU; ++u) sv[u] = 0.0;
+        for (int k = 0; k < nvv; ++k) {
+            const double ck = cr[k], *vk = tab[(size_t)(sv0 + k) * R + r];
+#pragma unroll
+            for (int u = 0; u < kFuseU; ++u) {
+                const int64_t i = i0 + u * stride;
+                const double v = i < n ? vk[i] : 0.0;
+                sv[u] += ck * v;
+                if (k == nvv - 1) lv[u] = v;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFuseU; ++u) {
+            const int64_t i = i0 + u * stride;
+            if (i >= n) break;
+            out[i] = sv[u];
+            if (csc >= 0) vl[i] = lv[u] * sc;
+            s2 += sv[u] * sv[u];
+        }
+    }
+    red[t] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[((size_t)r * dvs + o) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// hval[r*dvs + slot] = sum of the slot's np partials (k_sum_part's order) for
+// the slots [s0, s1) and the extra slots e0, e1 (when >= 0); one block per
+// column.  hval and hflag are host memory (pinned, coherent, mapped): after
+// its values, each block publishes hflag[r] = seq with system-scope release,
+// and the host polls the flag instead of waiting for a copy.
+__global__ void __launch_bounds__(256) k_sum_slots(int np, const double *__restrict__ part,
+    double *__restrict__ hval, int64_t *__restrict__ hflag, int64_t seq, int64_t dvs, int s0, int s1,
+    int e0, int e1, const int32_t *__restrict__ act)
+{
+    __shared__ double red[256];
+    const int r = act[blockIdx.x];
+    const double *pr = part + (size_t)r * dvs * np;
+    for (int k = s0; k < s1 + 2; ++k) {
+        const int slot = k < s1 ? k : k == s1 ? e0 : e1;
+        if (slot < 0) continue;
+        const double v = sum_partials(pr + (size_t)slot * np, np, red);
+        if (threadIdx.x == 0) hval[(size_t)r * dvs + slot] = v;
+    }
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(hflag + r, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+
 // out = a - b
 __global__ void k_subt(double *const *__restrict__ tab, int R, int sa, int sb, int so,
     const int64_t *__restrict__ nv, const int32_t *__restrict__ act)
@@ -2632,12 +2763,12 @@ __global__ void k_fillt(double *const *__restrict__ tab, int R, int so, const in
 // host orchestration
 // ---------------------------------------------------------------------------
 template <int P>
-void launch_moments(tabi_ctx *c)
+void launch_moments(tabi_ctx *c, const double *x)
 {
     if (c->nmom == 0) return;
     k_mom_raw<P><<<c->nchunk, 256, 0, c->s>>>(c->n, c->chunk_node.p, c->chunk_beg.p,
         c->chunk_end.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
-        c->tarea.p, c->vx.p, c->mom_part.p);
+        c->tarea.p, x, c->mom_part.p);
     k_mom_final<P><<<c->nmom, 256, 0, c->s>>>(c->mom_chunk_off.p, c->mom_part.p, c->prm.eps,
                                               c->mom_w.p);
 }
@@ -2659,15 +2790,15 @@ void launch_far(tabi_ctx *c)
         c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->part_far.p);
 }
 
-void dispatch_moments(tabi_ctx *c)
+void dispatch_moments(tabi_ctx *c, const double *x)
 {
     switch (c->p) {
-    case 1: launch_moments<1>(c); break;
-    case 2: launch_moments<2>(c); break;
-    case 3: launch_moments<3>(c); break;
-    case 4: launch_moments<4>(c); break;
-    case 5: launch_moments<5>(c); break;
-    case 6: launch_moments<6>(c); break;
+    case 1: launch_moments<1>(c, x); break;
+    case 2: launch_moments<2>(c, x); break;
+    case 3: launch_moments<3>(c, x); break;
+    case 4: launch_moments<4>(c, x); break;
+    case 5: launch_moments<5>(c, x); break;
+    case 6: launch_moments<6>(c, x); break;
     default: throw Error("unsupported Taylor order");
     }
 }
@@ -2694,20 +2825,22 @@ NearSlots near_slots(const tabi_ctx *c)
 
 void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
 {
-    // the moment and field kernels read x from c->vx
-    if (dx != c->vx.p)
+    // the kernels read x in place; an in-place product (dy == dx) reads a copy
+    const double *x = dx;
+    if (dx == dy) {
         HIPCHK(hipMemcpyAsync(c->vx.p, dx, sizeof(double) * 2 * c->n, hipMemcpyDeviceToDevice, c->s));
-    { KTimer kt(c, 0); dispatch_moments(c); }
+        x = c->vx.p;
+    }
+    { KTimer kt(c, 0); dispatch_moments(c, x); }
     if (c->n_near_task) {
         KTimer kt(c, 1);
         if (c->prm.precision == 1) {
-            k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p,
-                                                               c->src32.p);
+            k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src32.p);
             k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src32.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
                 (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p);
         } else {
-            k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, c->vx.p, c->src8.p);
+            k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src8.p);
             k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src8.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
                 c->prm.eps, c->part_near.p);
@@ -2718,7 +2851,7 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     {
         KTimer kt(c, 3);
         k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, near_slots(c), c->far_task_off.p,
-            c->part_near.p, c->part_far.p, c->vx.p, pre1, pre2, dy);
+            c->part_near.p, c->part_far.p, x, pre1, pre2, dy);
     }
     HIPCHK(hipGetLastError());
 }
@@ -3436,6 +3569,84 @@ void ctx_stats(tabi_ctx *c, int64_t *o)
 }
 
 
+// Events that are destroyed with their owner.
+struct EventSet {
+    std::vector<hipEvent_t> e;
+    explicit EventSet(int count) : e(count, nullptr)
+    {
+        for (auto &x : e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    }
+    ~EventSet()
+    {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+// Waits for an event by polling: the wake-up latency of a blocking wait is
+// tens of microseconds, several times the work between the GMRES round trips.
+void spin_wait(hipEvent_t ev)
+{
+    for (;;) {
+        const hipError_t q = hipEventQuery(ev);
+        if (q == hipSuccess) return;
+        if (q != hipErrorNotReady) HIPCHK(q);
+    }
+}
+
+constexpr int kCoefRing = 4;   // coefficient upload slots
+
+// Pinned host memory for the GMRES scalars of one host thread, kept for the
+// life of the thread: val and flag are coherent and mapped (written by
+// k_sum_slots, read by the host while kernels run), coef stages uploads.
+struct PinnedScalars {
+    double *val = nullptr, *coef = nullptr, *dval = nullptr;
+    int64_t *flag = nullptr, *dflag = nullptr;
+    size_t nval = 0, nflag = 0, ncoef = 0;
+    int64_t seq = 0;           // last flag value published
+};
+
+PinnedScalars &pinned_scalars(size_t nval, size_t nflag, size_t ncoef)
+{
+    thread_local PinnedScalars ps;
+    if (nval > ps.nval || nflag > ps.nflag || ncoef > ps.ncoef) {
+        (void)hipDeviceSynchronize();   // nothing may still use the old buffers
+        if (ps.val) (void)hipHostFree(ps.val);
+        if (ps.flag) (void)hipHostFree(ps.flag);
+        if (ps.coef) (void)hipHostFree(ps.coef);
+        ps = PinnedScalars{};
+        const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ps.val), sizeof(double) * nval, fl));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ps.flag), sizeof(int64_t) * nflag, fl));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ps.coef), sizeof(double) * ncoef, hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&ps.dval), ps.val, 0));
+        HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&ps.dflag), ps.flag, 0));
+        std::memset(ps.flag, 0, sizeof(int64_t) * nflag);
+        ps.nval = nval;
+        ps.nflag = nflag;
+        ps.ncoef = ncoef;
+    }
+    return ps;
+}
+
+// Waits until flag[r] == seq for every listed run.  The stream is queried
+// now and then: if it has drained and a flag is still short, the kernel that
+// should have raised it did not, and the wait ends with an error.
+void wait_flags(const int64_t *flag, const std::vector<int32_t> &cols, int64_t seq, hipStream_t st)
+{
+    for (size_t k = 0; k < cols.size();) {
+        if (__atomic_load_n(flag + cols[k], __ATOMIC_ACQUIRE) == seq) { ++k; continue; }
+        for (int spin = 0; spin < 4096; ++spin)
+            if (__atomic_load_n(flag + cols[k], __ATOMIC_ACQUIRE) == seq) break;
+        if (__atomic_load_n(flag + cols[k], __ATOMIC_ACQUIRE) == seq) { ++k; continue; }
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipErrorNotReady) continue;
+        HIPCHK(q);
+        if (__atomic_load_n(flag + cols[k], __ATOMIC_ACQUIRE) != seq)
+            throw Error("GMRES scalars were not delivered");
+    }
+}
+
 // ---- R runs of GMRES in lockstep -----------------------------------------
 // Slots of the per-run vector table: fixed work vectors, then the Krylov basis.
 enum : int { kSlotR = 0, kSlotZ, kSlotWK, kSlotDL, kSlotB, kSlotX, kSlotT1, kSlotT2, kSlotV0 };
@@ -3484,10 +3695,9 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
     int64_t nmax = 0;
     for (int64_t v : rs.n) nmax = std::max(nmax, v);
     const unsigned nb = nblk(nmax, 256);
-    DBuf<double> coef, dval;
+    DBuf<double> coef;
     coef.alloc((size_t)R * cst);
-    dval.alloc((size_t)R * dvs);
-    red.alloc((size_t)R * kRedBlocks);
+    red.alloc((size_t)R * dvs * kRedBlocks);          // dot partials per run and slot
     double *const *T = rs.tab.p;
     const int64_t *NV = rs.dn.p;
     auto V = [&](int k) { return kSlotV0 + k; };
@@ -3499,20 +3709,41 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
         cur = v;
         d.upload(v.data(), std::max<size_t>(v.size(), 1), st);
     };
-    std::vector<double> hv((size_t)R * dvs), hc((size_t)R * cst);
-    auto fetch = [&]() {
-        HIPCHK(hipMemcpyAsync(hv.data(), dval.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> hc((size_t)R * cst);
+    // Scalars travel through pinned host memory.  The sums the host reads are
+    // written there by k_sum_slots, which then raises a per-run flag that the
+    // host polls (no copy, no blocking wait); coefficient uploads are
+    // asynchronous, each from its own slot of a small ring that is reused only
+    // after its copy has completed.
+    PinnedScalars &ps = pinned_scalars((size_t)R * dvs, R, kCoefRing * hc.size());
+    double *const hv = ps.val;
+    int64_t seq = ps.seq;
+    EventSet evs(kCoefRing);
+    int ring = 0;
+    auto sum_fetch = [&](const std::vector<int32_t> &cols, const int32_t *dcols, int s0, int s1, int e0,
+                         int e1) {
+        ++seq;
+        k_sum_slots<<<(unsigned)cols.size(), 256, 0, st>>>(kRedBlocks, red.p, ps.dval, ps.dflag, seq, dvs,
+                                                           s0, s1, e0, e1, dcols);
+        HIPCHK(hipGetLastError());
+        ps.seq = seq;
+        wait_flags(ps.flag, cols, seq, st);
     };
     auto put_coef = [&]() {
-        HIPCHK(hipMemcpyAsync(coef.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, st));
+        double *slot = ps.coef + (size_t)ring * hc.size();
+        hipEvent_t done = evs.e[ring];
+        spin_wait(done);
+        std::memcpy(slot, hc.data(), sizeof(double) * hc.size());
+        HIPCHK(hipMemcpyAsync(coef.p, slot, sizeof(double) * hc.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(done, st));
+        ring = (ring + 1) % kCoefRing;
     };
     auto gy = [&](size_t cnt) { return dim3(nb, (unsigned)cnt); };
-    // dval[r][slot] = (a, b) for the listed runs, dev_dot's partition and order
-    auto dot = [&](const std::vector<int32_t> &cols, const int32_t *dcols, int sa, int sb, int slot) {
-        k_dot_partt<<<dim3(kRedBlocks, (unsigned)cols.size()), 256, 0, st>>>(T, R, sa, sb, NV, dcols,
-                                                                              red.p);
-        k_sum_partb<<<(unsigned)cols.size(), 256, 0, st>>>(kRedBlocks, red.p, dval.p, dvs, slot, dcols);
+    // hv[r*dvs + slot] = (a, b) for the listed runs, dev_dot's partition and order
+    auto dot_fetch = [&](const std::vector<int32_t> &cols, const int32_t *dcols, int sa, int sb, int slot) {
+        k_axpy_dott<<<dim3(kRedBlocks, (unsigned)cols.size()), 256, 0, st>>>(T, R, -1, sb, sa, NV, 0,
+                                                                             dcols, red.p, dvs, slot, -1);
+        sum_fetch(cols, dcols, slot, slot + 1, -1, -1);
     };
 
     std::vector<int32_t> all(R);
@@ -3520,8 +3751,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
     put(dact_a, cur_a, all);
     // initial residual
     rs.msolve(all, dact_a.p, kSlotB, kSlotR);
-    dot(all, dact_a.p, kSlotR, kSlotR, sT);
-    fetch();
+    dot_fetch(all, dact_a.p, kSlotR, kSlotR, sT);
     std::vector<double> bnrm(R), r0nrm(R), err(R, 0.0), prod(R), rho(R), snormw(R), vnrm(R);
     std::vector<char> guessed(R, 0);        // run started from a nonzero x0
     std::vector<int> iter(R, 0), ierr(R, 0), lgmr(R), iflag(R);
@@ -3571,8 +3801,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
             rs.msolve(active, dact_a.p, kSlotWK, kSlotR);
         }
         k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotR, V(0), NV, dact_a.p);
-        dot(active, dact_a.p, V(0), V(0), sT);
-        fetch();
+        dot_fetch(active, dact_a.p, V(0), V(0), sT);
         for (int r : active) r0nrm[r] = std::sqrt(hv[(size_t)r * dvs + sT]);
         if (nrsts == 0 && fallback) {
             // an initial guess worse than none: those runs restart from x0 = 0
@@ -3614,13 +3843,15 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
             const unsigned ni = (unsigned)inner.size();
             for (int r : inner) lgmr[r] = ll;
             rs.apply(inner, dact_b.p, V(ll - 1), V(ll));
-            dot(inner, dact_b.p, V(ll), V(ll), sN);
-            for (int i = 0; i < ll; ++i) {
-                dot(inner, dact_b.p, V(i), V(ll), i);
-                k_axpyt<<<gy(ni), 256, 0, st>>>(T, R, V(i), V(ll), NV, dval.p, dvs, i, -1.0, dact_b.p);
-            }
-            dot(inner, dact_b.p, V(ll), V(ll), sS);
-            fetch();
+            // modified Gram-Schmidt (DORTH): |w|^2 and (V(0), w) in one pass,
+            // then each step's w -= h_i V(i) fused with the next step's dot,
+            // and the last one with |w|^2
+            const dim3 gr(kRedBlocks, ni);
+            k_axpy_dott<<<gr, 256, 0, st>>>(T, R, -1, V(ll), V(0), NV, 0, dact_b.p, red.p, dvs, 0, sN);
+            for (int i = 1; i <= ll; ++i)
+                k_axpy_dott<<<gr, 256, 0, st>>>(T, R, V(i - 1), V(ll), V(i), NV, i - 1, dact_b.p, red.p, dvs,
+                                                i < ll ? i : -1, i < ll ? -1 : sS);
+            sum_fetch(inner, dact_b.p, 0, ll, sN, sS);
             std::vector<int32_t> reo;
             for (int r : inner) {
                 double *hcol = &hes[r][(size_t)(ll - 1) * ldh];
@@ -3636,8 +3867,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
                 dreo.upload(reo.data(), reo.size(), st);
                 for (int i = 0; i < ll; ++i) {
                     std::vector<int32_t> dr;
-                    dot(reo, dreo.p, V(i), V(ll), sT);
-                    fetch();
+                    dot_fetch(reo, dreo.p, V(i), V(ll), sT);
                     for (int r : reo) {
                         double *hcol = &hes[r][(size_t)(ll - 1) * ldh];
                         const double tem = -hv[(size_t)r * dvs + sT];
@@ -3705,12 +3935,19 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
             }
             std::vector<int32_t> next;
             if (!go.empty()) {
-                put_coef();
+                // DRLCAL's residual and its norm, and the new basis vector
+                // scaled by 1/|w| for the next step, in one pass
+                for (int r : go) hc[(size_t)r * cst + maxl + 1] = 1.0 / snormw[r];
+                // one run: its coefficients as kernel arguments, no upload
+                const bool by_value = go.size() == 1 && cst <= kCoefArg;
+                CoefArg ca{};
+                if (by_value) std::copy_n(&hc[(size_t)go[0] * cst], cst, ca.v);
+                else put_coef();
                 put(dact_b, cur_b, go);
-                k_lincombt<<<gy(go.size()), 256, 0, st>>>(T, R, V(0), ll + 1, NV, coef.p, cst, kSlotDL,
-                                                         dact_b.p);
-                dot(go, dact_b.p, kSlotDL, kSlotDL, sT);
-                fetch();
+                k_lincomb_dott<<<dim3(kRedBlocks, (unsigned)go.size()), 256, 0, st>>>(T, R, V(0), ll + 1,
+                    NV, coef.p, cst, kSlotDL, ll < maxl ? maxl + 1 : -1, dact_b.p, red.p, dvs, sT,
+                    by_value, ca);
+                sum_fetch(go, dact_b.p, sT, sT + 1, -1, -1);
                 for (int r : go) {
                     err[r] = std::sqrt(hv[(size_t)r * dvs + sT]) / bnrm[r];
                     if (err[r] <= tol) conv[r] = 1;
@@ -3719,12 +3956,6 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
             }
             if (ll == maxl) break;
             inner = next;
-            if (!inner.empty()) {
-                for (int r : inner) hc[(size_t)r * cst] = 1.0 / snormw[r];
-                put_coef();
-                put(dact_b, cur_b, inner);
-                k_scalt<<<gy(inner.size()), 256, 0, st>>>(T, R, V(ll), NV, coef.p, cst, 0, dact_b.p);
-            }
         }
         // finish the cycle for every active run
         std::vector<int32_t> add_z, still;
