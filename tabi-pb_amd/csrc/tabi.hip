@@ -1813,39 +1813,36 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
 // ---------------------------------------------------------------------------
 // right-hand side and energy
 // ---------------------------------------------------------------------------
+// Charge records {x, y, z, q} for the right-hand side.
+__global__ void k_charge_rec(int nq, const double *__restrict__ qpos, const double *__restrict__ q,
+                             double4 *__restrict__ rec)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nq) rec[k] = make_double4(qpos[3 * k], qpos[3 * k + 1], qpos[3 * k + 2], q[k]);
+}
+
 // b_i = sum_k q_k/(4 pi eps_p r), b_{n+i} = sum_k q_k cos/(4 pi eps_p r^2)
-// (src/main.f90:597-607), one thread per face, charges staged through LDS
-__global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double *__restrict__ qpos,
-    const double *__restrict__ q, double eps_p, const double *__restrict__ tx,
+// (src/main.f90:597-607), one thread per face, the charge records read
+// wave-uniformly (scalar loads, no LDS staging)
+__global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double4 *__restrict__ qrec,
+    double eps_p, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
     const double *__restrict__ tnz, double *__restrict__ b)
 {
-    __shared__ double sq[4][256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
     const double vx = valid ? tnx[i] : 0.0, vy = valid ? tny[i] : 0.0, vz = valid ? tnz[i] : 0.0;
     double b1 = 0.0, b2 = 0.0;
-    for (int k0 = 0; k0 < nq; k0 += 256) {
-        const int k = k0 + threadIdx.x;
-        __syncthreads();
-        if (k < nq) {
-            sq[0][threadIdx.x] = qpos[3 * k];
-            sq[1][threadIdx.x] = qpos[3 * k + 1];
-            sq[2][threadIdx.x] = qpos[3 * k + 2];
-            sq[3][threadIdx.x] = q[k];
-        }
-        __syncthreads();
-        const int cnt = min(256, nq - k0);
 #pragma unroll 4
-        for (int c = 0; c < cnt; ++c) {
-            const double dx = sq[0][c] - px, dy = sq[1][c] - py, dz = sq[2][c] - pz;
-            const double inv = rsq_nr(dx * dx + dy * dy + dz * dz);
-            const double ct = (vx * dx + vy * dy + vz * dz) * inv;
-            b1 += sq[3][c] * inv;
-            b2 += sq[3][c] * ct * inv * inv;
-        }
+    for (int k = 0; k < nq; ++k) {
+        const double4 c = qrec[k];
+        const double dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const double inv = rsq_nr(dx * dx + dy * dy + dz * dz);
+        const double ct = (vx * dx + vy * dy + vz * dz) * inv;
+        b1 += c.w * inv;
+        b2 += c.w * ct * inv * inv;
     }
     if (valid) {
         const double s = 1.0 / (4.0 * kPi * eps_p);
@@ -4078,7 +4075,10 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
              bool dev = false)
 {
     DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
-    k_rhs<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, nq, dqp.p, dq.p, eps_p, c->tx.p, c->ty.p,
+    DBuf<double4> qrec;
+    qrec.alloc(std::max(nq, 1));
+    if (nq > 0) k_charge_rec<<<nblk(nq, 256), 256, 0, c->s>>>(nq, dqp.p, dq.p, qrec.p);
+    k_rhs<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, nq, qrec.p, eps_p, c->tx.p, c->ty.p,
         c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, db);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
