@@ -43,8 +43,8 @@ METRIC = "PB solves/hour at ~500k faces, ΔG err<1e-3 vs FP64; GMRES iters learn
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE runs of `bench.py --kernels-only 5` on the default workload,
-# each counter scaled by what it reads for the matvec's 7,737,152-byte copy of
-# x (FETCH_SIZE x 1.994, WRITE_SIZE x 1.000): {(kernel, faces): bytes}
+# each counter scaled by what it reads for the in-place matvec's 7,737,152-byte
+# copy of x (FETCH_SIZE x 1.994, WRITE_SIZE x 1.000): {(kernel, faces): bytes}
 TRAFFIC = {("k_far32", 483572): 333.0e6 + 50.0e6, ("k_near32", 483572): 410.4e6 + 98.8e6}
 
 
@@ -252,8 +252,10 @@ def main():
         dx = hip.DeviceArray.from_host(np.random.default_rng(0).standard_normal(2 * S.n))
         dy = hip.DeviceArray(2 * S.n)
         for _ in range(a.kernels_only):
-            S.matvec_dev(dx.ptr, dy.ptr)
-            S.msolve_dev(dy.ptr, dx.ptr)
+            # in place: the library first copies x (7,737,152 B on the default
+            # surface), the copy of known size the traffic counters are scaled by
+            S.matvec_dev(dx.ptr, dx.ptr)
+            S.msolve_dev(dx.ptr, dy.ptr)
         S.sync()
         print(f"[bench] {a.kernels_only} matvec + msolve on {S.n} faces", file=sys.stderr, flush=True)
         return
