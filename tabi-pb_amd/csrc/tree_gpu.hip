@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -197,14 +198,20 @@ __global__ void k_part_move(int n, const Seg *__restrict__ segs, int nseg,
     ord2[dest] = ord[p];
 }
 
-// min/max of x, y, z over each range [b, e]: one workgroup per range
+// min/max of x, y, z over each range [b, e], in chunks of kMinMaxChunk points:
+// workgroup (r, c) reduces chunk c of range r into part[(r*gridDim.y + c)*6 ..]
+// (chunks past the range's end write nothing); k_seg_minmax_fin combines them.
+// Min and max do not depend on the order, so the boxes are exact.
+constexpr int kMinMaxChunk = 4096;
 __global__ void __launch_bounds__(256) k_seg_minmax(const int32_t *__restrict__ rng,
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-    double *__restrict__ out)
+    double *__restrict__ part)
 {
     __shared__ double sh[6][256];
-    const int r = blockIdx.x, t = threadIdx.x;
-    const int b = rng[2 * r], e = rng[2 * r + 1];
+    const int r = blockIdx.x, c = blockIdx.y, t = threadIdx.x;
+    const int b0 = rng[2 * r], e0 = rng[2 * r + 1];
+    const int b = b0 + c * kMinMaxChunk, e = min(e0, b + kMinMaxChunk - 1);
+    if (c > 0 && b > e0) return;   // chunk 0 always writes (+-inf for an empty range)
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int p = b + t; p <= e; p += 256) {
         const double v[3] = {x[p], y[p], z[p]};
@@ -227,7 +234,40 @@ __global__ void __launch_bounds__(256) k_seg_minmax(const int32_t *__restrict__ 
         }
         __syncthreads();
     }
-    if (t < 6) out[6 * r + t] = sh[t][0];
+    if (t < 6) part[((size_t)r * gridDim.y + c) * 6 + t] = sh[t][0];
+}
+
+// out[6r + k] = min (k even) or max (k odd) over range r's chunk partials
+__global__ void k_seg_minmax_fin(int nr, int nch, const int32_t *__restrict__ rng,
+                                 const double *__restrict__ part, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 6 * nr) return;
+    const int r = i / 6, k = i % 6;
+    const int len = rng[2 * r + 1] - rng[2 * r];
+    const int cnt = len < 0 ? 1 : len / kMinMaxChunk + 1;
+    const double *p = part + (size_t)r * nch * 6 + k;
+    double v = p[0];
+    for (int c = 1; c < cnt; ++c) v = (k & 1) ? fmax(v, p[(size_t)c * 6]) : fmin(v, p[(size_t)c * 6]);
+    out[i] = v;
+}
+
+// identity permutation
+__global__ void k_iota(int n, int32_t *__restrict__ a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// Waits for an event by polling (a blocking wait wakes tens of microseconds
+// late, once per tree level).
+void spin_wait(hipEvent_t ev)
+{
+    for (;;) {
+        const hipError_t q = hipEventQuery(ev);
+        if (q == hipSuccess) return;
+        if (q != hipErrorNotReady) TCHK(q);
+    }
 }
 
 template <typename T>
@@ -265,26 +305,56 @@ void build_tree_gpu(int32_t n, const double *d_x, const double *d_y, const doubl
     TCHK(hipMemcpyAsync(X[0], d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     TCHK(hipMemcpyAsync(Y[0], d_y, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     TCHK(hipMemcpyAsync(Z[0], d_z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    {
-        std::vector<int32_t> id(n);
-        for (int i = 0; i < n; ++i) id[i] = i;
-        TCHK(hipMemcpyAsync(O[0], id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-        TCHK(hipStreamSynchronize(s));
-    }
-    Buf<Seg> dseg(8 * (size_t)n / std::max(maxparnode, 1) + 16);
-    Buf<int32_t> dsegout(2 * (8 * (size_t)n / std::max(maxparnode, 1) + 16));
-    Buf<int32_t> drng(2 * (8 * (size_t)n / std::max(maxparnode, 1) + 16));
-    Buf<double> dmm(6 * (8 * (size_t)n / std::max(maxparnode, 1) + 16));
+    k_iota<<<nb_of(n, 256), 256, 0, s>>>(n, O[0]);
+    TCHK(hipGetLastError());
+    const size_t mseg = 8 * (size_t)n / std::max(maxparnode, 1) + 16;   // segments / ranges per level
+    Buf<Seg> dseg(mseg);
+    Buf<int32_t> dsegout(2 * mseg);
+    Buf<int32_t> drng(2 * mseg);
+    Buf<double> dmm(6 * mseg);
+    std::unique_ptr<Buf<double>> dmmp;     // chunk partials, grown on demand
+    size_t dmmp_cap = 0;
+    // the per-level tables travel through pinned host memory (inputs in the
+    // first half, results in the second); each round trip ends with a polled
+    // event, after which the buffer may be rewritten
+    const size_t half = std::max(sizeof(Seg), 6 * sizeof(double)) * mseg;
+    struct Pinned {
+        char *p = nullptr;
+        ~Pinned() { if (p) (void)hipHostFree(p); }
+    } pin;
+    TCHK(hipHostMalloc(reinterpret_cast<void **>(&pin.p), 2 * half, hipHostMallocDefault));
+    char *const pin_in = pin.p, *const pin_out = pin.p + half;
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev() { if (e) (void)hipEventDestroy(e); }
+    } ev;
+    TCHK(hipEventCreateWithFlags(&ev.e, hipEventDisableTiming));
+    auto round_trip_end = [&]() {
+        TCHK(hipEventRecord(ev.e, s));
+        spin_wait(ev.e);
+    };
 
     auto minmax = [&](const std::vector<int32_t> &rng, std::vector<double> &out) {
         const int nr = (int)rng.size() / 2;
         out.assign(6 * nr, 0.0);
         if (!nr) return;
-        TCHK(hipMemcpyAsync(drng.p, rng.data(), sizeof(int32_t) * rng.size(), hipMemcpyHostToDevice, s));
-        k_seg_minmax<<<nr, 256, 0, s>>>(drng.p, X[cur], Y[cur], Z[cur], dmm.p);
+        int nch = 1;
+        for (int r = 0; r < nr; ++r)
+            nch = std::max(nch, std::max(rng[2 * r + 1] - rng[2 * r], 0) / kMinMaxChunk + 1);
+        const size_t need = (size_t)6 * nr * nch;
+        if (need > dmmp_cap) {   // every earlier use has completed (round trips end with a wait)
+            dmmp.reset();
+            dmmp.reset(new Buf<double>(need));
+            dmmp_cap = need;
+        }
+        std::memcpy(pin_in, rng.data(), sizeof(int32_t) * rng.size());
+        TCHK(hipMemcpyAsync(drng.p, pin_in, sizeof(int32_t) * rng.size(), hipMemcpyHostToDevice, s));
+        k_seg_minmax<<<dim3(nr, nch), 256, 0, s>>>(drng.p, X[cur], Y[cur], Z[cur], dmmp->p);
+        k_seg_minmax_fin<<<nb_of(6 * nr, 256), 256, 0, s>>>(nr, nch, drng.p, dmmp->p, dmm.p);
         TCHK(hipGetLastError());
-        TCHK(hipMemcpyAsync(out.data(), dmm.p, sizeof(double) * 6 * nr, hipMemcpyDeviceToHost, s));
-        TCHK(hipStreamSynchronize(s));
+        TCHK(hipMemcpyAsync(pin_out, dmm.p, sizeof(double) * 6 * nr, hipMemcpyDeviceToHost, s));
+        round_trip_end();
+        std::memcpy(out.data(), pin_out, sizeof(double) * 6 * nr);
     };
 
     // one partition pass over the given segments on coordinate d (0 x, 1 y, 2 z);
@@ -304,7 +374,8 @@ void build_tree_gpu(int32_t n, const double *d_x, const double *d_y, const doubl
         std::vector<Seg> sorted(segs.size());
         for (size_t k = 0; k < perm.size(); ++k) sorted[k] = segs[perm[k]];
         const int ns = (int)sorted.size();
-        TCHK(hipMemcpyAsync(dseg.p, sorted.data(), sizeof(Seg) * ns, hipMemcpyHostToDevice, s));
+        std::memcpy(pin_in, sorted.data(), sizeof(Seg) * ns);
+        TCHK(hipMemcpyAsync(dseg.p, pin_in, sizeof(Seg) * ns, hipMemcpyHostToDevice, s));
         const double *A = d == 0 ? X[cur] : d == 1 ? Y[cur] : Z[cur];
         k_part_flags<<<nb_of(n, 256), 256, 0, s>>>(n, A, dseg.p, ns, flag.p);
         k_scan_blocks<<<nscan, kScanThreads, 0, s>>>(n, flag.p, G.p, bsum.p);
@@ -318,9 +389,9 @@ void build_tree_gpu(int32_t n, const double *d_x, const double *d_y, const doubl
                                                   Z[nx], O[nx]);
         TCHK(hipGetLastError());
         cur = nx;
-        std::vector<int32_t> so(2 * ns);
-        TCHK(hipMemcpyAsync(so.data(), dsegout.p, sizeof(int32_t) * 2 * ns, hipMemcpyDeviceToHost, s));
-        TCHK(hipStreamSynchronize(s));
+        TCHK(hipMemcpyAsync(pin_out, dsegout.p, sizeof(int32_t) * 2 * ns, hipMemcpyDeviceToHost, s));
+        round_trip_end();
+        const int32_t *so = reinterpret_cast<const int32_t *>(pin_out);
         for (int k = 0; k < ns; ++k) mid[idx[perm[k]]] = so[2 * k + 1];
     };
 
