@@ -741,6 +741,32 @@ struct Kahan {
 
 // geometry part of the FP32 source records: coordinates relative to the
 // centre of the source's leaf box (so nearby pairs keep their digits)
+// sum of the set bits of n masks (an integer sum: exact in any order)
+__global__ void __launch_bounds__(256) k_popc_sum(int64_t n, const u64 *__restrict__ mask,
+                                                  unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long red[256];
+    unsigned long long c = 0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+        c += (unsigned long long)__popcll(mask[k]);
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+// fl[i] = the leaf node holding particle i: one workgroup per leaf
+__global__ void k_leaf_fill(const int32_t *__restrict__ leaves, const int32_t *__restrict__ nibeg,
+                            const int32_t *__restrict__ nnum, int32_t *__restrict__ fl)
+{
+    const int nd = leaves[blockIdx.x];
+    const int b = nibeg[nd], e = b + nnum[nd];
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) fl[i] = nd;
+}
+
 __global__ void k_src32_geom(int n, const int32_t *__restrict__ face_leafnode,
     const double4 *__restrict__ ngeo, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
@@ -2994,6 +3020,41 @@ void msolve_block_dev(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *
     HIPCHK(hipGetLastError());
 }
 
+// Pinned host staging of one host thread for the set-up's bulk transfers, one
+// buffer per direction, grown on demand and kept; callers finish every copy
+// that uses a buffer (stream synchronisation) before it is asked for again.
+struct PinnedBuf {
+    char *p = nullptr;
+    size_t cap = 0;
+    char *get(size_t bytes)
+    {
+        if (bytes > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&p), bytes, hipHostMallocDefault));
+            cap = bytes;
+        }
+        return p;
+    }
+};
+PinnedBuf &pinned_down() { thread_local PinnedBuf b; return b; }
+PinnedBuf &pinned_up() { thread_local PinnedBuf b; return b; }
+
+// Bump allocation in a pinned buffer, 256-byte aligned.
+struct PinnedArena {
+    char *base;
+    size_t used = 0;
+    template <class T>
+    T *take(size_t count)
+    {
+        T *r = reinterpret_cast<T *>(base + used);
+        used += (count * sizeof(T) + 255) / 256 * 256;
+        return r;
+    }
+};
+inline size_t arena_bytes(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
 void build_lists(tabi_ctx *c)
 {
 
@@ -3039,17 +3100,25 @@ void build_lists(tabi_ctx *c)
         c->need_mom.p, c->err_flag.p);
     HIPCHK(hipGetLastError());
 
-    // host: tasks, moment chunks, counters
-    std::vector<int32_t> near_leaf(nnr), need(nn), far_node(nf);
-    std::vector<u64> near_mask(nnr), far_mask(nf);
-    HIPCHK(hipMemcpyAsync(near_leaf.data(), c->near_leaf.p, sizeof(int32_t) * nnr, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(near_mask.data(), c->near_mask.p, sizeof(u64) * nnr, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(far_mask.data(), c->far_mask.p, sizeof(u64) * nf, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(need.data(), c->need_mom.p, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, c->s));
+    // host: tasks, moment chunks, counters (the far masks stay on the device:
+    // only their bit count is needed here)
+    DBuf<unsigned long long> dpf;
+    dpf.alloc(1);
+    HIPCHK(hipMemsetAsync(dpf.p, 0, sizeof(unsigned long long), c->s));
+    if (nf) k_popc_sum<<<std::min<unsigned>(nblk(nf, 256), 1024), 256, 0, c->s>>>(nf, c->far_mask.p, dpf.p);
+    HIPCHK(hipGetLastError());
+    PinnedArena down{pinned_down().get(arena_bytes(sizeof(int32_t) * nnr) + arena_bytes(sizeof(u64) * nnr) +
+                                       arena_bytes(sizeof(int32_t) * nn) + 256)};
+    int32_t *near_leaf = down.take<int32_t>(nnr);
+    u64 *near_mask = down.take<u64>(nnr);
+    int32_t *need = down.take<int32_t>(nn);
+    unsigned long long *pf = down.take<unsigned long long>(1);
+    HIPCHK(hipMemcpyAsync(near_leaf, c->near_leaf.p, sizeof(int32_t) * nnr, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(near_mask, c->near_mask.p, sizeof(u64) * nnr, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(need, c->need_mom.p, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(pf, dpf.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-
-    c->p_far = 0;
-    for (int64_t k = 0; k < nf; ++k) c->p_far += __builtin_popcountll(far_mask[k]);
+    c->p_far = (int64_t)*pf;
     // near field, compacted per leaf (the layout NearSlots describes): the
     // targets that list a leaf are cut into waves of 64, so no lane idles on a
     // target that does not need the leaf
@@ -3099,22 +3168,6 @@ void build_lists(tabi_ctx *c)
     }
     c->n_near_task = (int32_t)ntask;
     c->n_far_task = (int32_t)ft.size();
-    c->near_tasks.alloc(std::max<int64_t>(ntask, 1));
-    c->far_tasks.alloc(std::max<size_t>(ft.size(), 1));
-    c->near_ipos.upload(ipos.data(), ipos.size(), c->s);
-    c->node_toff.upload(toff.data(), nn, c->s);
-    c->node_nsplit.upload(nsplit.data(), nn, c->s);
-    if (ntask) {
-        c->near_tasks.upload(nt.data(), ntask, c->s);
-        c->near_tgt.alloc(ntask * kWave);
-        k_fill_i32<<<nblk(ntask * kWave, 256), 256, 0, c->s>>>(ntask * kWave, -1, c->near_tgt.p);
-        k_near_tgt<<<G, kWave, 0, c->s>>>(near_slots(c), c->near_tgt.p);
-        HIPCHK(hipGetLastError());
-    }
-    if (!ft.empty()) c->far_tasks.upload(ft.data(), ft.size(), c->s);
-    c->far_task_off.upload(fto.data(), G + 1, c->s);
-    c->part_near.alloc((size_t)std::max(c->n_near_task, 1) * kWave);
-    c->part_far.alloc((size_t)std::max(c->n_far_task, 1) * kWave);
 
     // moment nodes and their particle chunks
     std::vector<int32_t> n2m(nn, -1), cnode, cbeg, cend, mco(1, 0);
@@ -3133,27 +3186,59 @@ void build_lists(tabi_ctx *c)
     }
     c->nmom = nm;
     c->nchunk = (int32_t)cnode.size();
-    c->node2mom.upload(n2m.data(), nn, c->s);
+
+    // every table goes up through one pinned arena, asynchronously
+    const bool geom32 = c->prm.precision == 1;
+    const size_t up_bytes = arena_bytes(sizeof(int32_t) * ipos.size()) + 2 * arena_bytes(sizeof(int32_t) * nn) +
+                            arena_bytes(sizeof(NearTask) * nt.size()) + arena_bytes(sizeof(FarTask) * ft.size()) +
+                            arena_bytes(sizeof(int32_t) * fto.size()) + arena_bytes(sizeof(int32_t) * nn) +
+                            3 * arena_bytes(sizeof(int32_t) * cnode.size()) + arena_bytes(sizeof(int32_t) * mco.size()) +
+                            arena_bytes(sizeof(int32_t) * T.leaves.size()) + 256;
+    PinnedArena up{pinned_up().get(up_bytes)};
+    auto put = [&](auto &d, const auto &v) {
+        using E = typename std::decay_t<decltype(v)>::value_type;
+        E *stage = up.take<E>(v.size());
+        std::copy(v.begin(), v.end(), stage);
+        d.alloc(std::max<size_t>(v.size(), 1));
+        if (!v.empty())
+            HIPCHK(hipMemcpyAsync(d.p, stage, sizeof(E) * v.size(), hipMemcpyHostToDevice, c->s));
+    };
+    c->near_tasks.alloc(std::max<int64_t>(ntask, 1));
+    c->far_tasks.alloc(std::max<size_t>(ft.size(), 1));
+    put(c->near_ipos, ipos);
+    put(c->node_toff, toff);
+    put(c->node_nsplit, nsplit);
+    if (ntask) {
+        put(c->near_tasks, nt);
+        c->near_tgt.alloc(ntask * kWave);
+        k_fill_i32<<<nblk(ntask * kWave, 256), 256, 0, c->s>>>(ntask * kWave, -1, c->near_tgt.p);
+        k_near_tgt<<<G, kWave, 0, c->s>>>(near_slots(c), c->near_tgt.p);
+        HIPCHK(hipGetLastError());
+    }
+    if (!ft.empty()) put(c->far_tasks, ft);
+    put(c->far_task_off, fto);
+    c->part_near.alloc((size_t)std::max(c->n_near_task, 1) * kWave);
+    c->part_far.alloc((size_t)std::max(c->n_far_task, 1) * kWave);
+    put(c->node2mom, n2m);
     if (nm) {
-        c->chunk_node.upload(cnode.data(), cnode.size(), c->s);
-        c->chunk_beg.upload(cbeg.data(), cbeg.size(), c->s);
-        c->chunk_end.upload(cend.data(), cend.size(), c->s);
-        c->mom_chunk_off.upload(mco.data(), mco.size(), c->s);
+        put(c->chunk_node, cnode);
+        put(c->chunk_beg, cbeg);
+        put(c->chunk_end, cend);
+        put(c->mom_chunk_off, mco);
         c->mom_part.alloc((size_t)c->nchunk * 4 * c->nk);
         c->mom_w.alloc((size_t)nm * 8 * c->nj);
-        if (c->prm.precision == 1) c->mom_w32.alloc((size_t)nm * 8 * c->nj);
+        if (geom32) c->mom_w32.alloc((size_t)nm * 8 * c->nj);
     }
-    if (c->prm.precision == 1) {
-        std::vector<int32_t> fl(c->n);
-        for (int32_t nd : T.leaves)
-            for (int i = T.ibeg[nd]; i <= T.iend[nd]; ++i) fl[i] = nd;
-        DBuf<int32_t> dfl;
-        dfl.upload(fl.data(), c->n, c->s);
+    DBuf<int32_t> dleaves, dfl;
+    if (geom32) {
+        // leaf-local FP32 source records: each particle's leaf node, filled on the GPU
+        put(dleaves, T.leaves);
+        dfl.alloc(c->n);
+        k_leaf_fill<<<(unsigned)T.leaves.size(), 256, 0, c->s>>>(dleaves.p, c->nibeg.p, c->nnum.p, dfl.p);
         c->src32.alloc((size_t)c->n * 8);
         k_src32_geom<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, dfl.p, c->ngeo.p, c->tx.p, c->ty.p,
             c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->src32.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->s));
     }
     HIPCHK(hipStreamSynchronize(c->s));
 }
