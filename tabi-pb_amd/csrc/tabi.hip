@@ -1926,21 +1926,28 @@ __global__ void k_energy_reduce(int nq, int nchunk, const double *__restrict__ p
     ptl[k] = s;
 }
 
-// Coulomb energy sum_{i<j} q_i q_j / (eps_p r_ij) (src/main.f90:149-153): one
-// thread per charge i over j > i, block partial sums
+// Coulomb energy sum_{i<j} q_i q_j / (eps_p r_ij) (src/main.f90:149-153):
+// workgroup (bi, bj) takes the charges i of block bi against the charges j > i
+// of chunk bj, both 256 wide, the j read wave-uniformly; chunks wholly at or
+// below the block contribute nothing.  One partial per workgroup, at
+// part[bj * gridDim.x + bi], summed in that order by the caller.
 __global__ void __launch_bounds__(256) k_coulomb(int nq, const double *__restrict__ qpos,
     const double *__restrict__ q, double *__restrict__ part)
 {
     __shared__ double red[256];
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.y * 256, j1 = min(nq, j0 + 256);
     double s = 0.0;
-    if (i < nq) {
-        const double xi = qpos[3 * i], yi = qpos[3 * i + 1], zi = qpos[3 * i + 2];
-        for (int j = i + 1; j < nq; ++j) {
+    if (j1 - 1 > (int)blockIdx.x * 256) {
+        const bool vi = i < nq;
+        const double xi = vi ? qpos[3 * i] : 0.0, yi = vi ? qpos[3 * i + 1] : 0.0,
+                     zi = vi ? qpos[3 * i + 2] : 0.0;
+        for (int j = j0; j < j1; ++j) {
             const double dx = xi - qpos[3 * j], dy = yi - qpos[3 * j + 1], dz = zi - qpos[3 * j + 2];
-            s += q[j] * rsq_nr(dx * dx + dy * dy + dz * dz);
+            const double v = q[j] * rsq_nr(dx * dx + dy * dy + dz * dz);
+            s += (vi && j > i) ? v : 0.0;
         }
-        s *= q[i];
+        s *= vi ? q[i] : 0.0;
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -1948,7 +1955,7 @@ __global__ void __launch_bounds__(256) k_coulomb(int nq, const double *__restric
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -4379,19 +4386,19 @@ void energy_block_dev(tabi_ctx *c, int R, int nq, const double *qpos, const doub
     k_energyb<RC><<<dim3(ngq * nch, Rp / RC), 64, 0, c->s>>>(c->n, nq, nch, R, Rp, dqp.p, c->src8.p,
                                                             wb.p, c->prm.eps, c->prm.kappa, part.p);
     k_energy_reduceb<<<dim3(nblk(nq, 256), R), 256, 0, c->s>>>(nq, nch, R, part.p, dptl.p);
-    const int ncb = (nq + 255) / 256;
-    cpart.alloc((size_t)R * ncb);
+    const int ncb = (nq + 255) / 256, ncp = ncb * ncb;
+    cpart.alloc((size_t)R * ncp);
     for (int r = 0; r < R; ++r)
-        k_coulomb<<<ncb, 256, 0, c->s>>>(nq, dqp.p, dq.p + (size_t)r * nq, cpart.p + (size_t)r * ncb);
+        k_coulomb<<<dim3(ncb, ncb), 256, 0, c->s>>>(nq, dqp.p, dq.p + (size_t)r * nq, cpart.p + (size_t)r * ncp);
     HIPCHK(hipGetLastError());
-    std::vector<double> ptl((size_t)R * nq), cp((size_t)R * ncb);
+    std::vector<double> ptl((size_t)R * nq), cp((size_t)R * ncp);
     HIPCHK(hipMemcpyAsync(ptl.data(), dptl.p, sizeof(double) * ptl.size(), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipMemcpyAsync(cp.data(), cpart.p, sizeof(double) * cp.size(), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     for (int r = 0; r < R; ++r) {
         double s = 0.0, cc = 0.0;
         for (int k = 0; k < nq; ++k) s += q[(size_t)r * nq + k] * ptl[(size_t)r * nq + k];
-        for (int b = 0; b < ncb; ++b) cc += cp[(size_t)r * ncb + b];
+        for (int b = 0; b < ncp; ++b) cc += cp[(size_t)r * ncp + b];
         sol[r] = s * 0.5 * para * 4.0 * kPi;
         cou[r] = cc / eps_p * para;
     }
@@ -4437,15 +4444,15 @@ void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double
     k_energy<<<ngq * nch, 64, 0, c->s>>>(c->n, nq, nch, dqp.p, c->src8.p, c->prm.eps,
                                          c->prm.kappa, part.p);
     k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, nch, part.p, dptl.p);
-    const int ncb = (nq + 255) / 256;
-    cpart.alloc(ncb + 1);
-    k_coulomb<<<ncb, 256, 0, c->s>>>(nq, dqp.p, dq.p, cpart.p);
-    k_sum_part<<<1, 256, 0, c->s>>>(ncb, cpart.p, cpart.p + ncb);
+    const int ncb = (nq + 255) / 256, ncp = ncb * ncb;
+    cpart.alloc(ncp + 1);
+    k_coulomb<<<dim3(ncb, ncb), 256, 0, c->s>>>(nq, dqp.p, dq.p, cpart.p);
+    k_sum_part<<<1, 256, 0, c->s>>>(ncp, cpart.p, cpart.p + ncp);
     HIPCHK(hipGetLastError());
     // sum_k q_k * potential_k (src/main.f90:148,156)
     const double s = dev_dot(c, nq, dq.p, dptl.p);
     double cc = 0.0;
-    HIPCHK(hipMemcpyAsync(&cc, cpart.p + ncb, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(&cc, cpart.p + ncp, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     *sol = s * 0.5 * para * 4.0 * kPi;
     *cou = cc / eps_p * para;
