@@ -1624,7 +1624,7 @@ __global__ void __launch_bounds__(256, 2) k_lu_strip_trsm(int K0, const int2 *__
 // U12[k = lane/16][col = lane%16], result register v = D[lane/16 + 4v][lane%16].
 
 template <int KD>
-__global__ void __launch_bounds__(256) k_lu_gemm(int kb, const int4 *__restrict__ tiles,
+__global__ void __launch_bounds__(256, KD == kLuNB ? 1 : 3) k_lu_gemm(int kb, const int4 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     double *__restrict__ lu)
 {
@@ -1639,20 +1639,26 @@ __global__ void __launch_bounds__(256) k_lu_gemm(int kb, const int4 *__restrict_
     double *A = lu + blk_off[l];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int rw = (w & 1) * 32, cw = (w >> 1) * 32, li = lane & 15, lk = lane >> 4;
-    // the trailing entries are loaded before the product so their latency
-    // overlaps the arithmetic
+    // after a panel (one chunk) the trailing entries are loaded before the
+    // product so their latency overlaps the arithmetic; below a strip (four
+    // chunks) they are loaded after it, which frees their registers for a
+    // third wave per SIMD
+    constexpr bool kEarlyC = KD == kLuNB;
     double cv[2][2][4];
+    auto load_c = [&]() {
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const int col = c0 + cw + 16 * bj + li;
+            for (int bj = 0; bj < 2; ++bj) {
+                const int col = c0 + cw + 16 * bj + li;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = r0 + rw + 16 * bi + lk + 4 * v;
-                cv[bi][bj][v] = (row < m && col < cend) ? A[(size_t)col * m + row] : 0.0;
+                for (int v = 0; v < 4; ++v) {
+                    const int row = r0 + rw + 16 * bi + lk + 4 * v;
+                    cv[bi][bj][v] = (row < m && col < cend) ? A[(size_t)col * m + row] : 0.0;
+                }
             }
-        }
+    };
+    if (kEarlyC) load_c();
     f64x4 acc[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -1697,6 +1703,7 @@ __global__ void __launch_bounds__(256) k_lu_gemm(int kb, const int4 *__restrict_
             acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
     }
+    if (!kEarlyC) load_c();
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
