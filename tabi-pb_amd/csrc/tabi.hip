@@ -3848,6 +3848,9 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
     // initial residual
     rs.msolve(all, dact_a.p, kSlotB, kSlotR);
     dot_fetch(all, dact_a.p, kSlotR, kSlotR, sT);
+    // M^-1 b is also M^-1 r0 for a run that starts from zero (r0 = b): kept in
+    // DL, which nothing writes before the first cycle's residual
+    k_copyt<<<gy(R), 256, 0, st>>>(T, R, kSlotR, kSlotDL, NV, dact_a.p);
     std::vector<double> bnrm(R), r0nrm(R), err(R, 0.0), prod(R), rho(R), snormw(R), vnrm(R);
     std::vector<char> guessed(R, 0);        // run started from a nonzero x0
     std::vector<int> iter(R, 0), ierr(R, 0), lgmr(R), iflag(R);
@@ -3893,8 +3896,19 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
         if (nrsts > 0) k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotDL, kSlotR, NV, dact_a.p);
         k_fillt<<<gy(na), 256, 0, st>>>(T, R, kSlotZ, NV, 0.0, dact_a.p);
         if (nrsts == 0) {
-            k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotR, kSlotWK, NV, dact_a.p);
-            rs.msolve(active, dact_a.p, kSlotWK, kSlotR);
+            // M^-1 r0: the kept M^-1 b for runs from zero (a deterministic
+            // solve of the same vector), a solve for runs from a guess
+            std::vector<int32_t> from0, fromg;
+            for (int r : active) (guessed[r] ? fromg : from0).push_back(r);
+            if (!from0.empty()) {
+                put(dact_b, cur_b, from0);
+                k_copyt<<<gy(from0.size()), 256, 0, st>>>(T, R, kSlotDL, kSlotR, NV, dact_b.p);
+            }
+            if (!fromg.empty()) {
+                put(dact_b, cur_b, fromg);
+                k_copyt<<<gy(fromg.size()), 256, 0, st>>>(T, R, kSlotR, kSlotWK, NV, dact_b.p);
+                rs.msolve(fromg, dact_b.p, kSlotWK, kSlotR);
+            }
         }
         k_copyt<<<gy(na), 256, 0, st>>>(T, R, kSlotR, V(0), NV, dact_a.p);
         dot_fetch(active, dact_a.p, V(0), V(0), sT);
@@ -3907,7 +3921,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
             if (!fb.empty()) {
                 put(dact_b, cur_b, fb);
                 k_fillt<<<gy(fb.size()), 256, 0, st>>>(T, R, kSlotX, NV, 0.0, dact_b.p);
-                rs.msolve(fb, dact_b.p, kSlotB, kSlotR);
+                k_copyt<<<gy(fb.size()), 256, 0, st>>>(T, R, kSlotDL, kSlotR, NV, dact_b.p);  // M^-1 b
                 k_copyt<<<gy(fb.size()), 256, 0, st>>>(T, R, kSlotR, V(0), NV, dact_b.p);
                 for (int r : fb) { r0nrm[r] = bnrm[r]; guessed[r] = 0; }
             }
