@@ -434,9 +434,15 @@ __device__ __forceinline__ void mom_raw_body(int c, int n, const int32_t *__rest
         for (int q = part_id; q < cnt; q += NPART) {
             double pxv = 1.0, pyv = 1.0, pzv = 1.0;
             const double dx = sd[0][q], dy = sd[1][q], dz = sd[2][q];
-            for (int r = 0; r < e1; ++r) pxv *= dx;
-            for (int r = 0; r < e2; ++r) pyv *= dy;
-            for (int r = 0; r < e3; ++r) pzv *= dz;
+            // powers by a fixed number of multiplications (by 1.0 past the
+            // exponent: exact), so the lanes with different exponents do not
+            // diverge
+#pragma unroll
+            for (int r = 0; r < P; ++r) {
+                pxv *= r < e1 ? dx : 1.0;
+                pyv *= r < e2 ? dy : 1.0;
+                pzv *= r < e3 ? dz : 1.0;
+            }
             const double mono = pxv * pyv * pzv;
             m0 += sd[3][q] * mono;
             m1 += sd[4][q] * mono;
