@@ -991,22 +991,40 @@ __global__ void k_reduce(int n, NearSlots ns, const int32_t *__restrict__ far_ta
     const double2 *__restrict__ part_near, const double2 *__restrict__ part_far,
     const double *__restrict__ x, double pre1, double pre2, double *__restrict__ y)
 {
+    // the group's item records are loaded by the wave's lanes together (one
+    // round of dependent loads instead of one per item), then walked in order
+    __shared__ u64 s_mask[4][kWave];
+    __shared__ int s_ipos[4][kWave], s_toff[4][kWave], s_nsp[4][kWave];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int g = i / kWave, lane = i % kWave;
+    const int g = i / kWave, lane = i % kWave, w = threadIdx.x / kWave;
+    if (g * kWave >= n) return;                  // the whole wave is past the end
     const u64 below = (1ull << lane) - 1;
     double p1 = 0.0, p2 = 0.0;
-    for (int k = ns.off[g]; k < ns.off[g + 1]; ++k) {
-        const u64 m = ns.mask[k];
-        if (!((m >> lane) & 1)) continue;
-        const int p = ns.ipos[k] + __popcll(m & below);
-        const int nd = ns.leaf[k], nsp = ns.nsplit[nd];
-        const double2 *pt = part_near + ((size_t)ns.toff[nd] + (size_t)(p / kWave) * nsp) * kWave + p % kWave;
-        for (int s = 0; s < nsp; ++s) {
-            const double2 v = pt[(size_t)s * kWave];
-            p1 += v.x; p2 += v.y;
+    const int k0 = ns.off[g], k1 = ns.off[g + 1];
+    for (int kb = k0; kb < k1; kb += kWave) {
+        const int cnt = min(kWave, k1 - kb);
+        if (lane < cnt) {
+            const int k = kb + lane, nd = ns.leaf[k];
+            s_mask[w][lane] = ns.mask[k];
+            s_ipos[w][lane] = ns.ipos[k];
+            s_toff[w][lane] = ns.toff[nd];
+            s_nsp[w][lane] = ns.nsplit[nd];
         }
+        __builtin_amdgcn_wave_barrier();
+        for (int kk = 0; kk < cnt; ++kk) {
+            const u64 m = s_mask[w][kk];
+            if (!((m >> lane) & 1)) continue;
+            const int p = s_ipos[w][kk] + __popcll(m & below);
+            const int nsp = s_nsp[w][kk];
+            const double2 *pt = part_near + ((size_t)s_toff[w][kk] + (size_t)(p / kWave) * nsp) * kWave + p % kWave;
+            for (int s = 0; s < nsp; ++s) {
+                const double2 v = pt[(size_t)s * kWave];
+                p1 += v.x; p2 += v.y;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();         // every lane has read the records
     }
+    if (i >= n) return;
     for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
         const double2 v = part_far[(size_t)t * kWave + lane];
         p1 += v.x; p2 += v.y;
