@@ -2321,23 +2321,40 @@ __global__ void k_reduceb(int n, int R, NearSlots ns, const int32_t *__restrict_
     const double *__restrict__ x, int64_t xs, double *__restrict__ y, int64_t ys,
     const int32_t *__restrict__ act, double pre1, double pre2)
 {
+    // item records loaded by the wave's lanes together, as in k_reduce
+    __shared__ u64 s_mask[4][kWave];
+    __shared__ int s_ipos[4][kWave], s_toff[4][kWave], s_nsp[4][kWave];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
-    if (i >= n) return;
-    const int g = i / kWave, lane = i % kWave;
+    const int g = i / kWave, lane = i % kWave, w = threadIdx.x / kWave;
+    if (g * kWave >= n) return;                  // the whole wave is past the end
     const u64 below = (1ull << lane) - 1;
     double p1 = 0.0, p2 = 0.0;
-    for (int k = ns.off[g]; k < ns.off[g + 1]; ++k) {
-        const u64 m = ns.mask[k];
-        if (!((m >> lane) & 1)) continue;
-        const int p = ns.ipos[k] + __popcll(m & below);
-        const int nd = ns.leaf[k], nsp = ns.nsplit[nd];
-        const size_t t0 = (size_t)ns.toff[nd] + (size_t)(p / kWave) * nsp;
-        for (int s = 0; s < nsp; ++s) {
-            const double2 v = part_near[((t0 + s) * R + r) * kWave + p % kWave];
-            p1 += v.x; p2 += v.y;
+    const int k0 = ns.off[g], k1 = ns.off[g + 1];
+    for (int kb = k0; kb < k1; kb += kWave) {
+        const int cnt = min(kWave, k1 - kb);
+        if (lane < cnt) {
+            const int k = kb + lane, nd = ns.leaf[k];
+            s_mask[w][lane] = ns.mask[k];
+            s_ipos[w][lane] = ns.ipos[k];
+            s_toff[w][lane] = ns.toff[nd];
+            s_nsp[w][lane] = ns.nsplit[nd];
         }
+        __builtin_amdgcn_wave_barrier();
+        for (int kk = 0; kk < cnt; ++kk) {
+            const u64 m = s_mask[w][kk];
+            if (!((m >> lane) & 1)) continue;
+            const int p = s_ipos[w][kk] + __popcll(m & below);
+            const int nsp = s_nsp[w][kk];
+            const size_t t0 = (size_t)s_toff[w][kk] + (size_t)(p / kWave) * nsp;
+            for (int s = 0; s < nsp; ++s) {
+                const double2 v = part_near[((t0 + s) * R + r) * kWave + p % kWave];
+                p1 += v.x; p2 += v.y;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
+    if (i >= n) return;
     for (int t = far_task_off[g]; t < far_task_off[g + 1]; ++t) {
         const double2 v = part_far[((size_t)t * R + r) * kWave + lane];
         p1 += v.x; p2 += v.y;
