@@ -501,7 +501,8 @@ __global__ void __launch_bounds__(256) k_mom_rawb(int n, const int32_t *__restri
 // where a0/ak are the Taylor coefficients for kappa = 0 and kappa.
 template <int P>
 __device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict__ mom_chunk_off,
-    const double *__restrict__ part, double eps, double *__restrict__ mom_w)
+    const double *__restrict__ part, double eps, double *__restrict__ mom_w,
+    float *__restrict__ mom_w32 = nullptr)
 {
     constexpr int NK = MomShape<P>::NK;
     constexpr int NJ = MomShape<P>::NJ;
@@ -548,20 +549,28 @@ __device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict
                 he[e] = s;
             }
         }
-        W[0 * NJ + j] = ha - hb;
-        W[1 * NJ + j] = -ha + eps * hb;
+        double wv[8];
+        wv[0] = ha - hb;
+        wv[1] = -ha + eps * hb;
         for (int d = 0; d < 3; ++d) {
-            W[(2 + d) * NJ + j] = -hc[d] - he[d];
-            W[(5 + d) * NJ + j] = hc[d] / eps + he[d];
+            wv[2 + d] = -hc[d] - he[d];
+            wv[5 + d] = hc[d] / eps + he[d];
+        }
+        for (int v = 0; v < 8; ++v) W[v * NJ + j] = wv[v];
+        if (mom_w32) {
+            // the FP32 far field's copy, rounded as a separate conversion would
+            float *W32 = mom_w32 + (size_t)mi * 8 * NJ;
+            for (int v = 0; v < 8; ++v) W32[v * NJ + j] = (float)wv[v];
         }
     }
 }
 
 template <int P>
 __global__ void __launch_bounds__(256) k_mom_final(const int32_t *__restrict__ mom_chunk_off,
-    const double *__restrict__ part, double eps, double *__restrict__ mom_w)
+    const double *__restrict__ part, double eps, double *__restrict__ mom_w,
+    float *__restrict__ mom_w32)
 {
-    mom_final_body<P>(blockIdx.x, mom_chunk_off, part, eps, mom_w);
+    mom_final_body<P>(blockIdx.x, mom_chunk_off, part, eps, mom_w, mom_w32);
 }
 
 // block column blockIdx.y: partials at part + col*pstride, vectors at
@@ -2848,7 +2857,7 @@ void launch_moments(tabi_ctx *c, const double *x)
         c->chunk_end.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
         c->tarea.p, x, c->mom_part.p);
     k_mom_final<P><<<c->nmom, 256, 0, c->s>>>(c->mom_chunk_off.p, c->mom_part.p, c->prm.eps,
-                                              c->mom_w.p);
+        c->mom_w.p, c->prm.precision == 1 ? c->mom_w32.p : nullptr);
 }
 
 template <int P>
@@ -2856,8 +2865,7 @@ void launch_far(tabi_ctx *c)
 {
     if (c->n_far_task == 0) return;
     if (c->prm.precision == 1) {
-        const int64_t nw = (int64_t)c->nmom * 8 * c->nj;
-        k_to_float<<<nblk(nw, 256), 256, 0, c->s>>>(nw, c->mom_w.p, c->mom_w32.p);
+        // mom_w32 was written beside mom_w by k_mom_final
         k_far32<P><<<c->n_far_task, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n,
             c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_w32.p, c->ngeo.p, c->tx.p,
             c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, (float)c->prm.kappa, c->part_far.p);
