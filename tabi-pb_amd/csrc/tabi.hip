@@ -1443,10 +1443,19 @@ __global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
     }
     __syncthreads();
     if (t < kLuNB) {
-        for (int k = 0; k < nbe - 1; ++k) {
-            const double xk = Xs[k][t];
-            for (int r = k + 1; r < nbe; ++r) Xs[r][t] = fma(-Ls[r][k], xk, Xs[r][t]);
+        // the column in registers; L's zero padding (rows past nb) leaves the
+        // identity entries unchanged, so the full 32-row sweep computes what
+        // the nb-row one does, with every product independent of LDS writes
+        double x[kLuNB];
+#pragma unroll
+        for (int r = 0; r < kLuNB; ++r) x[r] = Xs[r][t];
+#pragma unroll
+        for (int k = 0; k < kLuNB - 1; ++k) {
+#pragma unroll
+            for (int r = k + 1; r < kLuNB; ++r) x[r] = fma(-Ls[r][k], x[k], x[r]);
         }
+#pragma unroll
+        for (int r = 0; r < kLuNB; ++r) Xs[r][t] = x[r];
     }
     __syncthreads();
     double *Li = linv + ((size_t)l * (kLuW / kLuNB) + (k0 % kLuW) / kLuNB) * kLuNB * kLuNB;
