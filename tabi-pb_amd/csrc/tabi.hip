@@ -1762,17 +1762,24 @@ __global__ void __launch_bounds__(256, KD == kLuNB ? 1 : 3) k_lu_gemm(int kb, co
 
 // Composite row permutation of each leaf's pivot sequence: (P r)[k] = r[perm[k]]
 // (the interchanges of DGETRS/DLASWP applied in order, src/dgetrf_dep.f:346-448).
-// One thread per leaf; runs once per surface.
-__global__ void k_lu_perm(int nleaf, const int32_t *__restrict__ blk_m,
+// One wave per leaf: the identity is written by all lanes, and only a block
+// with an interchange goes through the sequential pass (lane 0), since the
+// interchanges compose in order.
+__global__ void __launch_bounds__(kWave) k_lu_perm(const int32_t *__restrict__ blk_m,
                           const int32_t *__restrict__ piv_off, const int32_t *__restrict__ piv,
                           int32_t *__restrict__ perm)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nleaf) return;
+    const int l = blockIdx.x, lane = threadIdx.x;
     const int m = blk_m[l];
     const int32_t *pv = piv + piv_off[l];
     int32_t *pm = perm + piv_off[l];
-    for (int k = 0; k < m; ++k) pm[k] = k;
+    int swaps = 0;
+    for (int k = lane; k < m; k += kWave) {
+        pm[k] = k;
+        swaps |= pv[k] != k;
+    }
+    if (!__syncthreads_or(swaps)) return;        // the block is one wave
+    if (lane != 0) return;
     for (int k = 0; k < m; ++k) {
         const int p = pv[k];
         if (p != k) { const int32_t t = pm[k]; pm[k] = pm[p]; pm[p] = t; }
@@ -3588,7 +3595,7 @@ void build_precond(tabi_ctx *c)
     }
     HIPCHK(hipGetLastError());
     c->perm.alloc(pof);
-    k_lu_perm<<<nblk(nl, 64), 64, 0, c->s>>>(nl, c->blk_m.p, c->piv_off.p, c->piv.p, c->perm.p);
+    k_lu_perm<<<nl, kWave, 0, c->s>>>(c->blk_m.p, c->piv_off.p, c->piv.p, c->perm.p);
     HIPCHK(hipGetLastError());
     std::vector<int32_t> info(nl);
     HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
