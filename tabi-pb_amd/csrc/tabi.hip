@@ -1469,9 +1469,11 @@ __global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
 // Row interchanges of panel k0 on every column outside the strip's factored
 // part, and U12 = L11^-1 A12 for the strip's columns right of the panel (their
 // rows are up to date: inside the strip the factorisation is right-looking).
-// A workgroup takes 64 columns of one active block; the touched rows (the
-// panel rows and the pivot rows below them) are staged in LDS with coalesced
-// loads.  Columns left of the strip and beyond it take the interchanges only.
+// A workgroup takes one of the block's 64-column tiles that hold strip
+// columns right of the panel (k_lu_swap_rest takes the other tiles); the
+// touched rows (the panel rows and the pivot rows below them) are staged in
+// LDS with coalesced loads.  The tile's columns outside the strip take the
+// interchanges only.
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
     double *__restrict__ lu, const int32_t *__restrict__ swrec, const double *__restrict__ linv)
@@ -1555,6 +1557,51 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
     for (int i = 0; i < kTs; ++i) {
         const int e = t + 256 * i, q = e % 64, j = j0 + e / 64;
         if (q < nr && col_ok(j)) A[(size_t)j * m + rows[q]] = T[q][e / 64];
+    }
+}
+
+// Row interchanges of panel k0 on the column tiles that k_lu_swap_trsm does not
+// take (those left of the panel's 64-column tile and those wholly beyond the
+// strip): one workgroup per active block, which exits at once when the panel
+// made no interchange.  Each tile's touched rows are read into registers,
+// then, after a barrier, written to their destinations.
+__global__ void __launch_bounds__(256) k_lu_swap_rest(int k0, const int32_t *__restrict__ lorder,
+    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
+    double *__restrict__ lu, const int32_t *__restrict__ swrec)
+{
+    __shared__ int rows[2 * kLuNB], src[2 * kLuNB];
+    const int l = lorder[blockIdx.x], t = threadIdx.x;
+    const int32_t *rec = swrec + (size_t)l * kSwRec;
+    if (rec[4 * kLuNB + 1]) return;              // no interchange in this panel
+    const int m = blk_m[l];
+    const int nbe = min(kLuNB, m - k0);
+    const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
+    const int nr = rec[4 * kLuNB];
+    if (t < 2 * kLuNB) { rows[t] = rec[t]; src[t] = rec[2 * kLuNB + t]; }
+    __syncthreads();
+    double *A = lu + blk_off[l];
+    // the tiles [r0, r1] belong to k_lu_swap_trsm (none when send <= k0 + kLuNB)
+    const bool has_right = send > k0 + kLuNB;
+    const int r0 = (k0 + kLuNB) / 64, r1 = (send - 1) / 64;
+    auto col_ok = [&](int j) { return j < m && (j < K0 || j >= k0 + nbe); };
+    constexpr int kTs = 2 * kLuNB * 64 / 256;
+    const int nct = (m + 63) / 64;
+    for (int ct = 0; ct < nct; ++ct) {
+        if (has_right && ct >= r0 && ct <= r1) continue;   // uniform
+        const int j0 = ct * 64;
+        double v[kTs];
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + 256 * i, q = e % 64, j = min(j0 + e / 64, m - 1);
+            v[i] = A[(size_t)j * m + src[min(q, nr - 1)]];
+        }
+        __syncthreads();                         // every touched row is read
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + 256 * i, q = e % 64, j = j0 + e / 64;
+            if (q < nr && col_ok(j)) A[(size_t)j * m + rows[q]] = v[i];
+        }
+        __syncthreads();
     }
 }
 
@@ -3332,7 +3379,7 @@ constexpr int kXcd = 8;
 
 // The tile lists of the batched LU, one entry per workgroup of a launch.
 enum LuTileKind : int {
-    kTilesSwap = 0,    // {block, column tile}: interchanges / in-strip solve
+    kTilesSwap = 0,    // {block, column tile}: in-strip solve right of the panel
     kTilesPanel = 1,   // {block, tile row, tile col, strip end}: update after a panel
     kTilesStrip = 2,   // {block, column tile}: solve of the columns beyond a strip
     kTilesBelow = 3,   // {block, tile row, tile col, m}: update below a strip
@@ -3347,7 +3394,8 @@ __host__ __device__ inline void lu_tile_grid(int kind, int k0, int mq, int &ny, 
     const bool strip_end = k0 + kLuNB == K0 + kLuW;
     ny = 1; nx = 0;
     if (kind == kTilesSwap) {
-        nx = (mq + 63) / 64;
+        // the column tiles holding strip columns right of the panel
+        if (send > k0 + kLuNB) nx = (send - 1) / 64 - (k0 + kLuNB) / 64 + 1;
     } else if (kind == kTilesPanel) {
         const int rem = mq - k0 - kLuNB, cols = send - k0 - kLuNB;
         if (rem > 0 && cols > 0) { ny = (rem + 63) / 64; nx = (cols + 63) / 64; }
@@ -3393,6 +3441,7 @@ __global__ void k_lu_tiles(int kind, int nsteps, const int64_t *__restrict__ off
         const int K0 = k0 - k0 % kLuW;
         if (kind == kTilesPanel) v = make_int4(l, idx / nx, idx % nx, min(mq, K0 + kLuW));
         else if (kind == kTilesBelow) v = make_int4(l, idx / nx, idx % nx, mq);
+        else if (kind == kTilesSwap) v = make_int4(l, (k0 + kLuNB) / 64 + idx, 0, 0);
         else v = make_int4(l, idx, 0, 0);
     }
     if (kind == kTilesPanel || kind == kTilesBelow) {
@@ -3578,8 +3627,11 @@ void build_precond(tabi_ctx *c)
         const int64_t nsw = sw_off[step + 1] - sw_off[step];
         k_lu_linv<<<na, 64, 0, c->s>>>(k0, 0, dlorder_p, c->blk_m.p, c->blk_off.p, c->lu.p,
                                        c->lu_linv.p);
-        k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
-            c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p);
+        k_lu_swap_rest<<<(unsigned)na, 256, 0, c->s>>>(k0, dlorder_p, c->blk_m.p, c->blk_off.p,
+                                                      c->lu.p, c->lu_swrec.p);
+        if (nsw > 0)
+            k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
+                c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
         if (nt > 0)
             k_lu_gemm<kLuNB><<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
