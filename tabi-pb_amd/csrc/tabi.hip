@@ -1298,6 +1298,42 @@ __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nb
 // each column step is a block-wide arg-max, a row exchange through LDS and a
 // broadcast of the pivot row.  Afterwards the panel's interchanges are applied
 // to the strip's earlier columns K0..k0-1 (staged in LDS).
+// Row interchanges of panel k0 on the block's column tiles that
+// k_lu_swap_trsm does not take (those left of the panel's 64-column tile and
+// those wholly beyond the strip), by the panel's workgroup when the panel made
+// interchanges: each tile's touched rows are read into registers, then, after
+// a barrier, written to their destinations.  Out of line: only panels that
+// pivot reach it.
+template <int kPanelThreads>
+__device__ __attribute__((noinline)) void lu_swap_rest(int k0, int m, int nbe, double *A, int nr,
+                                                       const int *rows, const int *src)
+{
+    const int t = threadIdx.x;
+    const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
+    // the tiles [r0, r1] belong to k_lu_swap_trsm (none when send <= k0 + kLuNB)
+    const bool has_right = send > k0 + kLuNB;
+    const int r0 = (k0 + kLuNB) / 64, r1 = (send - 1) / 64;
+    constexpr int kTs = 64 * 64 / kPanelThreads;
+    const int nct = (m + 63) / 64;
+    for (int ct = 0; ct < nct; ++ct) {
+        if (has_right && ct >= r0 && ct <= r1) continue;   // uniform
+        const int j0 = ct * 64;
+        double v[kTs];
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + kPanelThreads * i, q = e % 64, j = min(j0 + e / 64, m - 1);
+            v[i] = A[(size_t)j * m + src[min(q, nr - 1)]];
+        }
+        __syncthreads();                         // every touched row is read
+#pragma unroll
+        for (int i = 0; i < kTs; ++i) {
+            const int e = t + kPanelThreads * i, q = e % 64, j = j0 + e / 64;
+            if (q < nr && j < m && (j < K0 || j >= k0 + nbe)) A[(size_t)j * m + rows[q]] = v[i];
+        }
+        __syncthreads();
+    }
+}
+
 template <int RPT, int kPanelThreads>
 __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
@@ -1392,23 +1428,29 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     // the panel's interchanges on the strip's earlier columns K0..k0-1: every
     // touched row is read, then written to its destination
     const int K0 = k0 - k0 % kLuW, ncl = k0 - K0;
-    if (ncl == 0 || fast) return;
+    if (fast) return;
     __syncthreads();
     const int nr = s_prow;
-    // element e = t + kPanelThreads i: row slot e % 64, column e / 64
-    constexpr int kEs = (2 * kLuNB * (kLuW - kLuNB) + kPanelThreads - 1) / kPanelThreads;
-    double v[kEs];
+    if (ncl > 0) {
+        // element e = t + kPanelThreads i: row slot e % 64, column e / 64
+        constexpr int kEs = (2 * kLuNB * (kLuW - kLuNB) + kPanelThreads - 1) / kPanelThreads;
+        double v[kEs];
 #pragma unroll
-    for (int i = 0; i < kEs; ++i) {
-        const int e = t + kPanelThreads * i, q = e % 64, jj = min(e / 64, ncl - 1);
-        v[i] = A[(size_t)(K0 + jj) * m + s_src[min(q, nr - 1)]];
-    }
-    __syncthreads();
+        for (int i = 0; i < kEs; ++i) {
+            const int e = t + kPanelThreads * i, q = e % 64, jj = min(e / 64, ncl - 1);
+            v[i] = A[(size_t)(K0 + jj) * m + s_src[min(q, nr - 1)]];
+        }
+        __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kEs; ++i) {
-        const int e = t + kPanelThreads * i, q = e % 64, jj = e / 64;
-        if (q < nr && jj < ncl) A[(size_t)(K0 + jj) * m + s_dst[q]] = v[i];
+        for (int i = 0; i < kEs; ++i) {
+            const int e = t + kPanelThreads * i, q = e % 64, jj = e / 64;
+            if (q < nr && jj < ncl) A[(size_t)(K0 + jj) * m + s_dst[q]] = v[i];
+        }
+        __syncthreads();
     }
+    // and on the block's column tiles outside the strip that k_lu_swap_trsm
+    // does not take
+    lu_swap_rest<kPanelThreads>(k0, m, nbe, A, nr, s_dst, s_src);
 }
 
 // Inverse of the unit lower triangle L11 of panel k0 (rows and columns
@@ -1470,7 +1512,7 @@ __global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
 // part, and U12 = L11^-1 A12 for the strip's columns right of the panel (their
 // rows are up to date: inside the strip the factorisation is right-looking).
 // A workgroup takes one of the block's 64-column tiles that hold strip
-// columns right of the panel (k_lu_swap_rest takes the other tiles); the
+// columns right of the panel (the panel kernel takes the other tiles); the
 // touched rows (the panel rows and the pivot rows below them) are staged in
 // LDS with coalesced loads.  The tile's columns outside the strip take the
 // interchanges only.
@@ -1560,50 +1602,6 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
     }
 }
 
-// Row interchanges of panel k0 on the column tiles that k_lu_swap_trsm does not
-// take (those left of the panel's 64-column tile and those wholly beyond the
-// strip): one workgroup per active block, which exits at once when the panel
-// made no interchange.  Each tile's touched rows are read into registers,
-// then, after a barrier, written to their destinations.
-__global__ void __launch_bounds__(256) k_lu_swap_rest(int k0, const int32_t *__restrict__ lorder,
-    const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    double *__restrict__ lu, const int32_t *__restrict__ swrec)
-{
-    __shared__ int rows[2 * kLuNB], src[2 * kLuNB];
-    const int l = lorder[blockIdx.x], t = threadIdx.x;
-    const int32_t *rec = swrec + (size_t)l * kSwRec;
-    if (rec[4 * kLuNB + 1]) return;              // no interchange in this panel
-    const int m = blk_m[l];
-    const int nbe = min(kLuNB, m - k0);
-    const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
-    const int nr = rec[4 * kLuNB];
-    if (t < 2 * kLuNB) { rows[t] = rec[t]; src[t] = rec[2 * kLuNB + t]; }
-    __syncthreads();
-    double *A = lu + blk_off[l];
-    // the tiles [r0, r1] belong to k_lu_swap_trsm (none when send <= k0 + kLuNB)
-    const bool has_right = send > k0 + kLuNB;
-    const int r0 = (k0 + kLuNB) / 64, r1 = (send - 1) / 64;
-    auto col_ok = [&](int j) { return j < m && (j < K0 || j >= k0 + nbe); };
-    constexpr int kTs = 2 * kLuNB * 64 / 256;
-    const int nct = (m + 63) / 64;
-    for (int ct = 0; ct < nct; ++ct) {
-        if (has_right && ct >= r0 && ct <= r1) continue;   // uniform
-        const int j0 = ct * 64;
-        double v[kTs];
-#pragma unroll
-        for (int i = 0; i < kTs; ++i) {
-            const int e = t + 256 * i, q = e % 64, j = min(j0 + e / 64, m - 1);
-            v[i] = A[(size_t)j * m + src[min(q, nr - 1)]];
-        }
-        __syncthreads();                         // every touched row is read
-#pragma unroll
-        for (int i = 0; i < kTs; ++i) {
-            const int e = t + 256 * i, q = e % 64, j = j0 + e / 64;
-            if (q < nr && col_ok(j)) A[(size_t)j * m + rows[q]] = v[i];
-        }
-        __syncthreads();
-    }
-}
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -3627,8 +3625,6 @@ void build_precond(tabi_ctx *c)
         const int64_t nsw = sw_off[step + 1] - sw_off[step];
         k_lu_linv<<<na, 64, 0, c->s>>>(k0, 0, dlorder_p, c->blk_m.p, c->blk_off.p, c->lu.p,
                                        c->lu_linv.p);
-        k_lu_swap_rest<<<(unsigned)na, 256, 0, c->s>>>(k0, dlorder_p, c->blk_m.p, c->blk_off.p,
-                                                      c->lu.p, c->lu_swrec.p);
         if (nsw > 0)
             k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
                 c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p);
