@@ -3617,14 +3617,16 @@ void build_precond(tabi_ctx *c)
             else
                 k_lu_panel<1, 256><<<cnt, 256, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
                     c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
+            // the class's panel inverses follow its panels on the same stream,
+            // beside the other classes' panels
+            k_lu_linv<<<cnt, 64, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p, c->blk_off.p, c->lu.p,
+                                          c->lu_linv.p);
             if (ncls > 0) HIPCHK(hipEventRecord(ev_join[ncls - 1], ps));
             ++ncls;
             q = e;
         }
         for (int k = 1; k < ncls; ++k) HIPCHK(hipStreamWaitEvent(c->s, ev_join[k - 1], 0));
         const int64_t nsw = sw_off[step + 1] - sw_off[step];
-        k_lu_linv<<<na, 64, 0, c->s>>>(k0, 0, dlorder_p, c->blk_m.p, c->blk_off.p, c->lu.p,
-                                       c->lu_linv.p);
         if (nsw > 0)
             k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
                 c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p);
