@@ -1050,6 +1050,7 @@ __global__ void k_reduce(int n, NearSlots ns, const int32_t *__restrict__ far_ta
 // face j (four per workgroup), lanes over the targets i of j's leaf, so a
 // leaf of ~150 faces keeps most lanes busy.  flags bit 0 selects
 // the matvec-consistent placement of the L2/L3 couplings.
+constexpr int kAsmPerWave = 4;   // source faces per wave in the block assembly
 __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__restrict__ leaf_of,
     const int32_t *__restrict__ blk_ibeg, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const double *__restrict__ tx,
@@ -1058,8 +1059,9 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
     const double *__restrict__ tnz, const double *__restrict__ tarea, double kappa,
     double eps, int flags, double *__restrict__ lu)
 {
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (j >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int jw = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kAsmPerWave;
+    for (int j = jw; j < min(n, jw + kAsmPerWave); ++j) {
     const int l = leaf_of[j];
     const int ib = blk_ibeg[l], m = blk_m[l], nl = m / 2;
     const int jj = j - ib;
@@ -1104,6 +1106,7 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
         A[(size_t)jj * m + ii + nl] = e2;
         A[(size_t)(jj + nl) * m + ii] = e3;
         A[(size_t)(jj + nl) * m + ii + nl] = e4;
+    }
     }
 }
 
@@ -3515,7 +3518,7 @@ void build_precond(tabi_ctx *c)
     HIPCHK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
     for (auto &e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventRecord(ev_up, c->s));
-    k_lu_assemble<<<nblk(c->n, 4), 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
+    k_lu_assemble<<<nblk(c->n, 4 * kAsmPerWave), 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
         c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
         c->prm.kappa, c->prm.eps, c->prm.flags, c->lu.p);
     HIPCHK(hipGetLastError());
