@@ -66,10 +66,28 @@ struct Error : std::runtime_error {
             throw Error(std::string(#expr) + " failed: " + hipGetErrorString(e_));     \
     } while (0)
 
+// The stream whose work uses the device blocks this thread allocates and frees
+// (the context's stream while the library works for a context; null outside,
+// where the caller guarantees that no queued work uses a freed block).
+thread_local hipStream_t g_pool_stream = nullptr;
+
+struct PoolScope {
+    hipStream_t prev;
+    explicit PoolScope(hipStream_t s) : prev(g_pool_stream) { g_pool_stream = s; }
+    ~PoolScope() { g_pool_stream = prev; }
+    PoolScope(const PoolScope &) = delete;
+    PoolScope &operator=(const PoolScope &) = delete;
+};
+
 // Caching device allocator: freed blocks are kept per size class and handed
 // out again, so repeated solves of similar surfaces do not pay hipMalloc /
-// hipFree (which map and unmap pages) each time.  Memory is returned to the
-// runtime by tabi_release_memory().
+// hipFree (which map and unmap pages) each time.  Frees are stream-ordered:
+// a block remembers the stream it was freed on (kernels queued there may
+// still read it) and is handed out again at once only for work on that same
+// stream, which runs after them; for work on another stream only once the
+// freeing stream has drained (hipStreamQuery).  Contexts used from different
+// host threads therefore never receive a block another thread's pending work
+// still reads.  Memory is returned to the runtime by tabi_release_memory().
 class DevicePool {
 public:
     static DevicePool &get()
@@ -82,13 +100,27 @@ public:
     void *alloc(size_t bytes)
     {
         const size_t cls = size_class(bytes);
+        const hipStream_t s = g_pool_stream;
         {
             std::lock_guard<std::mutex> g(mu_);
             auto it = free_.find(cls);
-            if (it != free_.end() && !it->second.empty()) {
-                void *p = it->second.back();
-                it->second.pop_back();
-                return p;
+            if (it != free_.end()) {
+                auto &v = it->second;
+                int looked = 0;
+                for (size_t k = v.size(); k-- > 0;) {
+                    const hipStream_t fs = v[k].s;
+                    bool ok = fs == nullptr || fs == s;
+                    if (!ok && looked < 8) {
+                        ++looked;
+                        ok = hipStreamQuery(fs) == hipSuccess;
+                        if (ok) v[k].s = nullptr;   // drained: safe for any stream
+                    }
+                    if (ok) {
+                        void *p = v[k].p;
+                        v.erase(v.begin() + (std::ptrdiff_t)k);
+                        return p;
+                    }
+                }
             }
         }
         void *p = nullptr;
@@ -105,21 +137,35 @@ public:
     {
         if (!p) return;
         std::lock_guard<std::mutex> g(mu_);
-        free_[size_of_[p]].push_back(p);
+        free_[size_of_[p]].push_back({p, g_pool_stream});
+    }
+    // The stream is about to be destroyed after a synchronisation: its blocks
+    // have no pending users.
+    void retire(hipStream_t s)
+    {
+        if (!s) return;
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto &kv : free_)
+            for (auto &b : kv.second)
+                if (b.s == s) b.s = nullptr;
     }
     void release_all()
     {
         std::lock_guard<std::mutex> g(mu_);
         (void)hipDeviceSynchronize();
         for (auto &kv : free_)
-            for (void *p : kv.second) {
-                (void)hipFree(p);
-                size_of_.erase(p);
+            for (auto &b : kv.second) {
+                (void)hipFree(b.p);
+                size_of_.erase(b.p);
             }
         free_.clear();
     }
 
 private:
+    struct Block {
+        void *p;
+        hipStream_t s;   // stream the block was freed on (null: no pending user)
+    };
     static size_t size_class(size_t b)
     {
         if (b < 256) return 256;
@@ -130,9 +176,19 @@ private:
         return (b + step - 1) / step * step;
     }
     std::mutex mu_;
-    std::map<size_t, std::vector<void *>> free_;
+    std::map<size_t, std::vector<Block>> free_;
     std::map<void *, size_t> size_of_;
 };
+
+// Synchronises a stream, marks its cached blocks free of pending users and
+// destroys it.
+void destroy_stream(hipStream_t s)
+{
+    if (!s) return;
+    (void)hipStreamSynchronize(s);
+    DevicePool::get().retire(s);
+    (void)hipStreamDestroy(s);
+}
 
 template <typename T>
 struct DBuf {
@@ -3705,6 +3761,7 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     c->nj = (c->p + 3) * (c->p + 4) * (c->p + 5) / 6;
     c->ngroups = (n + kWave - 1) / kWave;
     if (!c->s) HIPCHK(hipStreamCreate(&c->s));
+    PoolScope ps_(c->s);
     auto t0 = std::chrono::steady_clock::now();
     // inputs in original face order, host or device
     DevIn d_xyz(xyz, (size_t)3 * n, dev, c->s), d_nrm(nrm, (size_t)3 * n, dev, c->s),
@@ -3903,6 +3960,7 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
 {
     const int R = rs.R, maxl = rs.maxl, ldh = maxl + 1;
     hipStream_t st = rs.s;
+    PoolScope ps_(st);
     const int64_t cst = maxl + 2;                      // per-run stride of coefficients
     const int64_t dvs = ldh + 4;                       // per-run stride of device scalars
     const int sN = ldh, sS = ldh + 1, sT = ldh + 2;    // slots: |w|^2, |w|^2 after MGS, temp
@@ -4675,7 +4733,11 @@ int tabi_init(const tabi_params *prm, int32_t n, const double *xyz, const double
         auto c = std::make_unique<tabi_ctx>();
         ctx_setup(c.get(), prm, n, xyz, nrm, area, orderarr_out, nullptr);
         if (nleaf_out) *nleaf_out = c->nleaf;
-        if (g_default && g_default->s) (void)hipStreamDestroy(g_default->s);
+        if (g_default) {
+            hipStream_t s0 = g_default->s;
+            g_default->s = nullptr;
+            destroy_stream(s0);
+        }
         g_default = std::move(c);
     });
 }
@@ -4684,6 +4746,7 @@ int tabi_matvec(int32_t n2, const double *x, double *y)
 {
     return guarded([&] {
         tabi_ctx *c = need_default();
+        PoolScope ps_(c->s);
         check_n2(c, n2);
         ctx_matvec_host(c, x, y);
     });
@@ -4693,6 +4756,7 @@ int tabi_msolve(int32_t n2, const double *r, double *z)
 {
     return guarded([&] {
         tabi_ctx *c = need_default();
+        PoolScope ps_(c->s);
         check_n2(c, n2);
         ctx_msolve_host(c, r, z);
     });
@@ -4719,8 +4783,9 @@ void tabi_finalize(void)
 {
     if (g_default) {
         hipStream_t s = g_default->s;
+        if (s) (void)hipStreamSynchronize(s);
         g_default.reset();
-        if (s) (void)hipStreamDestroy(s);
+        destroy_stream(s);
     }
 }
 
@@ -4749,13 +4814,14 @@ void tabi_ctx_destroy(tabi_ctx *c)
     hipStream_t s = c->s;
     if (s) (void)hipStreamSynchronize(s);  // pending work may still use the buffers
     delete c;
-    if (s) (void)hipStreamDestroy(s);
+    destroy_stream(s);
 }
 
 int tabi_ctx_stats(tabi_ctx *c, int64_t *out10)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         ctx_stats(c, out10);
     });
 }
@@ -4764,6 +4830,7 @@ int tabi_ctx_matvec_dev(tabi_ctx *c, const double *d_x, double *d_y)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         matvec_dev(c, d_x, d_y);
     });
 }
@@ -4772,6 +4839,7 @@ int tabi_ctx_msolve_dev(tabi_ctx *c, const double *d_r, double *d_z)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         msolve_dev(c, d_r, d_z);
     });
 }
@@ -4780,6 +4848,7 @@ int tabi_ctx_matvec_block(tabi_ctx *c, int32_t nrhs, const double *x, double *y)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (nrhs < 1) throw Error("need at least one right-hand side");
         const int64_t N = 2 * (int64_t)c->n;
         std::vector<int32_t> live;
@@ -4809,6 +4878,7 @@ int tabi_ctx_msolve_block(tabi_ctx *c, int32_t nrhs, const double *r, double *z)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (nrhs < 1) throw Error("need at least one right-hand side");
         const int64_t N = 2 * (int64_t)c->n;
         DBuf<double> dR, dZ;
@@ -4829,6 +4899,7 @@ int tabi_ctx_gmres_block(tabi_ctx *c, int32_t nrhs, const double *b, double *x, 
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (nrhs < 1) throw Error("need at least one right-hand side");
         if (maxl < 1) throw Error("maxl must be >= 1");
         const int64_t N = 2 * (int64_t)c->n;
@@ -4849,6 +4920,7 @@ int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qp
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (nrhs < 1) throw Error("need at least one charge set");
         if (nq < 1) throw Error("need at least one charge");
         const int64_t N = 2 * (int64_t)c->n;
@@ -4889,10 +4961,9 @@ int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
         auto cleanup = [&]() {
             for (auto &c : ctx)
                 if (c && c->s) {
-                    (void)hipStreamSynchronize(c->s);
                     hipStream_t st = c->s;
                     c->s = nullptr;
-                    (void)hipStreamDestroy(st);
+                    destroy_stream(st);
                 }
         };
         // set-up and right-hand sides, several surfaces at a time from host threads
@@ -4906,6 +4977,7 @@ int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
                         for (int k = t; k < nsurf; k += nth) {
                             tabi_ctx *c = ctx[k].get();
                             ctx_setup(c, prm, nface[k], xyz[k], nrm[k], area[k], nullptr, nullptr);
+                            PoolScope ps_(c->s);
                             const int64_t N = 2 * (int64_t)nface[k];
                             bx[k]->alloc((size_t)2 * N);
                             rhs_dev(c, nq[k], qpos[k], q[k], eps_p, bx[k]->p);
@@ -4939,15 +5011,15 @@ int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
                 g_many[1] = (double)rs.graph_launches;
             }
         } catch (...) {
-            (void)hipStreamSynchronize(main);
-            (void)hipStreamDestroy(main);
+            destroy_stream(main);
             cleanup();
             throw;
         }
         HIPCHK(hipStreamSynchronize(main));
-        (void)hipStreamDestroy(main);
+        destroy_stream(main);
         auto t2 = std::chrono::steady_clock::now();
         for (int k = 0; k < nsurf; ++k) {
+            PoolScope ps_(ctx[k]->s);
             energy_dev(ctx[k].get(), nq[k], qpos[k], q[k], eps_p, dX[k], soleng + k, couleng + k);
             iter[k] = it[k];
             ierr[k] = ie[k];
@@ -5000,6 +5072,7 @@ int tabi_ctx_sync(tabi_ctx *c)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         HIPCHK(hipStreamSynchronize(c->s));
     });
 }
@@ -5008,6 +5081,7 @@ int tabi_ctx_matvec(tabi_ctx *c, const double *x, double *y)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         ctx_matvec_host(c, x, y);
     });
 }
@@ -5016,6 +5090,7 @@ int tabi_ctx_msolve(tabi_ctx *c, const double *r, double *z)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         ctx_msolve_host(c, r, z);
     });
 }
@@ -5025,6 +5100,7 @@ int tabi_ctx_rhs(tabi_ctx *c, int32_t nq, const double *qpos, const double *q, d
 {
     return guarded([&] {
         if (!c) throw Error("tabi_init has not been called");
+        PoolScope ps_(c->s);
         DBuf<double> db;
         db.alloc((size_t)2 * c->n);
         rhs_dev(c, nq, qpos, q, eps_p, db.p);
@@ -5038,6 +5114,7 @@ int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q
 {
     return guarded([&] {
         if (!c) throw Error("tabi_init has not been called");
+        PoolScope ps_(c->s);
         DBuf<double> dxs;
         dxs.upload(x, (size_t)2 * c->n, c->s);
         energy_dev(c, nq, qpos, q, eps_p, dxs.p, soleng, couleng);
@@ -5049,6 +5126,7 @@ int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t 
 {
     return guarded([&] {
         if (!c) throw Error("tabi_init has not been called");
+        PoolScope ps_(c->s);
         int it = 0, ie = 0;
         ctx_gmres_host(c, b, x, tol, maxl, nrmax, &it, err, &ie);
         *iter = it;
@@ -5084,6 +5162,7 @@ void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const doub
     try {
         double ph[3] = {0, 0, 0};
         ctx_setup(&c, prm, n, xyz, nrm, area, orderarr_out, ph, dev);
+        PoolScope ps_(c.s);
         auto t1 = std::chrono::steady_clock::now();
         const int64_t N = 2 * (int64_t)n;
         DBuf<double> db, dxs_own;
@@ -5117,16 +5196,14 @@ void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const doub
         g_times[3] = ms(t1, t2); g_times[4] = ms(t2b, t3); g_times[5] = ms(t3, t4);
         g_times[6] = ms(t0, t4); g_times[7] = ms(t2, t2b);
     } catch (...) {
-        if (c.s) {
-            (void)hipStreamSynchronize(c.s);
-            (void)hipStreamDestroy(c.s);
-            c.s = nullptr;
-        }
+        hipStream_t s = c.s;
+        c.s = nullptr;
+        destroy_stream(s);
         throw;
     }
     hipStream_t s = c.s;
     c.s = nullptr;
-    (void)hipStreamDestroy(s);
+    destroy_stream(s);
 }
 
 }  // namespace
@@ -5204,6 +5281,7 @@ int tabi_ctx_ws_features(tabi_ctx *c, int32_t nq, const double *qpos, const doub
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (nq < 1) throw Error("need at least one charge");
         DBuf<double> db;
         DBuf<float> feat;
@@ -5255,6 +5333,7 @@ int tabi_ctx_gmres_ws(tabi_ctx *c, tabi_ws *model, int32_t nq, const double *qpo
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (!model || !model->m) throw Error("null warm-start model");
         const int64_t N = 2 * (int64_t)c->n;
         DBuf<double> db, dxs;
@@ -5280,6 +5359,7 @@ int tabi_ctx_profile(tabi_ctx *c, int32_t enable)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         if (enable && !c->ev[0]) {
             HIPCHK(hipEventCreate(&c->ev[0]));
             HIPCHK(hipEventCreate(&c->ev[1]));
@@ -5293,6 +5373,7 @@ int tabi_ctx_kernel_times(tabi_ctx *c, double *out_ms, int64_t *out_launches)
 {
     return guarded([&] {
         if (!c) throw Error("null context");
+        PoolScope ps_(c->s);
         for (int i = 0; i < 8; ++i) { out_ms[i] = c->kt_ms[i]; out_launches[i] = c->kt_n[i]; }
     });
 }

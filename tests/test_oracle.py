@@ -14,6 +14,7 @@ import cases
 from conftest import load_golden
 
 FAST_CASES = ["sph3_l100_p3", "sph4_off_p6", "sph3_k0_p2", "sph3_direct"]
+_LARGE_CPU = ["mol1500"]
 
 
 def oracle_problem(O, name):
@@ -28,6 +29,8 @@ def oracle_problem(O, name):
 
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fixture_inputs_unchanged(name):
+    if name in cases.LARGE and name not in _LARGE_CPU:
+        pytest.skip("meshing this surface takes minutes; its inputs are checked by its GPU test")
     g = load_golden(name)
     v, nv, f, atoms, _ = cases.build(name)
     assert str(g["input_sha256"]) == cases.input_hash(v, nv, f, atoms)
