@@ -37,6 +37,9 @@ typedef struct {
                             placement instead of the reference's (tc.f:1481-1488);
                             bit 1: build the tree with the sequential host
                             partition instead of the GPU one (same result);
+                            bit 2: diagonal-scaling preconditioner MSOLVEprec
+                            (src/main.f90:532-542, usrdata.in iprcd != 1)
+                            instead of the leaf-block LU (no factorisation);
                             bits 8-11 (t): field kernels cut the work into about
                             2^(10+t) tasks per launch (t = 0: 2^16) */
 } tabi_params;
@@ -72,6 +75,24 @@ int tabi_rhs(int32_t nq, const double *qpos, const double *q, double eps_p, doub
  * (src/main.f90:144-157, potential_molecule src/main.f90:614-657). */
 int tabi_energy(int32_t nq, const double *qpos, const double *q, double eps_p,
                 const double *x, double *soleng, double *couleng);
+
+/* Vertex-averaged surface potential (output_potential, src/main.f90:226-326):
+ * for each of nvert vertices vpos[3*nvert], the mean of the tree-order solution
+ * x[2n] over the first 14 faces in the original numbering that use the vertex,
+ * weighted by 1/|centroid - vertex|, scaled by para*4*pi (src/main.f90:245,
+ * 289): vpot[2*nvert] = (phi, dphi/dn) per vertex, as surface_potential.dat
+ * holds them; a vertex no face uses gets NaN, as in the reference.  faces[3n]
+ * are the surface's (culled) faces in the original order, 1-based vertex
+ * indices (the nvert array of the reference). */
+int tabi_vertex_potential(int32_t nvert, const double *vpos, const int32_t *faces,
+                          const double *x, double *vpot);
+
+/* Writes surface_potential.dat in the reference's layout (src/main.f90:307-316):
+ * "nspt nface", per vertex (i10,6f12.6,2f20.10) of index, position, normal and
+ * vpot, then each face's three 1-based vertex indices. */
+int tabi_write_surface_potential(const char *path, int32_t nvert, const double *vpos,
+                                 const double *vnrm, const double *vpot, int32_t nface,
+                                 const int32_t *faces);
 
 /* GPU-resident restarted GMRES with the reference's settings and stopping rule
  * (DGMRES with ITOL=2, JPRE=-1, JSCAL=0, src/dgmres_dep.f:279-852; MAXL/NRMAX
@@ -113,6 +134,8 @@ int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q
                     double eps_p, const double *x, double *soleng, double *couleng);
 int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
                    int32_t nrmax, int32_t *iter, double *err, int32_t *ierr);
+int tabi_ctx_vertex_potential(tabi_ctx *c, int32_t nvert, const double *vpos,
+                              const int32_t *faces, const double *x, double *vpot);
 
 /* ---- blocks of right-hand sides on one surface ------------------------------
  * Several vectors (charge sets, Krylov vectors) share one tree pass: the near-

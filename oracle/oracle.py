@@ -188,3 +188,35 @@ def taylor_coeffs(d, kappa, lim):
     lib().or_taylor_coeffs(C.c_double(d[0]), C.c_double(d[1]), C.c_double(d[2]),
                            C.c_double(kappa), C.c_int32(lim), _p(out))
     return out.reshape(lim + 1, lim + 1, lim + 1)
+
+
+def direct_rows(prm, xyz, nrm, area, x, targets):
+    """Rows of the O(n^2) direct operator for the listed targets, shape (k, 2)."""
+    xyz, nrm, area, x = _d(xyz), _d(nrm), _d(area), _d(x)
+    targets = _i(targets)
+    y = np.zeros(2 * len(targets))
+    lib().or_direct_rows(C.byref(prm), C.c_int32(xyz.shape[0]), _p(xyz), _p(nrm), _p(area),
+                         _p(x), C.c_int32(len(targets)), _p(targets), _p(y))
+    return y.reshape(-1, 2)
+
+
+def diag_prec(eps, r):
+    r = _d(r)
+    z = np.zeros_like(r)
+    lib().or_diag_prec(C.c_int32(r.size), C.c_double(eps), _p(r), _p(z))
+    return z
+
+
+def vertex_potential(vpos, faces, xyz, x):
+    """output_potential's vertex values; faces 0-based and xyz, x in the
+    original face order."""
+    vpos, xyz, x = _d(vpos), _d(xyz), _d(x)
+    f1 = _i(np.asarray(faces).reshape(-1, 3) + 1)
+    out = np.zeros((vpos.shape[0], 2))
+    f = lib().or_vertex_potential
+    f.restype = C.c_int32
+    rc = f(C.c_int32(f1.shape[0]), C.c_int32(vpos.shape[0]), _p(vpos), _p(f1), _p(xyz), _p(x),
+           _p(out))
+    if rc:
+        raise ValueError("vertex index out of range")
+    return out

@@ -1149,3 +1149,76 @@ double or_energy(int32_t n, const double *xyz, const double *nrm, const double *
     if (couleng) *couleng = cou * para;
     return sol;
 }
+
+/* ------------------------------------------------------------------------- */
+/* post-processing and the other preconditioner                               */
+/* ------------------------------------------------------------------------- */
+void or_direct_rows(const or_params *prm, int32_t n, const double *xyz, const double *nrm,
+                    const double *area, const double *x, int32_t nt, const int32_t *targets,
+                    double *y)
+{
+    /* rows of the O(n^2) direct operator (the treecode at theta = 0,
+     * src/treecode3d_pb.f:1158-1185 with every source direct) for the listed
+     * targets; y[2k], y[2k+1] for target k.  Targets are independent, so they
+     * are spread over threads (each row's sum keeps the serial order). */
+    double pre1 = 0.5 * (1.0 + prm->eps), pre2 = 0.5 * (1.0 + 1.0 / prm->eps);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int32_t k = 0; k < nt; ++k) {
+        const int32_t i = targets[k];
+        double peng[2] = {0.0, 0.0};
+        for (int32_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            pair_pb(prm->kappa, prm->eps, xyz + 3 * (int64_t)i, nrm + 3 * (int64_t)i,
+                    xyz + 3 * (int64_t)j, nrm + 3 * (int64_t)j, area[j], x[j], x[n + j], peng);
+        }
+        y[2 * (int64_t)k] = pre1 * x[i] - peng[0];
+        y[2 * (int64_t)k + 1] = pre2 * x[n + i] - peng[1];
+    }
+}
+
+void or_diag_prec(int32_t n2, double eps, const double *r, double *z)
+{
+    /* MSOLVEprec, src/main.f90:532-542 */
+    double scale1 = 0.5 * (1.0 + eps), scale2 = 0.5 * (1.0 + 1.0 / eps);
+    for (int32_t i = 0; i < n2 / 2; ++i) z[i] = r[i] / scale1;
+    for (int32_t i = n2 / 2; i < n2; ++i) z[i] = r[i] / scale2;
+}
+
+int32_t or_vertex_potential(int32_t n, int32_t nvert, const double *vpos, const int32_t *faces1,
+                            const double *xyz, const double *x, double *vpot)
+{
+    /* output_potential, src/main.f90:226-326, in the original face order
+     * (xyz: centroids, x: solution un-permuted as at main.f90:249-254).  Each
+     * vertex keeps the first 14 faces that list it (ind_vert, main.f90:257-268),
+     * then the 1/distance-weighted means (main.f90:270-288), scaled by
+     * para*4*pi (main.f90:245,289).  Returns 1 for a vertex index out of range. */
+    enum { NSLOT = 14 };
+    const double para_temp = 332.0716 * 4 * PI;
+    int32_t *cnt = calloc((size_t)nvert, sizeof(int32_t));
+    int32_t *slot = malloc(sizeof(int32_t) * (size_t)nvert * NSLOT);
+    if (!cnt || !slot) { free(cnt); free(slot); return 2; }
+    for (int32_t i = 0; i < n; ++i)
+        for (int j = 0; j < 3; ++j) {
+            int32_t v = faces1[3 * (int64_t)i + j] - 1;
+            if (v < 0 || v >= nvert) { free(cnt); free(slot); return 1; }
+            if (cnt[v] < NSLOT) slot[(int64_t)v * NSLOT + cnt[v]++] = i;
+        }
+    for (int32_t v = 0; v < nvert; ++v) {
+        double s1 = 0.0, s2 = 0.0, tot = 0.0;
+        const double *p = vpos + 3 * (int64_t)v;
+        for (int32_t k = 0; k < cnt[v]; ++k) {
+            int32_t f = slot[(int64_t)v * NSLOT + k];
+            const double *c = xyz + 3 * (int64_t)f;
+            double aa[3] = {c[0] - p[0], c[1] - p[1], c[2] - p[2]};
+            double len = sqrt(dot3(aa, aa));
+            s1 = s1 + 1.0 / len * x[f];
+            s2 = s2 + 1.0 / len * x[f + n];
+            tot = tot + 1.0 / len;
+        }
+        vpot[2 * (int64_t)v] = s1 / tot * para_temp;
+        vpot[2 * (int64_t)v + 1] = s2 / tot * para_temp;
+    }
+    free(cnt);
+    free(slot);
+    return 0;
+}

@@ -116,6 +116,16 @@ double or_energy(int32_t n, const double *xyz, const double *nrm, const double *
 void or_taylor_coeffs(double dx, double dy, double dz, double kappa, int32_t lim,
                       double *out);
 
+/* rows of the O(n^2) direct operator for the listed targets (y[2k], y[2k+1]) */
+void or_direct_rows(const or_params *prm, int32_t n, const double *xyz, const double *nrm,
+                    const double *area, const double *x, int32_t nt, const int32_t *targets,
+                    double *y);
+/* MSOLVEprec (src/main.f90:532-542) */
+void or_diag_prec(int32_t n2, double eps, const double *r, double *z);
+/* output_potential vertex values (src/main.f90:226-326), original face order */
+int32_t or_vertex_potential(int32_t n, int32_t nvert, const double *vpos, const int32_t *faces1,
+                            const double *xyz, const double *x, double *vpot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,7 @@
 //   (batched two-level blocked LU with partial pivoting over all leaf blocks
 //   at once).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -3062,8 +3063,29 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     HIPCHK(hipGetLastError());
 }
 
+// Diagonal scaling, MSOLVEprec (src/main.f90:532-542): z = r / (1/2)(1+eps) on
+// the potential half and r / (1/2)(1+1/eps) on the normal-derivative half,
+// divided as the reference divides (bit-identical).  Vector y of a block is
+// act[y] (all vectors in order without a list), stored at stride rs / zs.
+__global__ void k_diag_prec(int n, double eps, const double *__restrict__ rv, int64_t rs,
+                            double *__restrict__ zv, int64_t zs, const int32_t *__restrict__ act)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * (int64_t)n) return;
+    const int64_t k = act ? act[blockIdx.y] : blockIdx.y;
+    const double scale1 = 0.5 * (1.0 + eps), scale2 = 0.5 * (1.0 + 1.0 / eps);
+    zv[k * zs + i] = rv[k * rs + i] / (i < n ? scale1 : scale2);
+}
+
 void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
 {
+    if (c->prm.flags & 4) {
+        KTimer kt(c, 4);
+        k_diag_prec<<<nblk(2 * (int64_t)c->n, 256), 256, 0, c->s>>>(c->n, c->prm.eps, dr, 0, dz, 0,
+                                                                  nullptr);
+        HIPCHK(hipGetLastError());
+        return;
+    }
     const size_t shm = sizeof(double) * (size_t)c->mmax;
     KTimer kt(c, 4);
     k_lu_solve<4><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
@@ -3195,6 +3217,12 @@ void msolve_block_dev(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *
                       const int32_t *d_act)
 {
     KTimer kt(c, 4);
+    if (c->prm.flags & 4) {
+        k_diag_prec<<<dim3(nblk(2 * (int64_t)c->n, 256), (unsigned)R), 256, 0, c->s>>>(c->n,
+            c->prm.eps, Rv, rs, Z, zs, d_act);
+        HIPCHK(hipGetLastError());
+        return;
+    }
     const int ldz = c->mmax;
     const size_t lim = 64 * 1024;
     if (4 * sizeof(double) * (size_t)ldz <= lim) launch_lu_solveb<4>(c, R, Rv, rs, Z, zs, d_act, ldz);
@@ -3816,7 +3844,10 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     auto t1 = std::chrono::steady_clock::now();
     build_lists(c);
     auto t2 = std::chrono::steady_clock::now();
-    build_precond(c);
+    if (prm->flags & 4)
+        c->nleaf = (int32_t)c->tree.leaves.size();   // diagonal scaling: no leaf blocks
+    else
+        build_precond(c);
     auto t3 = std::chrono::steady_clock::now();
     if (t_phase) {
         t_phase[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -4530,12 +4561,12 @@ struct SingleRun : RunSet {
 // dropped and *x0_used reports whether the guess was kept.
 void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl, int nrmax,
                int *iter_out, double *err_out, int *ierr_out, bool fallback = false,
-               int *x0_used = nullptr)
+               int *x0_used = nullptr, bool x0_zero = false)
 {
     const int64_t N = 2 * (int64_t)c->n;
     if (maxl > N) maxl = (int)N;
     SingleRun rs(c, maxl, db, dxs);
-    gmres_runs(rs, c->blk_red, tol, nrmax, false, iter_out, err_out, ierr_out, fallback, x0_used);
+    gmres_runs(rs, c->blk_red, tol, nrmax, x0_zero, iter_out, err_out, ierr_out, fallback, x0_used);
 }
 
 // Right-hand sides of R charge sets (q[R][nq], host) at common positions
@@ -4653,6 +4684,130 @@ void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double
     HIPCHK(hipStreamSynchronize(c->s));
     *sol = s * 0.5 * para * 4.0 * kPi;
     *cou = cc / eps_p * para;
+}
+
+// ---------------------------------------------------------------------------
+// Vertex-averaged surface potential (output_potential, src/main.f90:226-326).
+// Every vertex takes the first 14 faces that use it, in the original face
+// numbering (the reference's ind_vert slots, nface_vert-1 = 14; later faces are
+// dropped), and averages phi and dphi/dn over them weighted by 1/|centroid -
+// vertex|; the results are scaled by para*4*pi as the file holds them.  A vertex
+// that no face uses gets 0/0 = NaN, as in the reference.  The incidences
+// (vertex, face) are listed in face order and stably sorted by vertex, which
+// keeps each vertex's faces ascending.
+// ---------------------------------------------------------------------------
+constexpr int kVertFaces = 14;
+
+__global__ void k_vp_incid(int n, int nvert, const int32_t *__restrict__ faces1,
+                           int32_t *__restrict__ key, int32_t *__restrict__ val,
+                           int32_t *__restrict__ bad)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * (int64_t)n) return;
+    int v = faces1[i] - 1;
+    if (v < 0 || v >= nvert) {
+        atomicOr(bad, 1);
+        v = nvert;          // sorted past every vertex, never read
+    }
+    key[i] = v;
+    val[i] = (int32_t)(i / 3);
+}
+
+// beg[v] = first sorted incidence of vertex v (lower bound), v = 0..nvert
+__global__ void k_vp_seg(int m, int nvert, const int32_t *__restrict__ key,
+                         int32_t *__restrict__ beg)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > nvert) return;
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    beg[v] = lo;
+}
+
+__global__ void k_vertex_pot(int n, int nvert, const int32_t *__restrict__ beg,
+                             const int32_t *__restrict__ face, const int32_t *__restrict__ inv,
+                             const double *__restrict__ vpos, const double *__restrict__ tx,
+                             const double *__restrict__ ty, const double *__restrict__ tz,
+                             const double *__restrict__ x, double scale, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvert) return;
+    const int b = beg[v], e = min(beg[v + 1], b + kVertFaces);
+    const double px = vpos[3 * (int64_t)v], py = vpos[3 * (int64_t)v + 1], pz = vpos[3 * (int64_t)v + 2];
+    double s1 = 0.0, s2 = 0.0, tot = 0.0;
+    for (int k = b; k < e; ++k) {
+        const int t = inv[face[k]];
+        const double ax = tx[t] - px, ay = ty[t] - py, az = tz[t] - pz;
+        const double w = 1.0 / sqrt(ax * ax + ay * ay + az * az);
+        s1 = s1 + w * x[t];
+        s2 = s2 + w * x[n + t];
+        tot = tot + w;
+    }
+    out[2 * (int64_t)v] = s1 / tot * scale;
+    out[2 * (int64_t)v + 1] = s2 / tot * scale;
+}
+
+// faces1[3n]: the surface's faces in the original numbering (1-based vertex
+// indices, the nvert array of the reference after culling); x: tree-order
+// solution; vpot[2*nvert]: (phi, dphi/dn) per vertex.  Host pointers.
+void vertex_potential(tabi_ctx *c, int nvert, const double *vpos, const int32_t *faces1,
+                      const double *x, double *vpot)
+{
+    if (nvert < 1) throw Error("need at least one vertex");
+    const int n = c->n;
+    const int m = 3 * n;
+    DBuf<int32_t> dfaces, key, val, key2, val2, beg, inv, bad;
+    DBuf<double> dv, dx, dout;
+    dfaces.upload(faces1, (size_t)m, c->s);
+    dv.upload(vpos, (size_t)3 * nvert, c->s);
+    dx.upload(x, (size_t)2 * n, c->s);
+    {
+        std::vector<int32_t> hinv(n);
+        for (int t = 0; t < n; ++t) hinv[c->tree.order[t]] = t;
+        inv.upload(hinv.data(), n, c->s);
+    }
+    key.alloc(m); val.alloc(m); key2.alloc(m); val2.alloc(m);
+    beg.alloc((size_t)nvert + 1); bad.alloc(1); dout.alloc((size_t)2 * nvert);
+    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int32_t), c->s));
+    k_vp_incid<<<nblk(m, 256), 256, 0, c->s>>>(n, nvert, dfaces.p, key.p, val.p, bad.p);
+    HIPCHK(hipGetLastError());
+    int end_bit = 1;
+    while (end_bit < 31 && (1 << end_bit) <= nvert) ++end_bit;
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key2.p, val.p, val2.p, m, 0,
+                                              end_bit, c->s));
+    DBuf<char> tmp;
+    tmp.alloc(std::max<size_t>(tmp_bytes, 1));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, key.p, key2.p, val.p, val2.p, m, 0,
+                                              end_bit, c->s));
+    k_vp_seg<<<nblk((int64_t)nvert + 1, 256), 256, 0, c->s>>>(m, nvert, key2.p, beg.p);
+    const double scale = 332.0716 * 4.0 * kPi;   // para*4*pi, src/main.f90:64,245
+    k_vertex_pot<<<nblk(nvert, 256), 256, 0, c->s>>>(n, nvert, beg.p, val2.p, inv.p, dv.p, c->tx.p,
+                                                     c->ty.p, c->tz.p, dx.p, scale, dout.p);
+    HIPCHK(hipGetLastError());
+    int32_t hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(vpot, dout.p, sizeof(double) * 2 * nvert, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (hbad) throw Error("face vertex index outside 1.." + std::to_string(nvert));
+}
+
+// One value in a Fortran Fw.d field: fixed notation right-aligned in w
+// columns, asterisks when it does not fit, NaN / Infinity spelled out.
+void fortran_f(std::string &out, double v, int w, int d)
+{
+    char buf[64];
+    if (std::isnan(v)) std::snprintf(buf, sizeof buf, "NaN");
+    else if (std::isinf(v)) std::snprintf(buf, sizeof buf, v > 0 ? "Infinity" : "-Infinity");
+    else std::snprintf(buf, sizeof buf, "%.*f", d, v);
+    const int len = (int)std::strlen(buf);
+    if (len > w) { out.append((size_t)w, '*'); return; }
+    out.append((size_t)(w - len), ' ');
+    out.append(buf);
 }
 
 std::unique_ptr<tabi_ctx> g_default;
@@ -5121,6 +5276,50 @@ int tabi_ctx_energy(tabi_ctx *c, int32_t nq, const double *qpos, const double *q
     });
 }
 
+int tabi_ctx_vertex_potential(tabi_ctx *c, int32_t nvert, const double *vpos,
+                              const int32_t *faces, const double *x, double *vpot)
+{
+    return guarded([&] {
+        if (!c) throw Error("tabi_init has not been called");
+        PoolScope ps_(c->s);
+        if (!vpos || !faces || !x || !vpot) throw Error("null array");
+        vertex_potential(c, nvert, vpos, faces, x, vpot);
+    });
+}
+
+int tabi_vertex_potential(int32_t nvert, const double *vpos, const int32_t *faces, const double *x,
+                          double *vpot)
+{
+    return tabi_ctx_vertex_potential(g_default.get(), nvert, vpos, faces, x, vpot);
+}
+
+int tabi_write_surface_potential(const char *path, int32_t nvert, const double *vpos,
+                                 const double *vnrm, const double *vpot, int32_t nface,
+                                 const int32_t *faces)
+{
+    return guarded([&] {
+        if (!path || nvert < 0 || nface < 0) throw Error("bad arguments");
+        FILE *f = std::fopen(path, "w");
+        if (!f) throw Error(std::string("cannot open ") + path);
+        std::string line;
+        line.reserve(160);
+        bool ok = std::fprintf(f, " %d %d\n", nvert, nface) > 0;
+        for (int64_t i = 0; ok && i < nvert; ++i) {
+            char idx[32];
+            std::snprintf(idx, sizeof idx, "%10lld", (long long)(i + 1));
+            line.assign(idx);
+            for (int k = 0; k < 3; ++k) fortran_f(line, vpos[3 * i + k], 12, 6);
+            for (int k = 0; k < 3; ++k) fortran_f(line, vnrm[3 * i + k], 12, 6);
+            for (int k = 0; k < 2; ++k) fortran_f(line, vpot[2 * i + k], 20, 10);
+            line.push_back('\n');
+            ok = std::fputs(line.c_str(), f) >= 0;
+        }
+        for (int64_t j = 0; ok && j < nface; ++j)
+            ok = std::fprintf(f, " %d %d %d\n", faces[3 * j], faces[3 * j + 1], faces[3 * j + 2]) > 0;
+        if (std::fclose(f) != 0 || !ok) throw Error(std::string("write failed: ") + path);
+    });
+}
+
 int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t maxl,
                    int32_t nrmax, int32_t *iter, double *err, int32_t *ierr)
 {
@@ -5180,7 +5379,9 @@ void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const doub
         }
         auto t2b = std::chrono::steady_clock::now();
         int it = 0, ie = 0, used = 0;
-        gmres_dev(&c, db.p, dxs, tol, 10, 10, &it, err, &ie, model != nullptr, &used);
+        // without a model dxs was just zeroed: no host-side check of the guess
+        gmres_dev(&c, db.p, dxs, tol, 10, 10, &it, err, &ie, model != nullptr, &used,
+                  model == nullptr);
         auto t3 = std::chrono::steady_clock::now();
         energy_dev(&c, nq, qpos, q, eps_p, dxs, soleng, couleng, dev);
         auto t4 = std::chrono::steady_clock::now();

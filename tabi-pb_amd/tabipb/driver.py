@@ -97,26 +97,53 @@ class Problem:
 
     def params(self, precision=0, flags=0, order=None, maxparnode=None, theta=None):
         u = self.usr
+        if u.iprcd != 1:
+            # MSOLVE => MSOLVEprec unless iprcd = 1 (src/main.f90:118-123)
+            flags |= hip.FLAG_DIAG_PREC
         return hip.make_params(self.eps, self.kappa, u.mac if theta is None else theta,
                                u.order if order is None else order,
                                u.maxparnode if maxparnode is None else maxparnode,
                                precision, flags)
 
 
-def load_problem(pqr_path: str, msms_prefix: str, usr: meshio.UsrData) -> Problem:
-    qpos, q, _ = meshio.read_pqr(pqr_path)
-    v, vn, f = meshio.read_msms(msms_prefix)
+def problem_from_mesh(usr: meshio.UsrData, v, vn, f, qpos, q) -> Problem:
+    """Culls the faces as readin does (surface_area) and forms the geometry."""
     f, _ = cull_faces(v, f)
     xyz, nrm, area = face_geometry(v, vn, f)
     return Problem(usr, xyz, nrm, area, qpos, q, v, vn, f)
 
 
+def load_problem(pqr_path: str, msms_prefix: str, usr: meshio.UsrData) -> Problem:
+    qpos, q, _ = meshio.read_pqr(pqr_path)
+    v, vn, f = meshio.read_msms(msms_prefix)
+    return problem_from_mesh(usr, v, vn, f, qpos, q)
+
+
 def load_run_dir(path: str) -> Problem:
-    """A directory laid out for the reference program: usrdata.in and
-    test_proteins/<fname>.{pqr,vert,face} (src/readin.f90:18-30,204,228)."""
+    """A directory laid out for the reference program: usrdata.in,
+    test_proteins/<fname>.pqr and the surface files of its isurf_type -- MSMS
+    test_proteins/<fname>.{vert,face} (1), NanoShaper triangulatedSurf.{vert,face}
+    (2) or ESES marching_cubes.obj (4) in the directory itself
+    (src/readin.f90:18-30,90-246)."""
     usr = meshio.UsrData.read(os.path.join(path, "usrdata.in"))
     base = os.path.join(path, "test_proteins", usr.fname)
-    return load_problem(base + ".pqr", base, usr)
+    qpos, q, _ = meshio.read_pqr(base + ".pqr")
+    if usr.isurf_type == 4:
+        v, vn, f = meshio.read_eses(os.path.join(path, "marching_cubes.obj"))
+    elif usr.isurf_type == 2:
+        v, vn, f = meshio.read_nanoshaper(path)
+    else:
+        v, vn, f = meshio.read_msms(base)
+    return problem_from_mesh(usr, v, vn, f, qpos, q)
+
+
+def output_potential(S: "hip.Surface", prob: Problem, x, path: str = None):
+    """output_potential (src/main.f90:226-326) for a solved surface: the vertex
+    values, and surface_potential.dat written to path when given."""
+    vpot = S.vertex_potential(prob.vpos, prob.faces, x)
+    if path is not None:
+        hip.write_surface_potential(path, prob.vpos, prob.vnrm, vpot, prob.faces)
+    return vpot
 
 
 def solve(prob: Problem, tol=1e-4, precision=0, flags=0, model=None, **over):

@@ -26,8 +26,14 @@ EXPORTS = (
     "tabi_ctx_ws_features", "tabi_ws_predict", "tabi_ws_grad", "tabi_ws_train",
     "tabi_ctx_gmres_ws", "tabi_solve_ws", "tabi_ctx_matvec_block", "tabi_ctx_msolve_block",
     "tabi_ctx_gmres_block", "tabi_ctx_solve_block", "tabi_solve_many",
-    "tabi_last_solve_many_stats", "tabi_solve_dev",
+    "tabi_last_solve_many_stats", "tabi_solve_dev", "tabi_vertex_potential",
+    "tabi_ctx_vertex_potential", "tabi_write_surface_potential",
 )
+
+# tabi_params.flags bits
+FLAG_MATVEC_L23 = 1       # preconditioner with matvec-consistent L2/L3 placement
+FLAG_HOST_TREE = 2        # sequential host tree build
+FLAG_DIAG_PREC = 4        # diagonal scaling MSOLVEprec (usrdata.in iprcd != 1)
 
 SOLVE_PHASES = ("tree", "lists", "precond", "rhs", "gmres", "energy", "total", "warmstart")
 
@@ -240,6 +246,20 @@ class Surface:
                                       C.c_double(eps_p), _p(x), C.byref(s), C.byref(cc)))
         return s.value, cc.value
 
+    def vertex_potential(self, vpos, faces, x):
+        """output_potential's vertex values (src/main.f90:226-326): (nvert, 2)
+        array of (phi, dphi/dn) scaled by para*4*pi; faces 0-based, original
+        order (the culled face list); x tree order."""
+        vpos = _d(vpos, (-1, 3))
+        f1 = np.ascontiguousarray(np.asarray(faces).reshape(-1, 3) + 1, dtype=np.int32)
+        if f1.shape[0] != self.n:
+            raise TabiError(f"{f1.shape[0]} faces for a surface of {self.n}")
+        x = _d(x)
+        out = np.empty((vpos.shape[0], 2))
+        _check(load().tabi_ctx_vertex_potential(self._h, C.c_int32(vpos.shape[0]), _p(vpos),
+                                                _p(f1), _p(x), _p(out)))
+        return out
+
     def gmres(self, b, x0=None, tol=1e-4, maxl=10, nrmax=10):
         b = _d(b)
         x = np.zeros_like(b) if x0 is None else _d(x0).copy()
@@ -394,6 +414,15 @@ class WsModel:
 
 
 # ---- default-context drop-in API (what a Fortran or C driver binds) --------
+def write_surface_potential(path, vpos, vnrm, vpot, faces):
+    """surface_potential.dat through the library's writer (faces 0-based)."""
+    vpos, vnrm, vpot = _d(vpos, (-1, 3)), _d(vnrm, (-1, 3)), _d(vpot, (-1, 2))
+    f1 = np.ascontiguousarray(np.asarray(faces).reshape(-1, 3) + 1, dtype=np.int32)
+    _check(load().tabi_write_surface_potential(os.fsencode(path), C.c_int32(vpos.shape[0]),
+                                               _p(vpos), _p(vnrm), _p(vpot),
+                                               C.c_int32(f1.shape[0]), _p(f1)))
+
+
 def dropin_init(prm: Params, xyz, nrm, area):
     """tabi_init: returns (orderarr 1-based, nleaf)."""
     xyz, nrm, area = _d(xyz, (-1, 3)), _d(nrm, (-1, 3)), _d(area)

@@ -7,7 +7,8 @@ computes as compressed .npz files next to this script:
   orderarr (1-based), leaf ranges, tree-order right-hand side, y = A x and
   z = M^-1 x for the harness test vector, GMRES iter/err/ierr, the solution,
   the solvation/Coulomb energies, and the vertex values of the
-  surface_potential.dat that the reference's output_potential writes.
+  surface_potential.dat that the reference's output_potential writes (and, up
+  to 8,000 faces, the file's text).
 
 Vectors are stored in full up to 8,000 faces, every 8th entry up to 50,000
 faces and every 64th entry beyond (plus norms); beyond 50,000 faces the
@@ -77,7 +78,7 @@ def read_outputs(d):
     return out
 
 
-def run_case(name, keep=None):
+def run_case(name):
     v, nv, f, atoms, usr = cases.build(name)
     opt = cases.harness_opts(name)
     args = [f"tol={opt['tol']!r}", f"nrmax={opt['nrmax']}", "time"]
@@ -95,9 +96,8 @@ def run_case(name, keep=None):
             raise RuntimeError(r.stdout + r.stderr)
         o = read_outputs(d)
         times = [ln for ln in r.stdout.splitlines() if ln.startswith("harness-time")]
-        if keep:
-            import shutil
-            shutil.copyfile(os.path.join(d, "surface_potential.dat"), keep)
+        pp = os.path.join(d, "surface_potential.dat")
+        pot_text = open(pp).read() if os.path.exists(pp) else None
     n = o["n"]
     stride = 1 if n <= 8000 else (8 if n <= 50000 else 64)
     s = slice(None, None, stride)
@@ -126,6 +126,8 @@ def run_case(name, keep=None):
                    pot_nface=np.int64(len(o["pot_faces"])))
     if stride == 1:
         rec.update(txyz=o["txyz"], tnrm=o["tnrm"], tarea=o["tarea"])
+        if pot_text is not None:
+            rec["pot_text"] = np.array(pot_text)
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **rec)
     print(f"{name}: n={n} leaves={len(o['leaf_beg'])} iter={o.get('iter')} "
           f"ierr={o.get('ierr')} soleng={o.get('soleng')}", flush=True)
