@@ -1,0 +1,193 @@
+"""GPU parity and accuracy at the benchmark's own sizes.
+
+* The 483,572-face surface of the 8,000-atom synthetic molecule (the
+  benchmark workload): tree permutation (SHA-256 of the reference's), y = A x
+  and z = M^-1 x against the reference's own products (flang build, golden
+  fixture mol8000_mv), at p=3/theta=0.8; rows of A x at p=3/theta=0.8 and
+  p=6/theta=0.5 against the oracle's treecode on 2,000 seeded targets (FP64 and
+  mixed precision); the truncation error of both against the O(n^2) FP64
+  direct sum on the same targets; the true preconditioned residual of the
+  GMRES solutions and the mixed-precision energy against FP64.
+* The 180,900-face surface of the 1,500-atom molecule (config 2): the
+  reference's complete solve (golden fixture mol1500).
+* 64 charge sets on that surface as one block (config 4a), the first set being
+  the fixture's charges.
+
+Tolerances as in test_gpu.py: integers exact, FP64 1e-10 relative L2, mixed
+precision 1e-5 on the matvec and 1e-3 on the energy (the benchmark's bound).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import cases
+from conftest import load_golden
+from tabipb import driver, hip
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+_PROB = {}
+
+
+def golden_problem(name):
+    """The case's problem as the reference reads it (charges through real*4)."""
+    if name not in _PROB:
+        print(f"[gpu] meshing {name}", flush=True)
+        g = load_golden(name)
+        v, nv, f, atoms, usr = cases.build(name)
+        assert str(g["input_sha256"]) == cases.input_hash(v, nv, f, atoms)
+        a32 = atoms.astype(np.float32).astype(np.float64)
+        _PROB[name] = (driver.problem_from_mesh(usr, v, nv, f, a32[:, :3], a32[:, 3]), g)
+    return _PROB[name]
+
+
+def check_order(S, g):
+    o = np.ascontiguousarray(S.orderarr, np.int32)
+    if "orderarr" in g:
+        np.testing.assert_array_equal(o, g["orderarr"])
+    else:
+        np.testing.assert_array_equal(o[::64], g["orderarr_sample"])
+        assert hashlib.sha256(o.tobytes()).hexdigest() == str(g["orderarr_sha256"])
+
+
+# ---------------------------------------------------------------------------
+# config 3: 483,572 faces
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def big():
+    P, g = golden_problem("mol8000_mv")
+    print(f"[gpu] set-up 483k surface n={P.xyz.shape[0]}", flush=True)
+    S = hip.Surface(P.params(), P.xyz, P.nrm, P.area)
+    yield P, g, S
+    S.close()
+
+
+def test_config3_matches_reference_products(big):
+    P, g, S = big
+    assert S.n == int(g["n"]) == 483572
+    check_order(S, g)
+    assert S.stats()["nleaf"] == len(g["leaf_beg"]) == 3354
+    s = int(g["stride"])
+    x = cases.harness_x(2 * S.n)
+    y = S.matvec(x)
+    assert rel_l2(y[::s], g["y"]) <= 1e-10
+    assert np.linalg.norm(y) == pytest.approx(float(g["y_norm"]), rel=1e-10)
+    z = S.msolve(x)
+    assert rel_l2(z[::s], g["z"]) <= 1e-10
+    assert np.linalg.norm(z) == pytest.approx(float(g["z_norm"]), rel=1e-10)
+    b = S.rhs(P.qpos, P.q, P.usr.epsp)
+    assert rel_l2(b[::s], g["bvct"]) <= 1e-13
+
+
+@pytest.mark.parametrize("order,theta", [(3, 0.8), (6, 0.5)])
+def test_config3_rows_against_oracle_and_direct_sum(big, oracle_lib, order, theta):
+    """GPU rows vs the oracle's treecode (same p, theta) on 2,000 seeded targets:
+    FP64 within 1e-10, mixed within 1e-5; both against the O(n^2) direct sum:
+    the treecode's truncation error, p=6/theta=0.5 well below p=3/theta=0.8."""
+    O = oracle_lib
+    P, g, _ = big
+    n = P.xyz.shape[0]
+    txyz, order0, nodes = O.build_tree(P.xyz, P.usr.maxparnode)
+    tn, ta = P.nrm[order0], P.area[order0]
+    x = cases.harness_x(2 * n)
+    targets = np.sort(np.random.default_rng(2024).choice(n, 2000, replace=False))
+    prm = O.params(P.eps, P.kappa, theta, order, P.usr.maxparnode)
+    yo, _ = O.treecode_matvec_targets(prm, txyz, tn, ta, nodes, x, targets)
+    yd = O.direct_rows(prm, txyz, tn, ta, x, targets)
+    errs = {}
+    for precision, tol in ((0, 1e-10), (1, 1e-5)):
+        S = hip.Surface(P.params(precision=precision, order=order, theta=theta),
+                        P.xyz, P.nrm, P.area)
+        assert np.array_equal(S.perm, order0)
+        y = S.matvec(x)
+        yg = np.stack([y[targets], y[n + targets]], axis=1)
+        assert rel_l2(yg, yo) <= tol, precision
+        errs[precision] = rel_l2(yg, yd)
+        S.close()
+    print(f"[gpu] p={order} theta={theta}: rel. error vs direct sum FP64 {errs[0]:.3e} "
+          f"mixed {errs[1]:.3e}", flush=True)
+    bound = 1e-3 if order == 3 else 1e-5
+    assert errs[0] <= bound and errs[1] <= bound
+
+
+def test_config3_solves_true_residual_and_mixed_energy(big):
+    """FP64 and mixed-precision solves at tol 1e-4: the true preconditioned
+    residual ||M^-1 (b - A x)|| / ||M^-1 b|| of each meets the tolerance under
+    the FP64 operator, and the mixed energy is within 1e-3 of FP64's."""
+    P, g, S = big
+    b = S.rhs(P.qpos, P.q, P.usr.epsp)
+    x64, it64, err64, ierr64 = S.gmres(b, tol=1e-4)
+    assert ierr64 == 0
+    S32 = hip.Surface(P.params(precision=1), P.xyz, P.nrm, P.area)
+    x32, it32, err32, ierr32 = S32.gmres(b, tol=1e-4)
+    assert ierr32 == 0
+    mb = np.linalg.norm(S.msolve(b))
+    for x in (x64, x32):
+        res = np.linalg.norm(S.msolve(b - S.matvec(x))) / mb
+        assert res <= 1e-4 * 1.01, res
+    e64, _ = S.energy(P.qpos, P.q, P.usr.epsp, x64)
+    e32, _ = S32.energy(P.qpos, P.q, P.usr.epsp, x32)
+    print(f"[gpu] 483k solve: iter {it64} / {it32}, dG {e64:.6f} / {e32:.6f}", flush=True)
+    assert abs(e32 - e64) <= 1e-3 * abs(e64)
+    S32.close()
+
+
+# ---------------------------------------------------------------------------
+# config 2: 180,900 faces, the reference's complete solve
+# ---------------------------------------------------------------------------
+def test_config2_solve_matches_reference():
+    P, g = golden_problem("mol1500")
+    S = hip.Surface(P.params(), P.xyz, P.nrm, P.area)
+    assert S.n == int(g["n"])
+    check_order(S, g)
+    assert S.stats()["nleaf"] == len(g["leaf_beg"])
+    s = int(g["stride"])
+    x = cases.harness_x(2 * S.n)
+    assert rel_l2(S.matvec(x)[::s], g["y"]) <= 1e-10
+    assert rel_l2(S.msolve(x)[::s], g["z"]) <= 1e-10
+    b = S.rhs(P.qpos, P.q, P.usr.epsp)
+    assert rel_l2(b[::s], g["bvct"]) <= 1e-13
+    xs, it, err, ierr = S.gmres(b, tol=1e-4)
+    assert (it, ierr) == (int(g["iter"]), int(g["ierr"]))
+    assert err == pytest.approx(float(g["err"]), rel=1e-6)
+    assert rel_l2(xs[::s], g["xvct"]) <= 1e-9
+    sol, cou = S.energy(P.qpos, P.q, P.usr.epsp, xs)
+    assert sol == pytest.approx(float(g["soleng"]), rel=1e-10)
+    assert cou == pytest.approx(float(g["couleng"]), rel=1e-12)
+    vp = S.vertex_potential(P.vpos, P.faces, xs)
+    ok = ~np.isnan(g["vpot"][:, 0])
+    np.testing.assert_allclose(vp[::s][ok], g["vpot"][ok], rtol=0,
+                               atol=1e-9 * np.abs(g["vpot"][ok]).max() + 5.1e-11)
+    S.close()
+
+
+def test_config4a_64_charge_sets_as_one_block():
+    """64 charge sets on the 1,500-atom surface as one block, mixed precision
+    and FP64: set 0 is the fixture's charges (its iteration count and, in
+    FP64, its energy are the reference's); every set equals its single solve."""
+    P, g = golden_problem("mol1500")
+    nq = P.q.shape[0]
+    rng = np.random.default_rng(7)
+    Q = rng.normal(0.0, 0.35, (64, nq))
+    Q -= (Q.sum(1, keepdims=True) + 3.0) / nq
+    Q = Q.astype(np.float32).astype(np.float64)
+    Q[0] = P.q
+    for precision in (0, 1):
+        S = hip.Surface(P.params(precision=precision), P.xyz, P.nrm, P.area)
+        blk = S.solve_block(P.qpos, Q, P.usr.epsp, want_x=False)
+        assert int(blk["iter"][0]) == int(g["iter"])
+        if precision == 0:
+            assert blk["soleng"][0] == pytest.approx(float(g["soleng"]), rel=1e-10)
+        for k in range(64):
+            b = S.rhs(P.qpos, Q[k], P.usr.epsp)
+            x, it, err, ierr = S.gmres(b, tol=1e-4)
+            assert it == int(blk["iter"][k]) and ierr == 0
+            e, _ = S.energy(P.qpos, Q[k], P.usr.epsp, x)
+            assert e == pytest.approx(float(blk["soleng"][k]), rel=1e-10)
+        S.close()
