@@ -10,14 +10,15 @@ learned initial guess (features + MLP inference inside the timed solve); the
 reported value is the fastest form whose energy is within 1e-3 of the FP64
 solve's.
 
-Besides the JSON line's throughput, the run measures the dominant per-matvec
-kernel live with HIP events on the library's stream (tabi_ctx_kernel_times) and
-reports its algorithmic FLOP rate against the vector-ALU peak of the precision
-it computes in, times the CPU oracle (a line-for-line C restatement of the
-reference, bit-identical to it on the golden cases) on a bounded sample of the
-same workload on one host core, and measures the batched forms: 64 charge sets
-on one surface as a block, 48 molecules solved together, and the autotuner's
-pick of leaf size, Taylor order and task granularity.
+Besides the JSON line's throughput, the run times the configuration's other
+expansion (p = 6, theta = 0.5) in both precisions, measures the dominant
+per-matvec kernel live with HIP events on the library's stream
+(tabi_ctx_kernel_times) and reports its algorithmic FLOP rate against the
+vector-ALU peak of the precision it computes in, times the reference's own
+Fortran (compiled from its sources, oracle/_ref/ref_harness.exe) on a bounded
+sample of the same workload on one host core, and measures the batched forms:
+64 charge sets on one surface as a block, 48 molecules solved together, and
+the autotuner's pick of leaf size, Taylor order and task granularity.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -41,11 +42,19 @@ FP32_PEAK_TFLOPS = 157.3
 METRIC = "PB solves/hour at ~500k faces, ΔG err<1e-3 vs FP64; GMRES iters learned vs 0"
 
 
-# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE runs of `bench.py --kernels-only 5` on the default workload,
-# each counter scaled by what it reads for the in-place matvec's 7,737,152-byte
-# copy of x (FETCH_SIZE x 1.994, WRITE_SIZE x 1.000): {(kernel, faces): bytes}
-TRAFFIC = {("k_far32", 483572): 333.0e6 + 50.0e6, ("k_near32", 483572): 410.4e6 + 98.8e6}
+def traffic_from_profiles(kernel, faces):
+    """HBM bytes per launch of a kernel on a surface of `faces` faces, from the
+    committed rocprofv3 --pmc summaries (profiles/*pmc*.md): their lines
+    "traffic-bytes <kernel> <faces> <read B> <written B>" hold FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the calibration the summary states.
+    None when no summary covers this kernel and surface."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc*.md"))):
+        for ln in open(path):
+            t = ln.strip().strip("`").split()
+            if len(t) == 5 and t[0] == "traffic-bytes" and t[1] == kernel and int(t[2]) == faces:
+                return float(t[3]) + float(t[4])
+    return None
 
 
 def w_far(p):
@@ -67,7 +76,7 @@ def workload(name):
         sub = {"sphere20k": 5, "sphere82k": 6, "sphere328k": 7}[name]
         v, nv, f = meshio.icosphere(sub, 2.0)
         xyz, nrm, area = driver.face_geometry(v, nv, f)
-        prob = driver.Problem(usr, xyz, nrm, area, np.zeros((1, 3)), np.ones(1))
+        prob = driver.Problem(usr, xyz, nrm, area, np.zeros((1, 3)), np.ones(1), v, nv, f)
         desc = dict(workload=f"Born sphere a=2A q=+1, icosphere {len(f)} faces", faces=len(f),
                     atoms=1)
         return prob, desc
@@ -82,59 +91,82 @@ def workload(name):
         # charges and positions as the driver reads them from a PQR (real*4)
         qpos = xyz.astype(np.float32).astype(np.float64)
         qq = q.astype(np.float32).astype(np.float64)
-        prob = driver.Problem(usr, fxyz, nrm, area, qpos, qq)
+        prob = driver.Problem(usr, fxyz, nrm, area, qpos, qq, v, nv, f)
         desc = dict(workload=f"synthetic {natom}-atom molecule (seed 1), SES probe 1.4 A "
                              f"grid {h} A, {len(f)} faces", faces=len(f), atoms=natom)
         return prob, desc
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(prob, iters, budget_s=12.0):
-    """Times the oracle on one core on a bounded sample of the same problem and
-    extrapolates the serial reference time of one solve."""
-    import oracle as O
-    xyz, usr = prob.xyz, prob.usr
-    n = xyz.shape[0]
-    prm = O.params(prob.eps, prob.kappa, usr.mac, usr.order, usr.maxparnode)
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness.exe")
+
+
+def stage_reference(prob, d):
+    """The workload as a run directory of the reference program (usrdata.in,
+    test_proteins/<fname>.{pqr,vert,face}), from the problem's own mesh."""
+    usr = meshio.UsrData(**{**prob.usr.__dict__, "fname": "mol"})
+    pref = os.path.join(d, "surf")
+    meshio.write_msms(pref, prob.vpos, prob.vnrm, prob.faces)
+    meshio.write_pqr(os.path.join(d, "mol.pqr"), prob.qpos, prob.q, np.full(len(prob.q), 1.5))
+    meshio.stage_reference_run(d, usr, os.path.join(d, "mol.pqr"), pref)
+
+
+def reference_sample(prob, kstride=64, lstride=16):
+    """Runs the reference's own compiled routines (oracle/_ref/ref_harness.exe,
+    built from the reference's Fortran sources) on a bounded sample of the
+    workload, on one host core; returns the measured pieces."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        stage_reference(prob, d)
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec taskset -c 0 {REF_HARNESS} "
+                                          f"sample={kstride} lstride={lstride}"],
+                           cwd=d, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+    out = {}
+    for ln in r.stdout.splitlines():
+        t = ln.split()
+        if len(t) == 5 and t[0] == "harness-sample":
+            out[t[1]] = (float(t[2]), float(t[3]), float(t[4]))
+        elif len(t) == 4 and t[0] == "harness-sample":
+            out[t[1]] = (float(t[2]), float(t[3]))
+    return out
+
+
+def cpu_baseline(prob, iters):
+    """Serial time of one reference solve of the workload, composed from the
+    reference's own compiled routines timed on a bounded sample (one core):
+    geometry, right-hand side and tree as measured (the right-hand side scaled
+    from m of the charges), a matvec as the per-target traversal of every 64th
+    target scaled to all targets plus the lazy moments of the first pass, a
+    preconditioner application (LEAFMATVECpara refactors every leaf on every
+    call) as the block fill and dgetrf/dgetrs of every 16th leaf scaled by the
+    leaves' (2n)^2 and (2n)^3, and the energy from m charges; the solve is
+    iters matvecs and iters+2 preconditioner applications (src/dgmres_dep.f),
+    with iters the FP64 GPU solve's count (the same as the reference's on every
+    fixture)."""
     t0 = time.perf_counter()
-    txyz, order, nodes = O.build_tree(xyz, usr.maxparnode)
-    t_tree = time.perf_counter() - t0
-    tn, ta = prob.nrm[order], prob.area[order]
-    x = np.random.default_rng(0).standard_normal(2 * n)
-    rng = np.random.default_rng(1)
-    # matvec: targets sampled uniformly over the surface
-    k = max(64, min(n, 2000))
-    targets = np.sort(rng.choice(n, size=k, replace=False))
-    t0 = time.perf_counter()
-    O.treecode_matvec_targets(prm, txyz, tn, ta, nodes, x, targets)
-    t_mv_s = time.perf_counter() - t0
-    per_target = t_mv_s / k
-    # preconditioner: a sample of leaves, factor + solve as the reference does
-    ib, ie = O.leaves(nodes)
-    sel = rng.permutation(len(ib))
-    tl, done, rows = 0.0, 0, 0
-    for l in sel:
-        t0 = time.perf_counter()
-        O.precond(prm, txyz, tn, ta, ib[l:l + 1], ie[l:l + 1], x)
-        tl += time.perf_counter() - t0
-        done += 1
-        rows += int(ie[l] - ib[l] + 1)
-        if tl > budget_s / 2:
-            break
-    t_ms = tl / done * len(ib)
-    t_mv = per_target * n
-    nq = prob.q.shape[0]
-    # right-hand side and energy: O(n * nq) pair sums, timed on a row sample
-    t0 = time.perf_counter()
-    O.rhs(prob.xyz[:min(n, 4000)], prob.nrm[:min(n, 4000)], prob.qpos, prob.q, usr.epsp)
-    t_rhs = (time.perf_counter() - t0) * n / min(n, 4000)
-    t_solve = t_tree + t_rhs * 2.5 + iters * t_mv + (iters + 2) * t_ms
-    sample = (f"oracle (C restatement of the reference, gcc -O2, 1 thread): {k} of {n} "
-              f"matvec targets ({t_mv_s:.1f}s), {done} of {len(ib)} leaf factor+solves "
-              f"({tl:.1f}s), rhs rows {min(n, 4000)}; per-solve model "
-              f"iters*matvec+(iters+2)*msolve+tree+rhs+energy with the GPU's iters={iters}")
-    return dict(value=3600.0 / t_solve, unit="solves/hour", cores=1, kind="port",
-                sample=sample, t_matvec_s=t_mv, t_msolve_s=t_ms, t_solve_s=t_solve)
+    m = reference_sample(prob)
+    wall = time.perf_counter() - t0
+    per = lambda k: m[k][0] * m[k][2] / m[k][1]      # noqa: E731
+    t_first, t_second = m["matvec_first"][0], m["matvec"][0]
+    t_mv = m["pb_kernel"][0] + (t_first - t_second) + per("matvec")
+    t_ms = per("leaf_fill") + per("leaf_lu")
+    t_setup = m["form_geom"][0] + per("form_rhs") + m["tree"][0]
+    t_energy = per("energy")
+    t_solve = t_setup + iters * t_mv + (iters + 2) * t_ms + t_energy
+    lv = m["leaves"]
+    sample = (f"reference Fortran (flang -O2, oracle/_ref/ref_harness.exe sample mode) on one "
+              f"pinned core, {wall:.0f} s: every 64th matvec target "
+              f"({int(m['matvec'][1])} of {int(m['matvec'][2])}), every 16th leaf block factored "
+              f"and solved ({int(lv[0])} of {int(lv[1])}), right-hand side and energy from "
+              f"{int(m['form_rhs'][1])} / {int(m['energy'][1])} of {int(m['energy'][2])} charges, "
+              f"tree and geometry in full; solve = setup + {iters} matvecs + {iters + 2} "
+              f"preconditioner applications + energy")
+    return dict(value=3600.0 / t_solve, unit="solves/hour", cores=1, kind="reference",
+                sample=sample, t_matvec_s=t_mv, t_msolve_s=t_ms, t_setup_s=t_setup,
+                t_energy_s=t_energy, t_solve_s=t_solve)
 
 
 def charge_set_surface(natom=1500, nsets=64):
@@ -270,16 +302,17 @@ def main():
     # inputs resident in device memory before the timed region (tabi_solve_dev)
     dprob = hip.DeviceProblem(prob.xyz, prob.nrm, prob.area, prob.qpos, prob.q)
 
-    def time_solves(precision, model=None):
+    def time_solves(precision, model=None, steps=None, **over):
         # tabi_solve_dev returns after its last device work has completed
-        prm = prob.params(precision)
+        prm = prob.params(precision, **over)
+        steps = a.steps if steps is None else steps
         for _ in range(a.warmup):
             r = dprob.solve(prm, prob.usr.epsp, tol=1e-4, model=model)
         t0 = time.perf_counter()
-        for _ in range(a.steps):
+        for _ in range(steps):
             r = dprob.solve(prm, prob.usr.epsp, tol=1e-4, model=model)
-        dt = (time.perf_counter() - t0) / a.steps
-        print(f"[bench] precision={precision} learned={model is not None} solve "
+        dt = (time.perf_counter() - t0) / steps
+        print(f"[bench] precision={precision} learned={model is not None} {over} solve "
               f"{dt * 1e3:.1f} ms iter={r['iter']} ierr={r['ierr']} dG={r['soleng']:.6f} "
               f"phases={ {k: round(float(v), 2) for k, v in r['times_ms'].items()} }",
               file=sys.stderr, flush=True)
@@ -289,6 +322,9 @@ def main():
     dt32, res32 = time_solves(1)
     model = hip.WsModel(warmstart.load()) if os.path.exists(warmstart.DEFAULT_MODEL) else None
     dtws, resws = time_solves(1, model) if model is not None else (None, None)
+    # the configuration's other expansion: p = 6, theta = 0.5
+    p6 = {prec_: time_solves(prec_, steps=max(1, a.steps // 2), order=6, theta=0.5)
+          for prec_ in (0, 1)}
     forms = {"fp64": (dt64, res64, 0), "mixed": (dt32, res32, 1)}
     if model is not None:
         forms["mixed+learned"] = (dtws, resws, 1)
@@ -343,6 +379,7 @@ def main():
     avg = {k: kt[k][0] / max(kt[k][1], 1) for k in ("near", "far", "moments", "reduce")}
     dom = max(("near", "far"), key=lambda k: avg[k])
     achieved = flops[dom] / (avg[dom] * 1e-3) / 1e12
+    kname = "k_" + dom + ("32" if prec == 1 else "") + ("<%d>" % p if dom == "far" else "")
     peak = FP32_PEAK_TFLOPS if prec == 1 else FP64_PEAK_TFLOPS
     print(f"[bench] matvec {mv_ms:.3f} ms (fp64 {mv64_ms:.3f} ms); kernel avg ms {avg}; "
           f"msolve {msolve_ms:.3f} ms; stats {st}", file=sys.stderr, flush=True)
@@ -385,19 +422,23 @@ def main():
                        mixed_learned_solves_per_hour=None if dtws is None else 3600.0 / dtws,
                        mixed_learned_ms_per_solve=None if dtws is None else dtws * 1e3,
                        pcie_inclusive_ms_per_solve=dt_host * 1e3,
+                       p6_theta05={k: dict(ms_per_solve=v[0] * 1e3, solves_per_hour=3600.0 / v[0],
+                                           gmres_iter=v[1]["iter"], soleng_kcal=v[1]["soleng"],
+                                           dG_rel_diff_vs_p3_fp64=abs(v[1]["soleng"] - res64["soleng"])
+                                           / abs(res64["soleng"]))
+                                   for k, v in (("fp64", p6[0]), ("mixed", p6[1]))},
                        matvec_ms=mv_ms, fp64_matvec_ms=mv64_ms, msolve_ms=msolve_ms,
                        matvecs_per_s=1e3 / mv_ms,
                        phases_ms={k: float(v) for k, v in res["times_ms"].items()},
                        batched_charge_sets=batch, many_surfaces=packed, autotune=tuned),
-        # compute-bound: priced against the published FP32 (or FP64) peak; the
-        # kernel runs on the vector ALUs without packed FP32, whose issue rate
-        # is half the FP32 figure
-        "roofline": {"bound": "mfma", "compute_unit": "VALU",
-                     "kernel": ("k_" + dom + ("32" if prec == 1 else "")),
+        # compute-bound on the vector ALUs (exp/rsqrt pair sums, no matrix
+        # cores): priced against the published FP32 (or FP64) vector peak; the
+        # kernels are built without packed FP32, whose issue rate is half the
+        # FP32 figure
+        "roofline": {"bound": "valu", "kernel": kname,
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": TRAFFIC.get(("k_" + dom + ("32" if prec == 1 else ""),
-                                             int(prob.xyz.shape[0]))),
+                     "traffic": traffic_from_profiles(kname, int(prob.xyz.shape[0])),
                      "alg_flops_per_launch": flops[dom], "avg_launch_ms": avg[dom]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},

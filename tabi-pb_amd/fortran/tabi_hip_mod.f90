@@ -11,6 +11,7 @@ module tabi_hip
   private
   public :: tabi_params_t, tabi_hip_setup, tabi_hip_release, tabi_hip_check
   public :: c_tabi_init, c_tabi_matvec, c_tabi_msolve, c_tabi_finalize
+  public :: c_tabi_rhs, c_tabi_energy, c_tabi_vertex_potential
 
   type, bind(C) :: tabi_params_t
     real(c_double) :: eps, kappa, theta
@@ -44,6 +45,35 @@ module tabi_hip
 
     subroutine c_tabi_finalize() bind(C, name='tabi_finalize')
     end subroutine c_tabi_finalize
+
+    ! right-hand side in tree order (form_matrix's charge loop, src/main.f90:597-607)
+    integer(c_int) function c_tabi_rhs(nq, qpos, q, eps_p, b) bind(C, name='tabi_rhs')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: nq
+      real(c_double), intent(in) :: qpos(*), q(*)
+      real(c_double), value :: eps_p
+      real(c_double), intent(out) :: b(*)
+    end function c_tabi_rhs
+
+    ! solvation and Coulomb energies in kcal/mol (src/main.f90:144-157)
+    integer(c_int) function c_tabi_energy(nq, qpos, q, eps_p, x, soleng, couleng) &
+        bind(C, name='tabi_energy')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: nq
+      real(c_double), intent(in) :: qpos(*), q(*), x(*)
+      real(c_double), value :: eps_p
+      real(c_double), intent(out) :: soleng, couleng
+    end function c_tabi_energy
+
+    ! vertex-averaged potentials (output_potential, src/main.f90:226-326)
+    integer(c_int) function c_tabi_vertex_potential(nvert, vpos, faces, x, vpot) &
+        bind(C, name='tabi_vertex_potential')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: nvert
+      real(c_double), intent(in) :: vpos(*), x(*)
+      integer(c_int32_t), intent(in) :: faces(*)
+      real(c_double), intent(out) :: vpot(*)
+    end function c_tabi_vertex_potential
 
     type(c_ptr) function c_tabi_last_error() bind(C, name='tabi_last_error')
       import :: c_ptr
