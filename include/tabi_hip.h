@@ -137,6 +137,18 @@ int tabi_ctx_gmres(tabi_ctx *c, const double *b, double *x, double tol, int32_t 
 int tabi_ctx_vertex_potential(tabi_ctx *c, int32_t nvert, const double *vpos,
                               const int32_t *faces, const double *x, double *vpot);
 
+/* tabi_ctx_energy without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * charges d_qpos[3*nq], d_q[nq] and the tree-order solution d_x[2n] are device
+ * pointers; the per-call scratch (energy partials, per-charge potentials,
+ * Coulomb partials: about 8*(nq/64*n/32 + nq + (nq/256)^2) bytes) lives in a
+ * static device array of 32 MiB held by every GPU the library runs on; larger
+ * problems are rejected with an error (use tabi_ctx_energy).  Calls that
+ * overlap on one GPU share that array and race.  Returns after completion. */
+int tabi_ctx_energy_noalloc(tabi_ctx *c, int32_t nq, const double *d_qpos, const double *d_q,
+                            double eps_p, const double *d_x, double *soleng, double *couleng);
+
 /* ---- blocks of right-hand sides on one surface ------------------------------
  * Several vectors (charge sets, Krylov vectors) share one tree pass: the near-
  * field kernel values and far-field Taylor expansions are formed once per pair

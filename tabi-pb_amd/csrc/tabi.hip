@@ -4656,34 +4656,84 @@ double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *q,
     return scale;
 }
 
+// Scratch of the energy evaluation for nq charges on n faces: per (charge
+// group, face chunk) partials, per-charge potentials, Coulomb partials.
+struct EnergyScratch {
+    int ngq, nch, ncb, ncp;
+    int64_t part, ptl, cpart, total;   // offsets and size in doubles
+    EnergyScratch(int n, int nq)
+    {
+        ngq = (nq + kWave - 1) / kWave;
+        nch = (n + kEnergyChunk - 1) / kEnergyChunk;
+        ncb = (nq + 255) / 256;
+        ncp = ncb * ncb;
+        part = 0;
+        ptl = (int64_t)ngq * nch * kWave;
+        cpart = ptl + nq;
+        total = cpart + ncp + 1;
+    }
+};
+
+// Solvation and Coulomb energies (src/main.f90:144-157) from device inputs,
+// with the scratch at w (EnergyScratch layout); returns after completion.
+void energy_core(tabi_ctx *c, int nq, const double *dqp, const double *dq, double eps_p,
+                 const double *dxs, double *w, double *sol, double *cou)
+{
+    const double para = 332.0716;  // src/main.f90:64
+    const EnergyScratch L(c->n, nq);
+    double *part = w + L.part, *dptl = w + L.ptl, *cpart = w + L.cpart;
+    k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, dxs, c->src8.p);
+    k_energy<<<L.ngq * L.nch, 64, 0, c->s>>>(c->n, nq, L.nch, dqp, c->src8.p, c->prm.eps,
+                                             c->prm.kappa, part);
+    k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, L.nch, part, dptl);
+    k_coulomb<<<dim3(L.ncb, L.ncb), 256, 0, c->s>>>(nq, dqp, dq, cpart);
+    k_sum_part<<<1, 256, 0, c->s>>>(L.ncp, cpart, cpart + L.ncp);
+    HIPCHK(hipGetLastError());
+    // sum_k q_k * potential_k (src/main.f90:148,156)
+    const double sdot = dev_dot(c, nq, dq, dptl);
+    double cc = 0.0;
+    HIPCHK(hipMemcpyAsync(&cc, cpart + L.ncp, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    *sol = sdot * 0.5 * para * 4.0 * kPi;
+    *cou = cc / eps_p * para;
+}
+
 void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p,
                 const double *dxs, double *sol, double *cou, bool dev = false)
 {
     KTimer kt(c, 7);
-    const double para = 332.0716;  // src/main.f90:64
     if (nq <= 0) { *sol = 0.0; *cou = 0.0; return; }
     DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
-    DBuf<double> dptl, part, cpart;
-    dptl.alloc(nq);
-    const int ngq = (nq + kWave - 1) / kWave;
-    const int nch = (c->n + kEnergyChunk - 1) / kEnergyChunk;
-    part.alloc((size_t)ngq * nch * kWave);
-    k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, dxs, c->src8.p);
-    k_energy<<<ngq * nch, 64, 0, c->s>>>(c->n, nq, nch, dqp.p, c->src8.p, c->prm.eps,
-                                         c->prm.kappa, part.p);
-    k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, nch, part.p, dptl.p);
-    const int ncb = (nq + 255) / 256, ncp = ncb * ncb;
-    cpart.alloc(ncp + 1);
-    k_coulomb<<<dim3(ncb, ncb), 256, 0, c->s>>>(nq, dqp.p, dq.p, cpart.p);
-    k_sum_part<<<1, 256, 0, c->s>>>(ncp, cpart.p, cpart.p + ncp);
-    HIPCHK(hipGetLastError());
-    // sum_k q_k * potential_k (src/main.f90:148,156)
-    const double s = dev_dot(c, nq, dq.p, dptl.p);
-    double cc = 0.0;
-    HIPCHK(hipMemcpyAsync(&cc, cpart.p + ncp, sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    *sol = s * 0.5 * para * 4.0 * kPi;
-    *cou = cc / eps_p * para;
+    DBuf<double> w;
+    w.alloc((size_t)EnergyScratch(c->n, nq).total);
+    energy_core(c, nq, dqp.p, dq.p, eps_p, dxs, w.p, sol, cou);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free energy (tabi_ctx_energy_noalloc): the scratch lives in a
+// file-scope device array of fixed size, one copy per GPU, whose address is
+// looked up once per device.  Host code uses only that address (the array's
+// name on the host is not a device pointer).  Every call on a GPU shares the
+// array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchDoubles = (int64_t)4 << 20;     // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+
+double *scratch_ptr()
+{
+    static std::mutex mu;
+    static double *ptrs[kMaxDevices] = {};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error("device index out of range");
+    std::lock_guard<std::mutex> g(mu);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptrs[dev] = static_cast<double *>(p);
+    }
+    return ptrs[dev];
 }
 
 // ---------------------------------------------------------------------------
@@ -5317,6 +5367,25 @@ int tabi_write_surface_potential(const char *path, int32_t nvert, const double *
         for (int64_t j = 0; ok && j < nface; ++j)
             ok = std::fprintf(f, " %d %d %d\n", faces[3 * j], faces[3 * j + 1], faces[3 * j + 2]) > 0;
         if (std::fclose(f) != 0 || !ok) throw Error(std::string("write failed: ") + path);
+    });
+}
+
+int tabi_ctx_energy_noalloc(tabi_ctx *c, int32_t nq, const double *d_qpos, const double *d_q,
+                            double eps_p, const double *d_x, double *soleng, double *couleng)
+{
+    return guarded([&] {
+        if (!c) throw Error("null context");
+        if (!soleng || !couleng) throw Error("null output");
+        if (nq <= 0) { *soleng = 0.0; *couleng = 0.0; return; }
+        if (!d_qpos || !d_q || !d_x) throw Error("null device array");
+        const EnergyScratch L(c->n, nq);
+        if (L.total > kScratchDoubles)
+            throw Error("tabi_ctx_energy_noalloc: " + std::to_string(nq) + " charges on " +
+                        std::to_string(c->n) + " faces need " + std::to_string(L.total * 8) +
+                        " bytes of scratch, more than the static " +
+                        std::to_string(kScratchDoubles * 8) + " (use tabi_ctx_energy)");
+        double *w = scratch_ptr();
+        energy_core(c, nq, d_qpos, d_q, eps_p, d_x, w, soleng, couleng);
     });
 }
 

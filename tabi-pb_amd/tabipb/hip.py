@@ -27,7 +27,7 @@ EXPORTS = (
     "tabi_ctx_gmres_ws", "tabi_solve_ws", "tabi_ctx_matvec_block", "tabi_ctx_msolve_block",
     "tabi_ctx_gmres_block", "tabi_ctx_solve_block", "tabi_solve_many",
     "tabi_last_solve_many_stats", "tabi_solve_dev", "tabi_vertex_potential",
-    "tabi_ctx_vertex_potential", "tabi_write_surface_potential",
+    "tabi_ctx_vertex_potential", "tabi_write_surface_potential", "tabi_ctx_energy_noalloc",
 )
 
 # tabi_params.flags bits
@@ -244,6 +244,15 @@ class Surface:
         cc = C.c_double()
         _check(load().tabi_ctx_energy(self._h, C.c_int32(q.shape[0]), _p(qpos), _p(q),
                                       C.c_double(eps_p), _p(x), C.byref(s), C.byref(cc)))
+        return s.value, cc.value
+
+    def energy_noalloc(self, d_qpos: int, d_q: int, nq: int, eps_p, d_x: int):
+        """tabi_ctx_energy_noalloc on device arrays (DeviceArray.ptr values)."""
+        s = C.c_double()
+        cc = C.c_double()
+        _check(load().tabi_ctx_energy_noalloc(self._h, C.c_int32(nq), C.c_void_p(d_qpos),
+                                              C.c_void_p(d_q), C.c_double(eps_p),
+                                              C.c_void_p(d_x), C.byref(s), C.byref(cc)))
         return s.value, cc.value
 
     def vertex_potential(self, vpos, faces, x):
