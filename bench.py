@@ -258,7 +258,8 @@ def autotuned(prob):
     default = [t for t in r["table"] if t["leaf"] == 500 and t["order"] == 3 and t["task_exp"] == 0][0]
     return dict(best=r["best"], default=default, speedup_vs_default=default["ms"] / r["best"]["ms"],
                 candidates=len(r["table"]), rejected=sum(not t["ok"] for t in r["table"]),
-                err_target=r["err_target"])
+                err_target=r["err_target"], reference=r["reference_form"],
+                own_params_fp64_rel_err=r["own_params_rel_err"])
 
 
 def main():

@@ -294,6 +294,7 @@ struct tabi_ctx {
     DBuf<int32_t> blk_m, blk_ibeg, piv_off, piv, perm, leaf_of, lu_info, lorder, lu_swrec;
     DBuf<double> lu_linv;
     DBuf<double> lu;
+    DBuf<float> lu32;                    // mixed precision: the factors stored in FP32
     // vectors
     DBuf<double> vx, vy, red;
     // blocks of vectors
@@ -1242,6 +1243,16 @@ __device__ __forceinline__ int lu_swap_record(int lane, int k0, int nbe, int pr,
 // first maximum, the waves' results combined by every thread), a row exchange
 // through LDS and the rank-1 update; then stored.  Out of line, so the common
 // pass keeps its registers.
+// The double held by lane `lane` (a compile-time or wave-uniform index) of
+// the wave, read into every lane through two 32-bit lane reads.
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int RPT, int kPanelThreads>
 __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nbe,
     double *__restrict__ A, int32_t *__restrict__ pv, int32_t *__restrict__ info_l,
@@ -1423,42 +1434,66 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
         for (int c = 0; c < kLuNB; ++c)
             p[j][c] = (row < m && c < nbe) ? A[(size_t)(k0 + c) * m + row] : 0.0;
     }
-    // Optimistic pass: each column's diagonal entry as its pivot, one barrier
-    // per column.  A thread notes when one of its rows below the diagonal is
-    // strictly larger in magnitude (IDAMAX takes the first maximum, so a tie
-    // keeps the diagonal) or the pivot is zero.  If no thread did, partial
-    // pivoting would have chosen the diagonal in every column and the factors
-    // are exactly those of the pivoted pass below (the same operations);
-    // otherwise the panel is reloaded and factored with partial pivoting.
+    // Optimistic pass: each column's diagonal entry as its pivot.  A thread
+    // notes when one of its rows below the diagonal is strictly larger in
+    // magnitude (IDAMAX takes the first maximum, so a tie keeps the diagonal)
+    // or the pivot is zero.  If no thread did, partial pivoting would have
+    // chosen the diagonal in every column and the factors are exactly those of
+    // the pivoted pass below (the same operations); otherwise the panel is
+    // reloaded and factored with partial pivoting.
+    // Wave 0 holds the panel's diagonal block (rows k0..k0+31 are its lanes
+    // 0..31) and factors its 64 rows column by column, the pivot row taken from
+    // lane c with cross-lane reads, without barriers; it then leaves the
+    // finished pivot rows (U11) and their reciprocals in LDS, and after one
+    // barrier every other row applies them in the same order (row-wise
+    // substitution: the same multiplications and fused updates as the
+    // column-by-column elimination, so the factors are bit-identical to it).
     int need = 0;
-#pragma clang loop unroll(full)
-    for (int c = 0; c < kLuNB; ++c) {
-        if (c < nbe) {
-        const int col = k0 + c, pb = c & 1;
+    __shared__ double s_u[kLuNB][kLuNB + 1];
+    __shared__ double s_rinv[kLuNB];
+    if (w == 0) {
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int row = k0 + t + j * kPanelThreads;
-            if (row == col) {
+        for (int c = 0; c < kLuNB; ++c) {
+            if (c < nbe) {
+                const int col = k0 + c;
+                const double pivot = readlane_d(p[0][c], c);
+                const double ap = fabs(pivot), rinv = 1.0 / pivot;
+                need |= pivot == 0.0;
+                if (lane == 0) s_rinv[c] = rinv;
+                double u[kLuNB];
 #pragma unroll
-                for (int cc = 0; cc < kLuNB; ++cc) s_row[pb][0][cc] = p[j][cc];
+                for (int cc = c + 1; cc < kLuNB; ++cc) u[cc] = readlane_d(p[0][cc], c);
+                const int row = k0 + t;
+                if (row > col && row < m) {
+                    need |= fabs(p[0][c]) > ap;
+                    p[0][c] *= rinv;
+                    const double lv = p[0][c];
+#pragma unroll
+                    for (int cc = c + 1; cc < kLuNB; ++cc) p[0][cc] = fma(-lv, u[cc], p[0][cc]);
+                }
             }
         }
-        __syncthreads();
-        const double pivot = s_row[pb][0][c];
-        const double ap = fabs(pivot), rinv = 1.0 / pivot;
-        need |= pivot == 0.0;
+        if (t < nbe) {
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int row = k0 + t + j * kPanelThreads;
-            if (row > col && row < m) {
-                need |= fabs(p[j][c]) > ap;
-                p[j][c] *= rinv;
-                const double lv = p[j][c];
-#pragma unroll
-                for (int cc = c + 1; cc < kLuNB; ++cc)
-                    p[j][cc] = fma(-lv, s_row[pb][0][cc], p[j][cc]);
-            }
+            for (int cc = 0; cc < kLuNB; ++cc) s_u[t][cc] = p[0][cc];
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int row = k0 + t + j * kPanelThreads;
+        if ((j > 0 || w > 0) && row < m) {          // every such row is below the panel
+#pragma unroll
+            for (int c = 0; c < kLuNB; ++c) {
+                if (c < nbe) {
+                    need |= fabs(p[j][c]) > fabs(s_u[c][c]);
+                    p[j][c] *= s_rinv[c];
+                    const double lv = p[j][c];
+#pragma unroll
+                    for (int cc = c + 1; cc < kLuNB; ++cc)
+                        p[j][cc] = fma(-lv, s_u[c][cc], p[j][cc]);
+                }
+            }
         }
     }
     const bool fast = !__syncthreads_or(need);
@@ -1901,7 +1936,8 @@ constexpr int kSolveB = 64;
 constexpr int kHalf = 32;
 
 // s - sum_{c < bs} A[c*m] * zt[c] for one row, loads issued 32 at a time
-__device__ __forceinline__ double gemv_row(const double *__restrict__ Ac, int m, int bs,
+template <typename FT>
+__device__ __forceinline__ double gemv_row(const FT *__restrict__ Ac, int m, int bs,
                                            const double *zt, double s)
 {
     double s1 = 0.0;
@@ -1909,7 +1945,7 @@ __device__ __forceinline__ double gemv_row(const double *__restrict__ Ac, int m,
     for (int h = 0; h < kSolveB; h += kHalf) {
         double a[kHalf];
 #pragma unroll
-        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? Ac[(size_t)(h + c) * m] : 0.0;
+        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? (double)Ac[(size_t)(h + c) * m] : 0.0;
 #pragma unroll
         for (int c = 0; c < kHalf; c += 2) {
             s = fma(-a[c], (h + c < bs) ? zt[h + c] : 0.0, s);
@@ -1924,17 +1960,21 @@ __device__ __forceinline__ double gemv_row(const double *__restrict__ Ac, int m,
 // waves, so a large leaf is streamed NW times faster than by one wave.  Leaves
 // are visited in the order lorder (largest first) so the longest solves start
 // first.
-template <int NW>
+//
+// FT is the stored factor type: double, or float for the mixed-precision form
+// (factors computed in FP64 and rounded once for storage; the substitutions
+// accumulate in FP64), which halves the stream.
+template <int NW, typename FT>
 __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__restrict__ lorder,
     const int32_t *__restrict__ blk_ibeg, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
-    const double *__restrict__ lu, const int32_t *__restrict__ perm, const double *__restrict__ r,
+    const FT *__restrict__ lu, const int32_t *__restrict__ perm, const double *__restrict__ r,
     double *__restrict__ z)
 {
     extern __shared__ double zs[];
     const int l = lorder[blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
-    const double *A = lu + blk_off[l];
+    const FT *A = lu + blk_off[l];
     const int32_t *pm = perm + piv_off[l];
     for (int k = t; k < m; k += 64 * NW) {
         const int src = pm[k];
@@ -1952,7 +1992,7 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
                 double tr[kHalf];
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c)
-                    tr[c] = (in && h + c < lane) ? A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                    tr[c] = (in && h + c < lane) ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c) zl -= tr[c] * __shfl(zl, h + c);
             }
@@ -1975,8 +2015,8 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
                 double tr[kHalf];
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c)
-                    tr[c] = (in && h + c >= lane && h + c < bs) ? A[(size_t)(kb + h + c) * m + kb + lane]
-                                                               : 0.0;
+                    tr[c] = (in && h + c >= lane && h + c < bs)
+                                ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
 #pragma unroll
                 for (int c = kHalf - 1; c >= 0; --c) {
                     if (lane == h + c && in) zl = zl / tr[c];
@@ -2497,8 +2537,8 @@ __global__ void k_reduceb(int n, int R, NearSlots ns, const int32_t *__restrict_
 
 // z = M^-1 r for RC block columns per workgroup (blockIdx.y): the factor
 // stream is read once and applied to every column; z lives in LDS [RC][m]
-template <int RC>
-__device__ __forceinline__ void gemv_rowb(const double *__restrict__ Ac, int m, int bs,
+template <int RC, typename FT>
+__device__ __forceinline__ void gemv_rowb(const FT *__restrict__ Ac, int m, int bs,
                                           const double *zt, int ldz, double (&s)[RC])
 {
     double s1[RC];
@@ -2508,7 +2548,7 @@ __device__ __forceinline__ void gemv_rowb(const double *__restrict__ Ac, int m, 
     for (int h = 0; h < kSolveB; h += kHalf) {
         double a[kHalf];
 #pragma unroll
-        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? Ac[(size_t)(h + c) * m] : 0.0;
+        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? (double)Ac[(size_t)(h + c) * m] : 0.0;
 #pragma unroll
         for (int k = 0; k < RC; ++k) {
 #pragma unroll
@@ -2522,10 +2562,10 @@ __device__ __forceinline__ void gemv_rowb(const double *__restrict__ Ac, int m, 
     for (int k = 0; k < RC; ++k) s[k] = s[k] + s1[k];
 }
 
-template <int RC>
+template <int RC, typename FT>
 __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *__restrict__ blk_ibeg,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    const int32_t *__restrict__ piv_off, const double *__restrict__ lu,
+    const int32_t *__restrict__ piv_off, const FT *__restrict__ lu,
     const int32_t *__restrict__ perm, const double *__restrict__ rv, int64_t rs_,
     double *__restrict__ zv, int64_t zs_, const int32_t *__restrict__ act, int ldz)
 {
@@ -2533,7 +2573,7 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
     const int l = blockIdx.x, lane = threadIdx.x;
     const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
-    const double *A = lu + blk_off[l];
+    const FT *A = lu + blk_off[l];
     const int32_t *pm = perm + piv_off[l];
     for (int k = 0; k < RC; ++k) {
         const double *r = k < rc ? rv + (size_t)act[r0 + k] * rs_ : nullptr;
@@ -2555,7 +2595,7 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
             double tr[kHalf];
 #pragma unroll
             for (int c = 0; c < kHalf; ++c)
-                tr[c] = (in && h + c < lane) ? A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                tr[c] = (in && h + c < lane) ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
 #pragma unroll
             for (int k = 0; k < RC; ++k)
 #pragma unroll
@@ -2572,7 +2612,7 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
                 double sv[RC];
 #pragma unroll
                 for (int k = 0; k < RC; ++k) sv[k] = zsh[k * ldz + row];
-                gemv_rowb<RC>(A + (size_t)kb * m + row, m, bs, zsh + kb, ldz, sv);
+                gemv_rowb<RC, FT>(A + (size_t)kb * m + row, m, bs, zsh + kb, ldz, sv);
 #pragma unroll
                 for (int k = 0; k < RC; ++k) zsh[k * ldz + row] = sv[k];
             }
@@ -2590,8 +2630,8 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
             double tr[kHalf];
 #pragma unroll
             for (int c = 0; c < kHalf; ++c)
-                tr[c] = (in && h + c >= lane && h + c < bs) ? A[(size_t)(kb + h + c) * m + kb + lane]
-                                                           : 0.0;
+                tr[c] = (in && h + c >= lane && h + c < bs)
+                            ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
 #pragma unroll
             for (int c = kHalf - 1; c >= 0; --c) {
 #pragma unroll
@@ -2611,7 +2651,7 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
             double sv[RC];
 #pragma unroll
             for (int k = 0; k < RC; ++k) sv[k] = zsh[k * ldz + q0 + lane];
-            gemv_rowb<RC>(A + (size_t)kb * m + q0 + lane, m, bs, zsh + kb, ldz, sv);
+            gemv_rowb<RC, FT>(A + (size_t)kb * m + q0 + lane, m, bs, zsh + kb, ldz, sv);
 #pragma unroll
             for (int k = 0; k < RC; ++k) zsh[k * ldz + q0 + lane] = sv[k];
         }
@@ -3088,8 +3128,12 @@ void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
     }
     const size_t shm = sizeof(double) * (size_t)c->mmax;
     KTimer kt(c, 4);
-    k_lu_solve<4><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
-        c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
+    if (c->lu32.p)
+        k_lu_solve<4, float><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
+            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu32.p, c->perm.p, dr, dz);
+    else
+        k_lu_solve<4, double><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
+            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
     HIPCHK(hipGetLastError());
 }
 
@@ -3209,8 +3253,13 @@ void launch_lu_solveb(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *
                       const int32_t *d_act, int ldz)
 {
     const size_t shm = sizeof(double) * (size_t)RC * ldz;
-    k_lu_solveb<RC><<<dim3(c->nleaf, (R + RC - 1) / RC), kWave, shm, c->s>>>(c->n, R, c->blk_ibeg.p,
-        c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, Rv, rs, Z, zs, d_act, ldz);
+    const dim3 grid(c->nleaf, (R + RC - 1) / RC);
+    if (c->lu32.p)
+        k_lu_solveb<RC, float><<<grid, kWave, shm, c->s>>>(c->n, R, c->blk_ibeg.p, c->blk_m.p,
+            c->blk_off.p, c->piv_off.p, c->lu32.p, c->perm.p, Rv, rs, Z, zs, d_act, ldz);
+    else
+        k_lu_solveb<RC, double><<<grid, kWave, shm, c->s>>>(c->n, R, c->blk_ibeg.p, c->blk_m.p,
+            c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, Rv, rs, Z, zs, d_act, ldz);
 }
 
 void msolve_block_dev(tabi_ctx *c, int R, const double *Rv, int64_t rs, double *Z, int64_t zs,
@@ -3743,6 +3792,15 @@ void build_precond(tabi_ctx *c)
     for (int l = 0; l < nl; ++l)
         if (info[l]) throw Error("singular leaf block " + std::to_string(l + 1) + " (column " +
                                  std::to_string(info[l]) + ")");
+    if (c->prm.precision == 1) {
+        // mixed precision: the FP64 factors are rounded once to FP32 for the
+        // solves, which then stream half the bytes
+        c->lu32.alloc(off);
+        k_to_float<<<nblk(off, 256), 256, 0, c->s>>>(off, c->lu.p, c->lu32.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->s));
+        c->lu.release();
+    }
 }
 
 // Centroids from interleaved {x,y,z} to three arrays.
@@ -3867,7 +3925,7 @@ void ctx_stats(tabi_ctx *c, int64_t *o)
     o[6] = c->n_far_items;
     o[7] = c->n_near_items;
     o[8] = (int64_t)c->n_near_task + c->n_far_task;
-    o[9] = c->lu.n * (int64_t)sizeof(double);
+    o[9] = c->lu.n * (int64_t)sizeof(double) + c->lu32.n * (int64_t)sizeof(float);
 }
 
 

@@ -358,6 +358,80 @@ def clean_mesh(v, faces, h, min_area=1e-4):
     return v[used], remap[faces]
 
 
+def _half_edges(faces):
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    fid = np.tile(np.arange(len(faces)), 3)
+    return e, fid
+
+
+def repair_manifold(v, faces, rounds=8):
+    """Make the mesh a closed, consistently oriented 2-manifold.  Edge collapses
+    can pinch the surface (an edge shared by three faces, a directed edge used
+    twice); the faces on such edges are removed, vertices where the remaining
+    surface meets itself lose their faces too, and every hole left is closed
+    by a fan around a new vertex at its boundary's centroid, oriented with the
+    surrounding faces.  Faces away from defects are untouched."""
+    v = np.asarray(v, dtype=np.float64)
+    faces = np.asarray(faces, dtype=np.int64)
+    for _ in range(rounds):
+        nv = len(v)
+        e, fid = _half_edges(faces)
+        key = e[:, 0] * nv + e[:, 1]
+        rkey = e[:, 1] * nv + e[:, 0]
+        uk, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+        rep = cnt[inv] > 1
+        has_twin = np.isin(rkey, uk)
+        ukey = np.minimum(key, rkey)
+        _, uinv, ucnt = np.unique(ukey, return_inverse=True, return_counts=True)
+        bad_edge = rep | ~has_twin | (ucnt[uinv] != 2)
+        if not np.any(bad_edge):
+            used = np.unique(faces)
+            remap = -np.ones(nv, dtype=np.int64)
+            remap[used] = np.arange(len(used))
+            return v[used], remap[faces]
+        bad_f = np.zeros(len(faces), dtype=bool)
+        bad_f[fid[rep | (ucnt[uinv] > 2)]] = True
+        f = faces[~bad_f]
+        # boundary half-edges; a vertex with more than one outgoing boundary
+        # half-edge joins two holes (non-manifold): drop its faces and retry
+        e, fid = _half_edges(f)
+        key = e[:, 0] * nv + e[:, 1]
+        bnd = ~np.isin(e[:, 1] * nv + e[:, 0], key)
+        out_deg = np.bincount(e[bnd, 0], minlength=nv)
+        pinched = out_deg > 1
+        if np.any(pinched):
+            f = f[~pinched[f].any(axis=1)]
+            faces = f
+            continue
+        nxt = -np.ones(nv, dtype=np.int64)
+        nxt[e[bnd, 0]] = e[bnd, 1]
+        seen = np.zeros(nv, dtype=bool)
+        newv, newf = [], []
+        for start in e[bnd, 0]:
+            if seen[start]:
+                continue
+            loop = [start]
+            seen[start] = True
+            cur = nxt[start]
+            while cur != start and cur >= 0 and not seen[cur]:
+                loop.append(cur)
+                seen[cur] = True
+                cur = nxt[cur]
+            if cur != start:
+                continue                      # not a closed loop: next round
+            if len(loop) == 3:
+                newf.append((loop[0], loop[2], loop[1]))
+                continue
+            c = len(v) + len(newv)
+            newv.append(v[loop].mean(axis=0))
+            for a, b in zip(loop, loop[1:] + loop[:1]):
+                newf.append((b, a, c))
+        if newv:
+            v = np.vstack([v, np.array(newv)])
+        faces = np.vstack([f, np.array(newf, dtype=np.int64).reshape(-1, 3)])
+    raise RuntimeError("mesh repair did not converge")
+
+
 def vertex_normals(verts, faces):
     fn = np.cross(verts[faces[:, 1]] - verts[faces[:, 0]], verts[faces[:, 2]] - verts[faces[:, 0]])
     vn = np.zeros_like(verts)
@@ -394,6 +468,7 @@ def ses_mesh(xyz, radius, probe=1.4, h=0.5, cavities=False):
     f, lo, h = ses_field(xyz, radius, probe, h)
     v, fc = marching_tets(f, lo, h)
     v, fc = clean_mesh(v, fc, h)
+    v, fc = repair_manifold(v, fc)
     if not cavities:
         v, fc = outer_component(v, fc)
     return v, vertex_normals(v, fc), fc

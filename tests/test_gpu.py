@@ -351,7 +351,8 @@ def test_leaf_factorisation_is_deterministic():
 
 # ---------------------------------------------------------------------------
 # mixed precision (precision=1): FP32 near/far field with compensated sums,
-# FP64 GMRES, leaf LU, self term and energy.  Tolerances: matvec rel. L2 <= 1e-5
+# FP64 GMRES, leaf LU factorisation (factors stored in FP32), self term and
+# energy.  Tolerances: matvec rel. L2 <= 1e-5
 # against the FP64 reference output, solvation energy rel. <= 1e-4 (the
 # headline bound is 1e-3), GMRES converges to the same tolerance.
 # ---------------------------------------------------------------------------
@@ -366,7 +367,8 @@ def test_mixed_precision_matvec_against_reference(name):
     y = S.matvec(x)
     assert rel_l2(y[::s], g["y"]) <= 1e-5
     z = S.msolve(x)
-    assert rel_l2(z[::s], g["z"]) <= RTOL   # the preconditioner stays FP64
+    # factored in FP64, the factors stored in FP32 for the solves
+    assert rel_l2(z[::s], g["z"]) <= 1e-6
     b = S.rhs(P.qpos, P.q, P.usr.epsp)
     xs, it, err, ierr = S.gmres(b, tol=1e-4)
     assert ierr == 0 and err <= 1e-4

@@ -1,6 +1,6 @@
 """GPU parity and accuracy at the benchmark's own sizes.
 
-* The 483,572-face surface of the 8,000-atom synthetic molecule (the
+* The ~484,000-face surface of the 8,000-atom synthetic molecule (the
   benchmark workload): tree permutation (SHA-256 of the reference's), y = A x
   and z = M^-1 x against the reference's own products (flang build, golden
   fixture mol8000_mv), at p=3/theta=0.8; rows of A x at p=3/theta=0.8 and
@@ -58,12 +58,12 @@ def check_order(S, g):
 
 
 # ---------------------------------------------------------------------------
-# config 3: 483,572 faces
+# config 3: the benchmark surface (about 484,000 faces)
 # ---------------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def big():
     P, g = golden_problem("mol8000_mv")
-    print(f"[gpu] set-up 483k surface n={P.xyz.shape[0]}", flush=True)
+    print(f"[gpu] set-up benchmark surface n={P.xyz.shape[0]}", flush=True)
     S = hip.Surface(P.params(), P.xyz, P.nrm, P.area)
     yield P, g, S
     S.close()
@@ -71,9 +71,9 @@ def big():
 
 def test_config3_matches_reference_products(big):
     P, g, S = big
-    assert S.n == int(g["n"]) == 483572
+    assert S.n == int(g["n"])
     check_order(S, g)
-    assert S.stats()["nleaf"] == len(g["leaf_beg"]) == 3354
+    assert S.stats()["nleaf"] == len(g["leaf_beg"])
     s = int(g["stride"])
     x = cases.harness_x(2 * S.n)
     y = S.matvec(x)
@@ -138,7 +138,8 @@ def test_config3_solves_true_residual_and_mixed_energy(big):
         assert res <= 1e-4 * 1.01, res
     e64, _ = S.energy(P.qpos, P.q, P.usr.epsp, x64)
     e32, _ = S32.energy(P.qpos, P.q, P.usr.epsp, x32)
-    print(f"[gpu] 483k solve: iter {it64} / {it32}, dG {e64:.6f} / {e32:.6f}", flush=True)
+    print(f"[gpu] benchmark surface solve: iter {it64} / {it32}, dG {e64:.6f} / {e32:.6f}",
+          flush=True)
     assert abs(e32 - e64) <= 1e-3 * abs(e64)
     S32.close()
 
