@@ -1936,6 +1936,23 @@ constexpr int kSolveB = 64;
 constexpr int kHalf = 32;
 
 // s - sum_{c < bs} A[c*m] * zt[c] for one row, loads issued 32 at a time
+// A factor entry that is used only when `use` holds.  Float factors are
+// loaded unconditionally from a clamped, in-block address and zeroed after the
+// widening: a guarded float load becomes a branch around load and conversion,
+// which serialises the loads (a wait per load); guarded double loads stay in
+// flight and are cheaper than the clamped addresses.
+template <typename FT>
+__device__ __forceinline__ FT factor_ld(const FT *__restrict__ base, size_t off, size_t off_clamped,
+                                        bool use)
+{
+    if constexpr (sizeof(FT) == 8) {
+        return use ? base[off] : FT(0);
+    } else {
+        const FT v = base[off_clamped];
+        return use ? v : FT(0);
+    }
+}
+
 template <typename FT>
 __device__ __forceinline__ double gemv_row(const FT *__restrict__ Ac, int m, int bs,
                                            const double *zt, double s)
@@ -1943,9 +1960,15 @@ __device__ __forceinline__ double gemv_row(const FT *__restrict__ Ac, int m, int
     double s1 = 0.0;
 #pragma unroll 1
     for (int h = 0; h < kSolveB; h += kHalf) {
+        // the loads first, all in flight, then the widening (converting inside
+        // the guarded load made the compiler wait for each load in turn)
+        FT af[kHalf];
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c)
+            af[c] = factor_ld(Ac, (size_t)(h + c) * m, (size_t)min(h + c, bs - 1) * m, h + c < bs);
         double a[kHalf];
 #pragma unroll
-        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? (double)Ac[(size_t)(h + c) * m] : 0.0;
+        for (int c = 0; c < kHalf; ++c) a[c] = (double)af[c];
 #pragma unroll
         for (int c = 0; c < kHalf; c += 2) {
             s = fma(-a[c], (h + c < bs) ? zt[h + c] : 0.0, s);
@@ -1989,10 +2012,15 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
             double zl = in ? zs[kb + lane] : 0.0;
 #pragma unroll 1
             for (int h = 0; h < kSolveB; h += kHalf) {
-                double tr[kHalf];
+                FT tf[kHalf];
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c)
-                    tr[c] = (in && h + c < lane) ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                    tf[c] = factor_ld(A, (size_t)(kb + h + c) * m + kb + lane,
+                                      (size_t)(kb + min(h + c, bs - 1)) * m + kb + min(lane, bs - 1),
+                                      in && h + c < lane);
+                double tr[kHalf];
+#pragma unroll
+                for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c) zl -= tr[c] * __shfl(zl, h + c);
             }
@@ -2012,11 +2040,15 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
             double zl = in ? zs[kb + lane] : 0.0;
 #pragma unroll 1
             for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
-                double tr[kHalf];
+                FT tf[kHalf];
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c)
-                    tr[c] = (in && h + c >= lane && h + c < bs)
-                                ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                    tf[c] = factor_ld(A, (size_t)(kb + h + c) * m + kb + lane,
+                                      (size_t)(kb + min(h + c, bs - 1)) * m + kb + min(lane, bs - 1),
+                                      in && h + c >= lane && h + c < bs);
+                double tr[kHalf];
+#pragma unroll
+                for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
 #pragma unroll
                 for (int c = kHalf - 1; c >= 0; --c) {
                     if (lane == h + c && in) zl = zl / tr[c];
@@ -2546,9 +2578,13 @@ __device__ __forceinline__ void gemv_rowb(const FT *__restrict__ Ac, int m, int 
     for (int k = 0; k < RC; ++k) s1[k] = 0.0;
 #pragma unroll 1
     for (int h = 0; h < kSolveB; h += kHalf) {
+        FT af[kHalf];
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c)
+            af[c] = factor_ld(Ac, (size_t)(h + c) * m, (size_t)min(h + c, bs - 1) * m, h + c < bs);
         double a[kHalf];
 #pragma unroll
-        for (int c = 0; c < kHalf; ++c) a[c] = (h + c < bs) ? (double)Ac[(size_t)(h + c) * m] : 0.0;
+        for (int c = 0; c < kHalf; ++c) a[c] = (double)af[c];
 #pragma unroll
         for (int k = 0; k < RC; ++k) {
 #pragma unroll
@@ -2592,10 +2628,15 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
         for (int k = 0; k < RC; ++k) zl[k] = in ? zsh[k * ldz + kb + lane] : 0.0;
 #pragma unroll 1
         for (int h = 0; h < kSolveB; h += kHalf) {
-            double tr[kHalf];
+            FT tf[kHalf];
 #pragma unroll
             for (int c = 0; c < kHalf; ++c)
-                tr[c] = (in && h + c < lane) ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                tf[c] = factor_ld(A, (size_t)(kb + h + c) * m + kb + lane,
+                                  (size_t)(kb + min(h + c, bs - 1)) * m + kb + min(lane, bs - 1),
+                                  in && h + c < lane);
+            double tr[kHalf];
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
 #pragma unroll
             for (int k = 0; k < RC; ++k)
 #pragma unroll
@@ -2627,11 +2668,15 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
         for (int k = 0; k < RC; ++k) zl[k] = in ? zsh[k * ldz + kb + lane] : 0.0;
 #pragma unroll 1
         for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
-            double tr[kHalf];
+            FT tf[kHalf];
 #pragma unroll
             for (int c = 0; c < kHalf; ++c)
-                tr[c] = (in && h + c >= lane && h + c < bs)
-                            ? (double)A[(size_t)(kb + h + c) * m + kb + lane] : 0.0;
+                tf[c] = factor_ld(A, (size_t)(kb + h + c) * m + kb + lane,
+                                  (size_t)(kb + min(h + c, bs - 1)) * m + kb + min(lane, bs - 1),
+                                  in && h + c >= lane && h + c < bs);
+            double tr[kHalf];
+#pragma unroll
+            for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
 #pragma unroll
             for (int c = kHalf - 1; c >= 0; --c) {
 #pragma unroll
