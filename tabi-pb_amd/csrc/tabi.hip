@@ -1957,28 +1957,31 @@ template <typename FT>
 __device__ __forceinline__ double gemv_row(const FT *__restrict__ Ac, int m, int bs,
                                            const double *zt, double s)
 {
+    // entries in flight per pass (all 64 for float factors took the kernel to
+    // 256 registers and tripled its time)
+    constexpr int H = kHalf;
     double s1 = 0.0;
 #pragma unroll 1
-    for (int h = 0; h < kSolveB; h += kHalf) {
+    for (int h = 0; h < kSolveB; h += H) {
         // the loads first, all in flight, then the widening (converting inside
         // the guarded load made the compiler wait for each load in turn)
-        FT af[kHalf];
+        FT af[H];
 #pragma unroll
-        for (int c = 0; c < kHalf; ++c)
+        for (int c = 0; c < H; ++c)
             af[c] = factor_ld(Ac, (size_t)(h + c) * m, (size_t)min(h + c, bs - 1) * m, h + c < bs);
-        double a[kHalf];
 #pragma unroll
-        for (int c = 0; c < kHalf; ++c) a[c] = (double)af[c];
-#pragma unroll
-        for (int c = 0; c < kHalf; c += 2) {
-            s = fma(-a[c], (h + c < bs) ? zt[h + c] : 0.0, s);
-            s1 = fma(-a[c + 1], (h + c + 1 < bs) ? zt[h + c + 1] : 0.0, s1);
+        for (int c = 0; c < H; c += 2) {
+            s = fma(-(double)af[c], (h + c < bs) ? zt[h + c] : 0.0, s);
+            s1 = fma(-(double)af[c + 1], (h + c + 1 < bs) ? zt[h + c + 1] : 0.0, s1);
         }
     }
     return s + s1;
 }
 
-// NW waves per leaf: wave 0 solves each 64x64 diagonal tile, and the row
+// NW waves per leaf: wave 0 solves each 64x64 diagonal tile (the solved
+// entry of each step reaches the other lanes through a scalar-register lane
+// read, not an LDS shuffle: this chain of 64 dependent steps per tile is the
+// critical path of a large leaf), and the row
 // updates outside the tile (the bulk of the factor stream) are shared by all NW
 // waves, so a large leaf is streamed NW times faster than by one wave.  Leaves
 // are visited in the order lorder (largest first) so the longest solves start
@@ -2022,7 +2025,7 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
 #pragma unroll
                 for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c) zl -= tr[c] * __shfl(zl, h + c);
+                for (int c = 0; c < kHalf; ++c) zl -= tr[c] * readlane_d(zl, h + c);
             }
             if (in) zs[kb + lane] = zl;
         }
@@ -2052,7 +2055,7 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
 #pragma unroll
                 for (int c = kHalf - 1; c >= 0; --c) {
                     if (lane == h + c && in) zl = zl / tr[c];
-                    const double zc = __shfl(zl, h + c);
+                    const double zc = readlane_d(zl, h + c);
                     if (lane < h + c) zl -= tr[c] * zc;
                 }
             }
@@ -2640,7 +2643,7 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
 #pragma unroll
             for (int k = 0; k < RC; ++k)
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c) zl[k] -= tr[c] * __shfl(zl[k], h + c);
+                for (int c = 0; c < kHalf; ++c) zl[k] -= tr[c] * readlane_d(zl[k], h + c);
         }
         if (in) {
 #pragma unroll
@@ -2682,7 +2685,7 @@ __global__ void __launch_bounds__(64) k_lu_solveb(int n, int R, const int32_t *_
 #pragma unroll
                 for (int k = 0; k < RC; ++k) {
                     if (lane == h + c && in) zl[k] = zl[k] / tr[c];
-                    const double zc = __shfl(zl[k], h + c);
+                    const double zc = readlane_d(zl[k], h + c);
                     if (lane < h + c) zl[k] -= tr[c] * zc;
                 }
             }
