@@ -392,3 +392,19 @@ def test_mixed_precision_full_solve_on_molecule():
     r32 = driver.solve(P, precision=1)
     assert r64["ierr"] == 0 and r32["ierr"] == 0
     assert abs(r32["soleng"] - r64["soleng"]) / abs(r64["soleng"]) < 1e-4
+
+
+def test_singular_leaf_block_is_reported():
+    """The reference ignores DGETRF's INFO (src/treecode3d_pb.f:1530); the
+    library reports a singular leaf block as an error from set-up.  With
+    eps = -1 both identity coefficients (1+eps)/2 and (1+1/eps)/2 vanish and
+    zero areas remove every coupling, so each leaf block is exactly zero (the
+    kernels run to completion; the zero pivot is read back by the host)."""
+    P = problem("sph3_l100_p3")
+    prm = hip.make_params(-1.0, 0.1, 0.8, 3, 100)
+    with pytest.raises(hip.TabiError, match="singular leaf block 1 "):
+        hip.Surface(prm, P.xyz, P.nrm, np.zeros_like(P.area))
+    # an ordinary surface still sets up afterwards
+    S = hip.Surface(P.params(), P.xyz, P.nrm, P.area)
+    assert S.stats()["nleaf"] == 56
+    S.close()

@@ -150,3 +150,17 @@ def test_oracle_gmres_restart_on_ses_molecule_matches_reference(oracle_lib):
     assert (it, ierr) == (int(g["iter"]), int(g["ierr"])) == (14, 0)
     assert err == pytest.approx(float(g["err"]), rel=1e-12)
     assert np.linalg.norm(x[::s] - g["xvct"]) <= 1e-12 * np.linalg.norm(g["xvct"])
+
+
+def test_oracle_reports_singular_leaf_block(oracle_lib):
+    """The oracle's partial-pivot LU flags a zero pivot and names the first
+    singular leaf (the GPU library turns the same condition into a set-up
+    error naming leaf 1)."""
+    O = oracle_lib
+    P = staged_problem("sph3_l100_p3")
+    txyz, order, nodes = O.build_tree(P.xyz, 100)
+    ib, ie = O.leaves(nodes)
+    prm = O.params(-1.0, 0.1, 0.8, 3, 100)
+    _, bad = O.precond(prm, txyz, P.nrm[order], np.zeros(len(order)), ib, ie,
+                       np.ones(2 * len(order)))
+    assert bad == 1
