@@ -260,7 +260,7 @@ struct tabi_ws {
 
 struct tabi_ctx {
     tabi_params prm{};
-    int32_t n = 0, ngroups = 0, p = 3, nj = 0, nk = 0;
+    int32_t n = 0, ngroups = 0, p = 3, nw = 0, nk = 0;
     hipStream_t s = nullptr;
     HostTree tree;
     // geometry, tree order
@@ -435,6 +435,9 @@ struct MomShape {
     static constexpr int NK = (P + 1) * (P + 2) * (P + 3) / 6;   // |k| <= P
     static constexpr int L = P + 2;
     static constexpr int NJ = (L + 1) * (L + 2) * (L + 3) / 6;   // |j| <= P+2
+    // length of each of a node's 8 far-field vectors: |k| <= P+1 (35 at p=3,
+    // against NJ = 56 Taylor coefficients); see mom_final_body for the layout
+    static constexpr int NW = (P + 2) * (P + 3) * (P + 4) / 6;
 };
 
 // inverse of tabi::tidx(): compact index -> exponents (i, j, k)
@@ -557,6 +560,10 @@ __global__ void __launch_bounds__(256) k_mom_rawb(int n, const int32_t *__restri
 //   far potential   peng1 = sum_j a0_j W[0][j] + ak_j W[1][j]
 //   far normal der. peng2 = sum_d n0_d sum_j a0_j W[2+d][j] + ak_j W[5+d][j]
 // where a0/ak are the Taylor coefficients for kappa = 0 and kappa.
+// Only entries that can be nonzero are stored, NW = |{k : |k| <= P+1}| per
+// vector: W[0], W[1] vanish at |j| = P+2 (stored at k = j), and W[2+d], W[5+d]
+// wherever exponent d of j is 0 (stored at k = j - e_d).  The contractions
+// read each vector contiguously and skip no nonzero term.
 template <int P>
 __device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict__ mom_chunk_off,
     const double *__restrict__ part, double eps, double *__restrict__ mom_w,
@@ -573,7 +580,9 @@ __device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict
         M[q / NK][q % NK] = s;
     }
     __syncthreads();
-    double *W = mom_w + (size_t)mi * 8 * NJ;
+    constexpr int NW = MomShape<P>::NW;
+    double *W = mom_w + (size_t)mi * 8 * NW;
+    float *W32 = mom_w32 ? mom_w32 + (size_t)mi * 8 * NW : nullptr;
     for (int j = t; j < NJ; j += 256) {
         int jv[3];
         tdecode(j, jv[0], jv[1], jv[2]);
@@ -614,11 +623,20 @@ __device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict
             wv[2 + d] = -hc[d] - he[d];
             wv[5 + d] = hc[d] / eps + he[d];
         }
-        for (int v = 0; v < 8; ++v) W[v * NJ + j] = wv[v];
-        if (mom_w32) {
-            // the FP32 far field's copy, rounded as a separate conversion would
-            float *W32 = mom_w32 + (size_t)mi * 8 * NJ;
-            for (int v = 0; v < 8; ++v) W32[v * NJ + j] = (float)wv[v];
+        // the FP32 far field's copy, rounded as a separate conversion would
+        if (dj <= P + 1) {
+            for (int v = 0; v < 2; ++v) {
+                W[v * NW + j] = wv[v];
+                if (W32) W32[v * NW + j] = (float)wv[v];
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            if (jv[d] < 1) continue;
+            const int k = tabi::tidx(jv[0] - (d == 0), jv[1] - (d == 1), jv[2] - (d == 2));
+            for (int v = 2 + d; v < 8; v += 3) {
+                W[v * NW + k] = wv[v];
+                if (W32) W32[v * NW + k] = (float)wv[v];
+            }
         }
     }
 }
@@ -657,6 +675,7 @@ __global__ void __launch_bounds__(64) k_far(const FarTask *__restrict__ tasks, i
 {
     constexpr int L = MomShape<P>::L;
     constexpr int NJ = MomShape<P>::NJ;
+    constexpr int NW = MomShape<P>::NW;
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const FarTask tk = tasks[task];
@@ -670,28 +689,28 @@ __global__ void __launch_bounds__(64) k_far(const FarTask *__restrict__ tasks, i
         const u64 m = far_mask[it];
         if (!((m >> lane) & 1ull)) continue;
         const double4 geo = ngeo[nd];
-        const double *W = mom_w + (size_t)node2mom[nd] * 8 * NJ;
+        const double *W = mom_w + (size_t)node2mom[nd] * 8 * NW;
         const double dx = px - geo.x, dy = py - geo.y, dz = pz - geo.z;
         {
             double a[NJ];
             tabi::taylor<L, false, double>(dx, dy, dz, 0.0, a);
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                acc1 = fma(a[j], W[j], acc1);
-                ax = fma(a[j], W[2 * NJ + j], ax);
-                ay = fma(a[j], W[3 * NJ + j], ay);
-                az = fma(a[j], W[4 * NJ + j], az);
+            for (int q = 0; q < NW; ++q) {
+                acc1 = fma(a[q], W[q], acc1);
+                ax = fma(a[tabi::tshift(q, 0)], W[2 * NW + q], ax);
+                ay = fma(a[tabi::tshift(q, 1)], W[3 * NW + q], ay);
+                az = fma(a[tabi::tshift(q, 2)], W[4 * NW + q], az);
             }
         }
         {
             double a[NJ];
             tabi::taylor<L, true, double>(dx, dy, dz, kappa, a);
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                acc1 = fma(a[j], W[NJ + j], acc1);
-                ax = fma(a[j], W[5 * NJ + j], ax);
-                ay = fma(a[j], W[6 * NJ + j], ay);
-                az = fma(a[j], W[7 * NJ + j], az);
+            for (int q = 0; q < NW; ++q) {
+                acc1 = fma(a[q], W[NW + q], acc1);
+                ax = fma(a[tabi::tshift(q, 0)], W[5 * NW + q], ax);
+                ay = fma(a[tabi::tshift(q, 1)], W[6 * NW + q], ay);
+                az = fma(a[tabi::tshift(q, 2)], W[7 * NW + q], az);
             }
         }
     }
@@ -961,6 +980,7 @@ __global__ void __launch_bounds__(64) k_far32(const FarTask *__restrict__ tasks,
 {
     constexpr int L = MomShape<P>::L;
     constexpr int NJ = MomShape<P>::NJ;
+    constexpr int NW = MomShape<P>::NW;
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const FarTask tk = tasks[task];
@@ -976,29 +996,29 @@ __global__ void __launch_bounds__(64) k_far32(const FarTask *__restrict__ tasks,
         const u64 m = far_mask[it];
         if (!((m >> lane) & 1ull)) continue;
         const double4 geo = ngeo[nd];
-        const float *W = mom_w + (size_t)node2mom[nd] * 8 * NJ;
+        const float *W = mom_w + (size_t)node2mom[nd] * 8 * NW;
         const float dx = (float)(px - geo.x), dy = (float)(py - geo.y), dz = (float)(pz - geo.z);
         float acc1 = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
         {
             float a[NJ];
             tabi::taylor<L, false, float>(dx, dy, dz, 0.f, a);
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                acc1 = fmaf(a[j], W[j], acc1);
-                ax = fmaf(a[j], W[2 * NJ + j], ax);
-                ay = fmaf(a[j], W[3 * NJ + j], ay);
-                az = fmaf(a[j], W[4 * NJ + j], az);
+            for (int q = 0; q < NW; ++q) {
+                acc1 = fmaf(a[q], W[q], acc1);
+                ax = fmaf(a[tabi::tshift(q, 0)], W[2 * NW + q], ax);
+                ay = fmaf(a[tabi::tshift(q, 1)], W[3 * NW + q], ay);
+                az = fmaf(a[tabi::tshift(q, 2)], W[4 * NW + q], az);
             }
         }
         {
             float a[NJ];
             tabi::taylor<L, true, float>(dx, dy, dz, kappa, a);
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                acc1 = fmaf(a[j], W[NJ + j], acc1);
-                ax = fmaf(a[j], W[5 * NJ + j], ax);
-                ay = fmaf(a[j], W[6 * NJ + j], ay);
-                az = fmaf(a[j], W[7 * NJ + j], az);
+            for (int q = 0; q < NW; ++q) {
+                acc1 = fmaf(a[q], W[NW + q], acc1);
+                ax = fmaf(a[tabi::tshift(q, 0)], W[5 * NW + q], ax);
+                ay = fmaf(a[tabi::tshift(q, 1)], W[6 * NW + q], ay);
+                az = fmaf(a[tabi::tshift(q, 2)], W[7 * NW + q], az);
             }
         }
         s1.add(acc1);
@@ -2368,7 +2388,7 @@ __global__ void __launch_bounds__(64) k_near32b(const NearTask *__restrict__ tas
                 make_double2((double)p1[k].s, (double)p2[k].s);
 }
 
-// moment vectors of column r: mom_w + r*wstride (wstride = nmom*8*NJ)
+// moment vectors of column r: mom_w + r*wstride (wstride = nmom*8*NW)
 template <int P, int RC>
 __global__ void __launch_bounds__(64) k_farb(const FarTask *__restrict__ tasks, int ntask, int n,
     int R, const int32_t *__restrict__ far_node, const u64 *__restrict__ far_mask,
@@ -2380,6 +2400,7 @@ __global__ void __launch_bounds__(64) k_farb(const FarTask *__restrict__ tasks, 
 {
     constexpr int L = MomShape<P>::L;
     constexpr int NJ = MomShape<P>::NJ;
+    constexpr int NW = MomShape<P>::NW;
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
@@ -2396,7 +2417,7 @@ __global__ void __launch_bounds__(64) k_farb(const FarTask *__restrict__ tasks, 
         const u64 m = far_mask[it];
         if (!((m >> lane) & 1ull)) continue;
         const double4 geo = ngeo[nd];
-        const double *W0 = mom_w + (size_t)r0 * wstride + (size_t)node2mom[nd] * 8 * NJ;
+        const double *W0 = mom_w + (size_t)r0 * wstride + (size_t)node2mom[nd] * 8 * NW;
         const double dx = px - geo.x, dy = py - geo.y, dz = pz - geo.z;
         {
             double a[NJ];
@@ -2406,11 +2427,11 @@ __global__ void __launch_bounds__(64) k_farb(const FarTask *__restrict__ tasks, 
                 if (k >= rc) break;
                 const double *W = W0 + (size_t)k * wstride;
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    acc1[k] = fma(a[j], W[j], acc1[k]);
-                    ax[k] = fma(a[j], W[2 * NJ + j], ax[k]);
-                    ay[k] = fma(a[j], W[3 * NJ + j], ay[k]);
-                    az[k] = fma(a[j], W[4 * NJ + j], az[k]);
+                for (int q = 0; q < NW; ++q) {
+                    acc1[k] = fma(a[q], W[q], acc1[k]);
+                    ax[k] = fma(a[tabi::tshift(q, 0)], W[2 * NW + q], ax[k]);
+                    ay[k] = fma(a[tabi::tshift(q, 1)], W[3 * NW + q], ay[k]);
+                    az[k] = fma(a[tabi::tshift(q, 2)], W[4 * NW + q], az[k]);
                 }
             }
         }
@@ -2422,11 +2443,11 @@ __global__ void __launch_bounds__(64) k_farb(const FarTask *__restrict__ tasks, 
                 if (k >= rc) break;
                 const double *W = W0 + (size_t)k * wstride;
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    acc1[k] = fma(a[j], W[NJ + j], acc1[k]);
-                    ax[k] = fma(a[j], W[5 * NJ + j], ax[k]);
-                    ay[k] = fma(a[j], W[6 * NJ + j], ay[k]);
-                    az[k] = fma(a[j], W[7 * NJ + j], az[k]);
+                for (int q = 0; q < NW; ++q) {
+                    acc1[k] = fma(a[q], W[NW + q], acc1[k]);
+                    ax[k] = fma(a[tabi::tshift(q, 0)], W[5 * NW + q], ax[k]);
+                    ay[k] = fma(a[tabi::tshift(q, 1)], W[6 * NW + q], ay[k]);
+                    az[k] = fma(a[tabi::tshift(q, 2)], W[7 * NW + q], az[k]);
                 }
             }
         }
@@ -2452,6 +2473,7 @@ __global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks
 {
     constexpr int L = MomShape<P>::L;
     constexpr int NJ = MomShape<P>::NJ;
+    constexpr int NW = MomShape<P>::NW;
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
@@ -2469,7 +2491,7 @@ __global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks
         const u64 m = far_mask[it];
         if (!((m >> lane) & 1ull)) continue;
         const double4 geo = ngeo[nd];
-        const float *W0 = mom_w + (size_t)r0 * wstride + (size_t)node2mom[nd] * 8 * NJ;
+        const float *W0 = mom_w + (size_t)r0 * wstride + (size_t)node2mom[nd] * 8 * NW;
         const float dx = (float)(px - geo.x), dy = (float)(py - geo.y), dz = (float)(pz - geo.z);
 #pragma unroll
         for (int k = 0; k < RC; ++k) { acc1[k] = 0.f; ax[k] = 0.f; ay[k] = 0.f; az[k] = 0.f; }
@@ -2481,11 +2503,11 @@ __global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks
                 if (k >= rc) break;
                 const float *W = W0 + (size_t)k * wstride;
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    acc1[k] = fmaf(a[j], W[j], acc1[k]);
-                    ax[k] = fmaf(a[j], W[2 * NJ + j], ax[k]);
-                    ay[k] = fmaf(a[j], W[3 * NJ + j], ay[k]);
-                    az[k] = fmaf(a[j], W[4 * NJ + j], az[k]);
+                for (int q = 0; q < NW; ++q) {
+                    acc1[k] = fmaf(a[q], W[q], acc1[k]);
+                    ax[k] = fmaf(a[tabi::tshift(q, 0)], W[2 * NW + q], ax[k]);
+                    ay[k] = fmaf(a[tabi::tshift(q, 1)], W[3 * NW + q], ay[k]);
+                    az[k] = fmaf(a[tabi::tshift(q, 2)], W[4 * NW + q], az[k]);
                 }
             }
         }
@@ -2497,11 +2519,11 @@ __global__ void __launch_bounds__(64) k_far32b(const FarTask *__restrict__ tasks
                 if (k >= rc) break;
                 const float *W = W0 + (size_t)k * wstride;
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    acc1[k] = fmaf(a[j], W[NJ + j], acc1[k]);
-                    ax[k] = fmaf(a[j], W[5 * NJ + j], ax[k]);
-                    ay[k] = fmaf(a[j], W[6 * NJ + j], ay[k]);
-                    az[k] = fmaf(a[j], W[7 * NJ + j], az[k]);
+                for (int q = 0; q < NW; ++q) {
+                    acc1[k] = fmaf(a[q], W[NW + q], acc1[k]);
+                    ax[k] = fmaf(a[tabi::tshift(q, 0)], W[5 * NW + q], ax[k]);
+                    ay[k] = fmaf(a[tabi::tshift(q, 1)], W[6 * NW + q], ay[k]);
+                    az[k] = fmaf(a[tabi::tshift(q, 2)], W[7 * NW + q], az[k]);
                 }
             }
         }
@@ -3217,9 +3239,9 @@ void launch_farb(tabi_ctx *c, int R, int64_t wstride)
 template <int P>
 void block_fields(tabi_ctx *c, int R, const double *X, int64_t xs, const int32_t *d_act)
 {
-    constexpr int NK = MomShape<P>::NK, NJ = MomShape<P>::NJ;
+    constexpr int NK = MomShape<P>::NK, NW = MomShape<P>::NW;
     // moments of every column
-    const int64_t pstride = (int64_t)c->nchunk * 4 * NK, wstride = (int64_t)c->nmom * 8 * NJ;
+    const int64_t pstride = (int64_t)c->nchunk * 4 * NK, wstride = (int64_t)c->nmom * 8 * NW;
     if (c->nmom > 0 && c->n_far_task > 0) {
         KTimer kt(c, 0);
         c->mom_partb.alloc((size_t)R * pstride);
@@ -3534,8 +3556,8 @@ void build_lists(tabi_ctx *c)
         put(c->chunk_end, cend);
         put(c->mom_chunk_off, mco);
         c->mom_part.alloc((size_t)c->nchunk * 4 * c->nk);
-        c->mom_w.alloc((size_t)nm * 8 * c->nj);
-        if (geom32) c->mom_w32.alloc((size_t)nm * 8 * c->nj);
+        c->mom_w.alloc((size_t)nm * 8 * c->nw);
+        if (geom32) c->mom_w32.alloc((size_t)nm * 8 * c->nw);
     }
     DBuf<int32_t> dleaves, dfl;
     if (geom32) {
@@ -3892,7 +3914,7 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     c->n = n;
     c->p = prm->order;
     c->nk = (c->p + 1) * (c->p + 2) * (c->p + 3) / 6;
-    c->nj = (c->p + 3) * (c->p + 4) * (c->p + 5) / 6;
+    c->nw = (c->p + 2) * (c->p + 3) * (c->p + 4) / 6;
     c->ngroups = (n + kWave - 1) / kWave;
     if (!c->s) HIPCHK(hipStreamCreate(&c->s));
     PoolScope ps_(c->s);

@@ -36,6 +36,28 @@ __host__ __device__ constexpr int tidx(int i, int j, int k)
     return (i + j + k) * (i + j + k + 1) * (i + j + k + 2) / 6 + (j + k) * (j + k + 1) / 2 + k;
 }
 
+// Inverse of tidx() without loops (sums of comparisons, degrees <= 11), so that
+// in a fully unrolled loop over id every use folds to a constant.
+__host__ __device__ constexpr int tdeg(int id)
+{
+    return (id >= 1) + (id >= 4) + (id >= 10) + (id >= 20) + (id >= 35) + (id >= 56) +
+           (id >= 84) + (id >= 120) + (id >= 165) + (id >= 220) + (id >= 286);
+}
+__host__ __device__ constexpr int texp(int id, int axis)
+{
+    const int d = tdeg(id);
+    const int r = id - d * (d + 1) * (d + 2) / 6;             // (j+k)(j+k+1)/2 + k
+    const int s = (r >= 1) + (r >= 3) + (r >= 6) + (r >= 10) + (r >= 15) + (r >= 21) +
+                  (r >= 28) + (r >= 36) + (r >= 45) + (r >= 55) + (r >= 66);
+    const int k = r - s * (s + 1) / 2;
+    return axis == 0 ? d - s : axis == 1 ? s - k : k;
+}
+// compact index of the multi-index id + e_axis
+__host__ __device__ constexpr int tshift(int id, int axis)
+{
+    return tidx(texp(id, 0) + (axis == 0), texp(id, 1) + (axis == 1), texp(id, 2) + (axis == 2));
+}
+
 template <typename T>
 __device__ __forceinline__ constexpr T cf1(int i) { return T(1) / T(i); }
 template <typename T>
