@@ -117,6 +117,18 @@ int tabi_ctx_create(const tabi_params *prm, int32_t n, const double *xyz,
                     const double *nrm, const double *area, int32_t *orderarr_out,
                     tabi_ctx **out);
 void tabi_ctx_destroy(tabi_ctx *c);
+/* One context over nsurf surfaces ("packed"; no counterpart in the reference,
+ * which holds one surface per run): surface k's faces occupy [face0[k],
+ * face0[k] + nface[k]) of a common tree-order numbering (face0[k] a multiple of
+ * 64; face0_out receives nsurf + 1 entries, the last = n, the packed length),
+ * the gaps being padding faces that are never read.  Each surface keeps its own
+ * tree and lists (no interaction crosses surfaces); tabi_ctx_matvec /
+ * tabi_ctx_msolve then act on packed vectors [first halves of all surfaces (n);
+ * second halves (n)] with one launch per kernel for every surface, and give
+ * each surface's entries exactly as its own context does. */
+int tabi_ctx_create_packed(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
+                           const double *const *xyz, const double *const *nrm,
+                           const double *const *area, int32_t *face0_out, tabi_ctx **out);
 int tabi_ctx_stats(tabi_ctx *c, int64_t *out10);
 
 /* device-pointer matvec / preconditioner, enqueued on the context's own stream
@@ -187,6 +199,9 @@ int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
 /* Of the last tabi_solve_many on this thread: {graph captures, graph launches,
  * set-up + right-hand sides ms, GMRES ms, energies ms, total ms}. */
 int tabi_last_solve_many_stats(double *out6);
+/* Of the last packed tabi_solve_many on this thread, the set-up's parts in ms:
+ * {trees and geometry, interaction lists and tasks, leaf LU, right-hand sides}. */
+int tabi_last_solve_many_setup(double *out4);
 
 /* The library keeps freed device blocks for reuse by later contexts and solves;
  * this returns all cached (unused) blocks to the runtime. */

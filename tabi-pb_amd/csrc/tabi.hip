@@ -54,7 +54,7 @@ constexpr int kLuNB = 32;        // LU panel width
 
 thread_local std::string g_err;
 thread_local double g_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-thread_local double g_many[6] = {0, 0, 0, 0, 0, 0};
+thread_local double g_many[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 struct Error : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -246,10 +246,10 @@ struct DevIn {
 };
 
 struct NearTask {
-    int32_t s0, s1, leaf, pad;  // source range [s0,s1) of leaf node `leaf`
+    int32_t s0, s1, leaf, sf;   // source range [s0,s1) of leaf node `leaf`; surface
 };
 struct FarTask {
-    int32_t g, i0, i1, pad;
+    int32_t g, i0, i1, sf;      // items [i0,i1) of target group g; surface
 };
 
 }  // namespace
@@ -306,6 +306,24 @@ struct tabi_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // packed surfaces (one context over several surfaces): per surface the
+    // first face (multiple of 64), face count, first node and first group,
+    // each with an end entry; empty for a context of one surface
+    std::vector<int32_t> sf_face0, sf_n, sf_node0, sf_group0;
+    DBuf<int32_t> groot, gend;           // per group: root node, end of its surface's faces
+    DBuf<int32_t> blk_sf;                // per leaf block: its surface
+    // surfaces whose field and preconditioner work is done (device flags, one
+    // per surface; null: all): far, near and leaf-solve tasks of a surface
+    // whose flag is 0 exit at once, leaving its entries of the outputs stale
+    const int32_t *gate = nullptr;
+    int nsurf() const { return sf_n.empty() ? 1 : (int)sf_n.size(); }
+    int32_t nreal() const
+    {
+        if (sf_n.empty()) return n;
+        int64_t t = 0;
+        for (int32_t v : sf_n) t += v;
+        return (int32_t)t;
+    }
     // stats
     int64_t p_near = 0, p_far = 0, p_mom = 0, n_far_items = 0, n_near_items = 0;
 };
@@ -363,7 +381,8 @@ __global__ void __launch_bounds__(256) k_lists(int pass, int n, int ngroups,
     int32_t *__restrict__ cnt_near, const int32_t *__restrict__ far_off,
     const int32_t *__restrict__ near_off, int32_t *__restrict__ far_node,
     u64 *__restrict__ far_mask, int32_t *__restrict__ near_leaf, u64 *__restrict__ near_mask,
-    int32_t *__restrict__ need_mom, int32_t *__restrict__ err_flag)
+    int32_t *__restrict__ need_mom, int32_t *__restrict__ err_flag,
+    const int32_t *__restrict__ groot, const int32_t *__restrict__ gend)
 {
     __shared__ int32_t s_node[4][kStack];
     __shared__ u64 s_mask[4][kStack];
@@ -371,11 +390,12 @@ __global__ void __launch_bounds__(256) k_lists(int pass, int n, int ngroups,
     const int g = blockIdx.x * 4 + w;
     if (g >= ngroups) return;
     const int i = g * kWave + lane;
-    const bool valid = i < n;
+    // packed surfaces: the group's own tree and the end of its surface's faces
+    const bool valid = i < (gend ? gend[g] : n);
     const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
     int sp = 0;
     const u64 vmask = __ballot(valid);
-    if (lane == 0) { s_node[w][0] = 0; s_mask[w][0] = vmask; }
+    if (lane == 0) { s_node[w][0] = groot ? groot[g] : 0; s_mask[w][0] = vmask; }
     sp = 1;
     int nf = 0, nn = 0;
     int fbase = 0, nbase = 0;
@@ -671,7 +691,8 @@ __global__ void __launch_bounds__(64) k_far(const FarTask *__restrict__ tasks, i
     const double4 *__restrict__ ngeo, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
-    const double *__restrict__ tnz, double kappa, double2 *__restrict__ part)
+    const double *__restrict__ tnz, double kappa, double2 *__restrict__ part,
+    const int32_t *__restrict__ gate)
 {
     constexpr int L = MomShape<P>::L;
     constexpr int NJ = MomShape<P>::NJ;
@@ -679,6 +700,7 @@ __global__ void __launch_bounds__(64) k_far(const FarTask *__restrict__ tasks, i
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const FarTask tk = tasks[task];
+    if (gate && !gate[tk.sf]) return;           // surface not being solved
     const int lane = lane_id();
     const int i = tk.g * kWave + lane;
     const bool valid = i < n;
@@ -783,11 +805,13 @@ __global__ void __launch_bounds__(64) k_near(const NearTask *__restrict__ tasks,
     const double *__restrict__ src8, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
-    const double *__restrict__ tnz, double kappa, double eps, double2 *__restrict__ part)
+    const double *__restrict__ tnz, double kappa, double eps, double2 *__restrict__ part,
+    const int32_t *__restrict__ gate)
 {
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const NearTask tk = tasks[task];
+    if (gate && !gate[tk.sf]) return;           // surface not being solved
     const int lane = lane_id();
     const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
     const bool act = i >= 0;
@@ -929,11 +953,13 @@ __global__ void __launch_bounds__(64) k_near32(const NearTask *__restrict__ task
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, const double *__restrict__ tnx,
     const double *__restrict__ tny, const double *__restrict__ tnz, float kappa, float eps,
-    double2 *__restrict__ part)
+    double2 *__restrict__ part,
+    const int32_t *__restrict__ gate)
 {
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const NearTask tk = tasks[task];
+    if (gate && !gate[tk.sf]) return;           // surface not being solved
     const int lane = lane_id();
     const int i = near_tgt[(size_t)task * kWave + lane];   // this lane's target, -1: none
     const bool act = i >= 0;
@@ -976,7 +1002,8 @@ __global__ void __launch_bounds__(64) k_far32(const FarTask *__restrict__ tasks,
     const double4 *__restrict__ ngeo, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
-    const double *__restrict__ tnz, float kappa, double2 *__restrict__ part)
+    const double *__restrict__ tnz, float kappa, double2 *__restrict__ part,
+    const int32_t *__restrict__ gate)
 {
     constexpr int L = MomShape<P>::L;
     constexpr int NJ = MomShape<P>::NJ;
@@ -984,6 +1011,7 @@ __global__ void __launch_bounds__(64) k_far32(const FarTask *__restrict__ tasks,
     const int task = blockIdx.x;
     if (task >= ntask) return;
     const FarTask tk = tasks[task];
+    if (gate && !gate[tk.sf]) return;           // surface not being solved
     const int lane = lane_id();
     const int i = tk.g * kWave + lane;
     const bool valid = i < n;
@@ -1027,11 +1055,14 @@ __global__ void __launch_bounds__(64) k_far32(const FarTask *__restrict__ tasks,
     part[(size_t)task * kWave + lane] = make_double2((double)s1.s, (double)s2.s);
 }
 
+// grid-stride: the factors of many packed surfaces exceed 2^32 entries, more
+// work-items than one launch dimension can address
 __global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) b[i] = (float)a[i];
 }
+inline unsigned to_float_grid(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20); }
 
 // Near-field task layout.  Item k of target group g (targets g*64 + lane)
 // lists source leaf nd for the lanes set in near_mask[k].  Leaf nd's targets
@@ -1141,6 +1172,7 @@ __global__ void __launch_bounds__(256) k_lu_assemble(int n, const int32_t *__res
     const int jw = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kAsmPerWave;
     for (int j = jw; j < min(n, jw + kAsmPerWave); ++j) {
     const int l = leaf_of[j];
+    if (l < 0) continue;                 // padding between packed surfaces
     const int ib = blk_ibeg[l], m = blk_m[l], nl = m / 2;
     const int jj = j - ib;
     double *A = lu + blk_off[l];
@@ -2015,10 +2047,11 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
     const int32_t *__restrict__ blk_ibeg, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
     const FT *__restrict__ lu, const int32_t *__restrict__ perm, const double *__restrict__ r,
-    double *__restrict__ z)
+    double *__restrict__ z, const int32_t *__restrict__ gate, const int32_t *__restrict__ blk_sf)
 {
     extern __shared__ double zs[];
     const int l = lorder[blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (gate && !gate[blk_sf[l]]) return;      // surface not being solved
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
     const FT *A = lu + blk_off[l];
     const int32_t *pm = perm + piv_off[l];
@@ -2105,12 +2138,13 @@ __global__ void k_charge_rec(int nq, const double *__restrict__ qpos, const doub
 
 // b_i = sum_k q_k/(4 pi eps_p r), b_{n+i} = sum_k q_k cos/(4 pi eps_p r^2)
 // (src/main.f90:597-607), one thread per face, the charge records read
-// wave-uniformly (scalar loads, no LDS staging)
+// wave-uniformly (scalar loads, no LDS staging); the second half is written
+// ho elements after the first (ho = n, or the packed length)
 __global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double4 *__restrict__ qrec,
     double eps_p, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
-    const double *__restrict__ tnz, double *__restrict__ b)
+    const double *__restrict__ tnz, double *__restrict__ b, int64_t ho)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
@@ -2129,7 +2163,7 @@ __global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double4 *__res
     if (valid) {
         const double s = 1.0 / (4.0 * kPi * eps_p);
         b[i] = b1 * s;
-        b[n + i] = b2 * s;
+        b[ho + i] = b2 * s;
     }
 }
 
@@ -2848,14 +2882,24 @@ __global__ void k_energy_reduceb(int nq, int nchunk, int R, const double *__rest
 // found through a pointer table, tab[slot*R + r], and have length nv[r]; the
 // column list act maps grid column blockIdx.y to a run.  Per run the arithmetic
 // is that of the single-vector kernels (same partition and summation order).
+// A vector may lie in two segments: its second half (the normal-derivative
+// unknowns) starts nv[R + r] elements after the end of its first half, which
+// is how a run's vector sits inside a vector of several packed surfaces
+// ([all potentials; all normal derivatives]); nv[R + r] = 0 for a contiguous
+// vector.
+__device__ __forceinline__ int64_t vix(int64_t i, int64_t half, int64_t gap)
+{
+    return i + (i >= half ? gap : 0);
+}
 // y += (sgn * coef[r*cs + cofs]) * x
 __global__ void k_axpyt(double *const *__restrict__ tab, int R, int sx, int sy,
     const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int cofs,
     double sgn, const int32_t *__restrict__ act)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i >= nv[r]) return;
+    if (i0 >= nv[r]) return;
+    const int64_t i = vix(i0, nv[r] >> 1, nv[R + r]);
     const double a = sgn * coef[(size_t)r * cs + cofs];
     double *y = tab[(size_t)sy * R + r];
     y[i] += a * tab[(size_t)sx * R + r][i];
@@ -2867,7 +2911,7 @@ __global__ void k_scalt(double *const *__restrict__ tab, int R, int sx, const in
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < nv[r]) tab[(size_t)sx * R + r][i] *= coef[(size_t)r * cs + cofs];
+    if (i < nv[r]) tab[(size_t)sx * R + r][vix(i, nv[r] >> 1, nv[R + r])] *= coef[(size_t)r * cs + cofs];
 }
 
 // out = sum_{k < nvv} coef[r*cs + k] * slot(sv0 + k)
@@ -2875,9 +2919,10 @@ __global__ void k_lincombt(double *const *__restrict__ tab, int R, int sv0, int 
     const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int so,
     const int32_t *__restrict__ act)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i >= nv[r]) return;
+    if (i0 >= nv[r]) return;
+    const int64_t i = vix(i0, nv[r] >> 1, nv[R + r]);
     const double *cr = coef + (size_t)r * cs;
     double s = 0.0;
     for (int k = 0; k < nvv; ++k) s += cr[k] * tab[(size_t)(sv0 + k) * R + r][i];
@@ -2925,7 +2970,7 @@ __global__ void __launch_bounds__(256) k_axpy_dott(double *const *__restrict__ t
     double *y = tab[(size_t)sy * R + r];
     const double *x = tab[(size_t)(sx >= 0 ? sx : sy) * R + r];
     const double *a = tab[(size_t)(o1 >= 0 ? sa : sy) * R + r];
-    const int64_t n = nv[r];
+    const int64_t n = nv[r], hn = n >> 1, gp = nv[R + r];
     double *pr = part + (size_t)r * dvs * gridDim.x;
     const double c = sx >= 0 ? -1.0 * sum_partials(pr + (size_t)hs * gridDim.x, gridDim.x, red[0]) : 0.0;
     double s1 = 0.0, s2 = 0.0;
@@ -2936,11 +2981,11 @@ __global__ void __launch_bounds__(256) k_axpy_dott(double *const *__restrict__ t
         double yv[kFuseU], xv[kFuseU], av[kFuseU];
 #pragma unroll
         for (int u = 0; u < kFuseU; ++u) {
-            const int64_t i = i0 + u * stride;
+            const int64_t i = i0 + u * stride, j = vix(i, hn, gp);
             const bool ok = i < n;
-            yv[u] = ok ? y[i] : 0.0;
-            xv[u] = ok && sx >= 0 ? x[i] : 0.0;
-            av[u] = ok && o1 >= 0 && sa != sy ? a[i] : 0.0;
+            yv[u] = ok ? y[j] : 0.0;
+            xv[u] = ok && sx >= 0 ? x[j] : 0.0;
+            av[u] = ok && o1 >= 0 && sa != sy ? a[j] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kFuseU; ++u) {
@@ -2949,7 +2994,7 @@ __global__ void __launch_bounds__(256) k_axpy_dott(double *const *__restrict__ t
             double yi = yv[u];
             if (sx >= 0) {
                 yi += c * xv[u];
-                y[i] = yi;
+                y[vix(i, hn, gp)] = yi;
             }
             if (o1 >= 0) s1 += (sa == sy ? yi : av[u]) * yi;
             if (o2 >= 0) s2 += yi * yi;
@@ -2991,7 +3036,7 @@ __global__ void __launch_bounds__(256) k_lincomb_dott(double *const *__restrict_
     double *out = tab[(size_t)so * R + r];
     double *vl = tab[(size_t)(sv0 + nvv - 1) * R + r];
     const double sc = csc >= 0 ? cr[csc] : 1.0;
-    const int64_t n = nv[r];
+    const int64_t n = nv[r], hn = n >> 1, gp = nv[R + r];
     double s2 = 0.0;
     const int64_t stride = (int64_t)gridDim.x * 256;
     for (int64_t i0 = (int64_t)blockIdx.x * 256 + t; i0 < n; i0 += stride * kFuseU) {
@@ -3003,7 +3048,7 @@ __global__ void __launch_bounds__(256) k_lincomb_dott(double *const *__restrict_
 #pragma unroll
             for (int u = 0; u < kFuseU; ++u) {
                 const int64_t i = i0 + u * stride;
-                const double v = i < n ? vk[i] : 0.0;
+                const double v = i < n ? vk[vix(i, hn, gp)] : 0.0;
                 sv[u] += ck * v;
                 if (k == nvv - 1) lv[u] = v;
             }
@@ -3012,8 +3057,9 @@ __global__ void __launch_bounds__(256) k_lincomb_dott(double *const *__restrict_
         for (int u = 0; u < kFuseU; ++u) {
             const int64_t i = i0 + u * stride;
             if (i >= n) break;
-            out[i] = sv[u];
-            if (csc >= 0) vl[i] = lv[u] * sc;
+            const int64_t j = vix(i, hn, gp);
+            out[j] = sv[u];
+            if (csc >= 0) vl[j] = lv[u] * sc;
             s2 += sv[u] * sv[u];
         }
     }
@@ -3057,7 +3103,9 @@ __global__ void k_subt(double *const *__restrict__ tab, int R, int sa, int sb, i
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < nv[r]) tab[(size_t)so * R + r][i] = tab[(size_t)sa * R + r][i] - tab[(size_t)sb * R + r][i];
+    if (i >= nv[r]) return;
+    const int64_t j = vix(i, nv[r] >> 1, nv[R + r]);
+    tab[(size_t)so * R + r][j] = tab[(size_t)sa * R + r][j] - tab[(size_t)sb * R + r][j];
 }
 
 __global__ void k_copyt(double *const *__restrict__ tab, int R, int sa, int so,
@@ -3065,7 +3113,10 @@ __global__ void k_copyt(double *const *__restrict__ tab, int R, int sa, int so,
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < nv[r]) tab[(size_t)so * R + r][i] = tab[(size_t)sa * R + r][i];
+    if (i < nv[r]) {
+        const int64_t j = vix(i, nv[r] >> 1, nv[R + r]);
+        tab[(size_t)so * R + r][j] = tab[(size_t)sa * R + r][j];
+    }
 }
 
 __global__ void k_fillt(double *const *__restrict__ tab, int R, int so, const int64_t *__restrict__ nv,
@@ -3073,7 +3124,7 @@ __global__ void k_fillt(double *const *__restrict__ tab, int R, int so, const in
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
-    if (i < nv[r]) tab[(size_t)so * R + r][i] = v;
+    if (i < nv[r]) tab[(size_t)so * R + r][vix(i, nv[r] >> 1, nv[R + r])] = v;
 }
 
 
@@ -3099,12 +3150,13 @@ void launch_far(tabi_ctx *c)
         // mom_w32 was written beside mom_w by k_mom_final
         k_far32<P><<<c->n_far_task, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n,
             c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_w32.p, c->ngeo.p, c->tx.p,
-            c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, (float)c->prm.kappa, c->part_far.p);
+            c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, (float)c->prm.kappa, c->part_far.p,
+            c->gate);
         return;
     }
     k_far<P><<<c->n_far_task, 64, 0, c->s>>>(c->far_tasks.p, c->n_far_task, c->n,
         c->far_node.p, c->far_mask.p, c->node2mom.p, c->mom_w.p, c->ngeo.p, c->tx.p, c->ty.p,
-        c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->part_far.p);
+        c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa, c->part_far.p, c->gate);
 }
 
 void dispatch_moments(tabi_ctx *c, const double *x)
@@ -3155,12 +3207,12 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
             k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src32.p);
             k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src32.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
-                (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p);
+                (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p, c->gate);
         } else {
             k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src8.p);
             k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src8.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
-                c->prm.eps, c->part_near.p);
+                c->prm.eps, c->part_near.p, c->gate);
         }
     }
     if (c->n_far_task) { KTimer kt(c, 2); dispatch_far(c); }
@@ -3200,10 +3252,10 @@ void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
     KTimer kt(c, 4);
     if (c->lu32.p)
         k_lu_solve<4, float><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
-            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu32.p, c->perm.p, dr, dz);
+            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu32.p, c->perm.p, dr, dz, c->gate, c->blk_sf.p);
     else
         k_lu_solve<4, double><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
-            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz);
+            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz, c->gate, c->blk_sf.p);
     HIPCHK(hipGetLastError());
 }
 
@@ -3400,7 +3452,8 @@ void build_lists(tabi_ctx *c)
     const unsigned nb = nblk(G, 4);
     k_lists<<<nb, 256, 0, c->s>>>(0, c->n, G, c->tx.p, c->ty.p, c->tz.p, c->ngeo.p, c->nnum.p,
         c->nchild.p, c->ncoff.p, c->nkids.p, c->prm.theta, c->cnt_far.p, c->cnt_near.p,
-        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->err_flag.p);
+        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->err_flag.p, c->groot.p,
+        c->gend.p);
     HIPCHK(hipGetLastError());
     std::vector<int32_t> cf(G), cn(G);
     int32_t ef = 0;
@@ -3427,7 +3480,7 @@ void build_lists(tabi_ctx *c)
     k_lists<<<nb, 256, 0, c->s>>>(1, c->n, G, c->tx.p, c->ty.p, c->tz.p, c->ngeo.p, c->nnum.p,
         c->nchild.p, c->ncoff.p, c->nkids.p, c->prm.theta, nullptr, nullptr, c->far_off.p,
         c->near_off.p, c->far_node.p, c->far_mask.p, c->near_leaf.p, c->near_mask.p,
-        c->need_mom.p, c->err_flag.p);
+        c->need_mom.p, c->err_flag.p, c->groot.p, c->gend.p);
     HIPCHK(hipGetLastError());
 
     // host: tasks, moment chunks, counters (the far masks stay on the device:
@@ -3461,41 +3514,49 @@ void build_lists(tabi_ctx *c)
         lcnt[nd] += pc;
         c->p_near += (int64_t)pc * T.numpar[nd];
     }
-    c->p_near -= c->n;  // self pairs are excluded
-    int64_t near_src_total = 0;   // source sweeps of the compacted waves
-    for (int nd = 0; nd < nn; ++nd)
-        if (lcnt[nd]) near_src_total += (int64_t)((lcnt[nd] + kWave - 1) / kWave) * T.numpar[nd];
+    c->p_near -= c->nreal();  // self pairs are excluded
 
     // task granularity: enough waves to fill the device several times over
     // work per task: the near source runs and far item runs are cut so that
     // each launch has about `target` tasks (flags bits 8-11 override the
-    // default of 2^16 with 2^(10+t), t = 1..15)
+    // default of 2^16 with 2^(10+t), t = 1..15).  Packed surfaces are cut
+    // each by its own totals, as it would be alone, so every target sums the
+    // same task partials in the same order as in its own context.
     const int tt = (c->prm.flags >> 8) & 15;
     const int64_t target = tt ? (int64_t)1 << (10 + tt) : 65536;
-    int S = 64;
-    while (S < 2048 && near_src_total / S > target) S *= 2;
-    int F = 1;
-    while (F < 64 && nf / F > target) F *= 2;
-    // tasks per leaf: waves of 64 targets x source runs of S
+    const int nsf = c->nsurf();
+    std::vector<int32_t> node0 = c->sf_node0, group0 = c->sf_group0;
+    if (c->sf_n.empty()) { node0 = {0, nn}; group0 = {0, G}; }
     std::vector<int32_t> toff(nn), nsplit(nn);
     std::vector<NearTask> nt;
-    for (int nd = 0; nd < nn; ++nd) {
-        if ((int64_t)nt.size() * kWave > INT32_MAX) throw Error("too many near-field tasks");
-        toff[nd] = (int32_t)nt.size();
-        nsplit[nd] = (T.numpar[nd] + S - 1) / S;
-        const int b = T.ibeg[nd], e = T.iend[nd] + 1;
-        for (int c0 = 0; c0 < lcnt[nd]; c0 += kWave)
-            for (int s0 = b; s0 < e; s0 += S) nt.push_back({s0, std::min(e, s0 + S), nd, 0});
+    std::vector<FarTask> ft;
+    std::vector<int32_t> fto(G + 1, 0);
+    for (int sf = 0; sf < nsf; ++sf) {
+        int64_t near_src_total = 0;   // source sweeps of the compacted waves
+        for (int nd = node0[sf]; nd < node0[sf + 1]; ++nd)
+            if (lcnt[nd]) near_src_total += (int64_t)((lcnt[nd] + kWave - 1) / kWave) * T.numpar[nd];
+        const int64_t nfs = c->h_far_off[group0[sf + 1]] - c->h_far_off[group0[sf]];
+        int S = 64;
+        while (S < 2048 && near_src_total / S > target) S *= 2;
+        int F = 1;
+        while (F < 64 && nfs / F > target) F *= 2;
+        // tasks per leaf: waves of 64 targets x source runs of S
+        for (int nd = node0[sf]; nd < node0[sf + 1]; ++nd) {
+            if ((int64_t)nt.size() * kWave > INT32_MAX) throw Error("too many near-field tasks");
+            toff[nd] = (int32_t)nt.size();
+            nsplit[nd] = (T.numpar[nd] + S - 1) / S;
+            const int b = T.ibeg[nd], e = T.iend[nd] + 1;
+            for (int c0 = 0; c0 < lcnt[nd]; c0 += kWave)
+                for (int s0 = b; s0 < e; s0 += S) nt.push_back({s0, std::min(e, s0 + S), nd, sf});
+        }
+        for (int g = group0[sf]; g < group0[sf + 1]; ++g) {
+            for (int32_t k = c->h_far_off[g]; k < c->h_far_off[g + 1]; k += F)
+                ft.push_back({g, k, std::min(c->h_far_off[g + 1], k + F), sf});
+            fto[g + 1] = (int32_t)ft.size();
+        }
     }
     const int64_t ntask = (int64_t)nt.size();
     if (ntask * kWave > INT32_MAX) throw Error("too many near-field tasks");
-    std::vector<FarTask> ft;
-    std::vector<int32_t> fto(G + 1, 0);
-    for (int g = 0; g < G; ++g) {
-        for (int32_t k = c->h_far_off[g]; k < c->h_far_off[g + 1]; k += F)
-            ft.push_back({g, k, std::min(c->h_far_off[g + 1], k + F), 0});
-        fto[g + 1] = (int32_t)ft.size();
-    }
     c->n_near_task = (int32_t)ntask;
     c->n_far_task = (int32_t)ft.size();
 
@@ -3564,6 +3625,8 @@ void build_lists(tabi_ctx *c)
         // leaf-local FP32 source records: each particle's leaf node, filled on the GPU
         put(dleaves, T.leaves);
         dfl.alloc(c->n);
+        if (!c->sf_n.empty())   // padding faces of packed surfaces: any valid node
+            k_fill_i32<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, 0, dfl.p);
         k_leaf_fill<<<(unsigned)T.leaves.size(), 256, 0, c->s>>>(dleaves.p, c->nibeg.p, c->nnum.p, dfl.p);
         c->src32.alloc((size_t)c->n * 8);
         k_src32_geom<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, dfl.p, c->ngeo.p, c->tx.p, c->ty.p,
@@ -3686,7 +3749,7 @@ void build_precond(tabi_ctx *c)
     const HostTree &T = c->tree;
     const int nl = (int)T.leaves.size();
     c->nleaf = nl;
-    std::vector<int32_t> bib(nl), bm(nl), poff(nl), leaf_of(c->n);
+    std::vector<int32_t> bib(nl), bm(nl), poff(nl), leaf_of(c->n, -1);
     c->h_blk_off.assign(nl, 0);
     int64_t off = 0;
     int32_t pof = 0;
@@ -3703,6 +3766,14 @@ void build_precond(tabi_ctx *c)
         for (int i = T.ibeg[nd]; i <= T.iend[nd]; ++i) leaf_of[i] = l;
     }
     c->h_blk_m = bm;
+    {
+        std::vector<int32_t> bsf(nl, 0);
+        if (!c->sf_n.empty())
+            for (int l = 0; l < nl; ++l)
+                bsf[l] = (int32_t)(std::upper_bound(c->sf_node0.begin(), c->sf_node0.end(), T.leaves[l]) -
+                                   c->sf_node0.begin()) - 1;
+        c->blk_sf.upload(bsf.data(), nl, c->s);
+    }
     c->blk_ibeg.upload(bib.data(), nl, c->s);
     c->blk_m.upload(bm.data(), nl, c->s);
     c->blk_off.upload(c->h_blk_off.data(), nl, c->s);
@@ -3866,7 +3937,7 @@ void build_precond(tabi_ctx *c)
         // mixed precision: the FP64 factors are rounded once to FP32 for the
         // solves, which then stream half the bytes
         c->lu32.alloc(off);
-        k_to_float<<<nblk(off, 256), 256, 0, c->s>>>(off, c->lu.p, c->lu32.p);
+        k_to_float<<<to_float_grid(off), 256, 0, c->s>>>(off, c->lu.p, c->lu32.p);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->s));
         c->lu.release();
@@ -3984,6 +4055,156 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     }
 }
 
+// One context over several surfaces ("packed"): surface r's faces occupy
+// [F_r, F_r + n_r) in a common numbering, F_r a multiple of 64 so a target group
+// of 64 faces never spans two surfaces; the faces between surfaces are
+// padding (zero geometry and area, in no leaf and no group's targets).  Each
+// surface keeps its own tree (built exactly as alone), merged into one node
+// table; a group's interaction lists are built from its own surface's root,
+// so no interaction crosses surfaces, and the tasks of each surface are cut by
+// its own totals.  The field, moment and leaf-block kernels then serve every
+// surface in one launch each, and a surface's results are those of its own
+// context bit for bit.  Vectors are [first halves of all surfaces (c->n
+// entries); second halves], surface r's at F_r and c->n + F_r.
+void ctx_setup_packed(tabi_ctx *c, const tabi_params *prm, int nsurf, const int32_t *nface,
+                      const double *const *xyz, const double *const *nrm,
+                      const double *const *area, double *t_phase = nullptr)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!prm) throw Error("null parameters");
+    if (nsurf < 1) throw Error("need at least one surface");
+    if (prm->order < 1 || prm->order > kMaxOrder) throw Error("Taylor order must be 1..6");
+    if (prm->maxparnode < 1) throw Error("maxparnode must be >= 1");
+    if (prm->precision != 0 && prm->precision != 1) throw Error("precision must be 0 or 1");
+    c->prm = *prm;
+    c->p = prm->order;
+    c->nk = (c->p + 1) * (c->p + 2) * (c->p + 3) / 6;
+    c->nw = (c->p + 2) * (c->p + 3) * (c->p + 4) / 6;
+    c->sf_face0.assign(nsurf + 1, 0);
+    c->sf_n.assign(nface, nface + nsurf);
+    int64_t F = 0;
+    for (int r = 0; r < nsurf; ++r) {
+        if (nface[r] < 1) throw Error("surface " + std::to_string(r) + " has no faces");
+        c->sf_face0[r] = (int32_t)F;
+        F += (nface[r] + kWave - 1) / kWave * kWave;
+        if (F > INT32_MAX / 2) throw Error("packed surfaces too large");
+    }
+    c->sf_face0[nsurf] = (int32_t)F;
+    const int32_t n = (int32_t)F;
+    c->n = n;
+    c->ngroups = n / kWave;
+    if (!c->s) HIPCHK(hipStreamCreate(&c->s));
+    PoolScope ps_(c->s);
+    c->tx.alloc(n); c->ty.alloc(n); c->tz.alloc(n);
+    c->tnx.alloc(n); c->tny.alloc(n); c->tnz.alloc(n); c->tarea.alloc(n);
+    c->src8.alloc((size_t)n * 8);
+    for (DBuf<double> *b : {&c->tx, &c->ty, &c->tz, &c->tnx, &c->tny, &c->tnz, &c->tarea})
+        HIPCHK(hipMemsetAsync(b->p, 0, sizeof(double) * n, c->s));
+    HIPCHK(hipMemsetAsync(c->src8.p, 0, sizeof(double) * 8 * (size_t)n, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    // trees, several surfaces at a time from host threads (own streams), each
+    // writing its own range of the packed arrays
+    std::vector<HostTree> trees(nsurf);
+    {
+        const int nth = std::max(1, std::min(nsurf, 8));
+        std::vector<std::thread> th;
+        std::vector<std::string> errs(nth);
+        for (int t = 0; t < nth; ++t)
+            th.emplace_back([&, t] {
+                hipStream_t st = nullptr;
+                try {
+                    HIPCHK(hipStreamCreate(&st));
+                    PoolScope pt_(st);
+                    for (int r = t; r < nsurf; r += nth) {
+                        const int32_t nr = nface[r];
+                        const int64_t f0 = c->sf_face0[r];
+                        DevIn d_xyz(xyz[r], (size_t)3 * nr, false, st), d_nrm(nrm[r], (size_t)3 * nr, false, st),
+                            d_area(area[r], nr, false, st);
+                        DBuf<double> dx, dy, dz;
+                        DBuf<int32_t> d_ord;
+                        dx.alloc(nr); dy.alloc(nr); dz.alloc(nr); d_ord.alloc(nr);
+                        k_deinterleave<<<nblk(nr, 256), 256, 0, st>>>(nr, d_xyz.p, dx.p, dy.p, dz.p);
+                        HIPCHK(hipGetLastError());
+                        tabi::build_tree_gpu(nr, dx.p, dy.p, dz.p, prm->maxparnode, st, trees[r],
+                                             c->tx.p + f0, c->ty.p + f0, c->tz.p + f0, d_ord.p);
+                        k_gather_geom<<<nblk(nr, 256), 256, 0, st>>>(nr, d_ord.p, d_nrm.p, d_area.p,
+                            c->tx.p + f0, c->ty.p + f0, c->tz.p + f0, c->tnx.p + f0, c->tny.p + f0,
+                            c->tnz.p + f0, c->tarea.p + f0, c->src8.p + 8 * f0);
+                        HIPCHK(hipGetLastError());
+                        HIPCHK(hipStreamSynchronize(st));
+                    }
+                } catch (const std::exception &e) {
+                    errs[t] = e.what();
+                }
+                destroy_stream(st);
+            });
+        for (auto &x : th) x.join();
+        for (auto &e : errs)
+            if (!e.empty()) throw Error(e);
+    }
+    // one node table: surface r's nodes follow surface r-1's
+    HostTree &T = c->tree;
+    T = HostTree{};
+    T.n = n;
+    c->sf_node0.assign(nsurf + 1, 0);
+    c->sf_group0.assign(nsurf + 1, 0);
+    for (int r = 0; r < nsurf; ++r) {
+        const HostTree &S = trees[r];
+        const int32_t no = T.nnode(), fo = c->sf_face0[r], co = (int32_t)T.children.size();
+        c->sf_node0[r] = no;
+        c->sf_group0[r] = fo / kWave;
+        for (int k = 0; k < S.nnode(); ++k) {
+            T.ibeg.push_back(S.ibeg[k] + fo);
+            T.iend.push_back(S.iend[k] + fo);
+            T.child_off.push_back(S.child_off[k] + co);
+        }
+        auto app = [](auto &d, const auto &v) { d.insert(d.end(), v.begin(), v.end()); };
+        app(T.numpar, S.numpar); app(T.level, S.level); app(T.nchild, S.nchild);
+        app(T.xmin, S.xmin); app(T.xmax, S.xmax); app(T.ymin, S.ymin); app(T.ymax, S.ymax);
+        app(T.zmin, S.zmin); app(T.zmax, S.zmax);
+        app(T.xmid, S.xmid); app(T.ymid, S.ymid); app(T.zmid, S.zmid); app(T.radius, S.radius);
+        for (int32_t k : S.children) T.children.push_back(k + no);
+        for (int32_t k : S.leaves) T.leaves.push_back(k + no);
+    }
+    c->sf_node0[nsurf] = T.nnode();
+    c->sf_group0[nsurf] = c->ngroups;
+    std::vector<int32_t> groot(c->ngroups), gend(c->ngroups);
+    for (int r = 0; r < nsurf; ++r)
+        for (int g = c->sf_group0[r]; g < c->sf_group0[r + 1]; ++g) {
+            groot[g] = c->sf_node0[r];
+            gend[g] = c->sf_face0[r] + nface[r];
+        }
+    c->groot.upload(groot.data(), groot.size(), c->s);
+    c->gend.upload(gend.data(), gend.size(), c->s);
+    const int nn = T.nnode();
+    std::vector<double4> geo(nn);
+    for (int k = 0; k < nn; ++k) geo[k] = make_double4(T.xmid[k], T.ymid[k], T.zmid[k], T.radius[k]);
+    c->ngeo.upload(geo.data(), nn, c->s);
+    c->nnum.upload(T.numpar.data(), nn, c->s);
+    c->nchild.upload(T.nchild.data(), nn, c->s);
+    c->ncoff.upload(T.child_off.data(), nn, c->s);
+    c->nibeg.upload(T.ibeg.data(), nn, c->s);
+    c->nkids.upload(T.children.empty() ? T.nchild.data() : T.children.data(),
+                    std::max<size_t>(T.children.size(), 1), c->s);
+    c->vx.alloc((size_t)2 * n);
+    c->vy.alloc((size_t)2 * n);
+    c->red.alloc(kRedBlocks + 8);
+    HIPCHK(hipStreamSynchronize(c->s));
+    auto t1 = std::chrono::steady_clock::now();
+    build_lists(c);
+    auto t2 = std::chrono::steady_clock::now();
+    if (prm->flags & 4)
+        c->nleaf = (int32_t)c->tree.leaves.size();
+    else
+        build_precond(c);
+    auto t3 = std::chrono::steady_clock::now();
+    if (t_phase) {
+        t_phase[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t_phase[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        t_phase[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    }
+}
+
 void ctx_stats(tabi_ctx *c, int64_t *o)
 {
     o[0] = c->n;
@@ -4088,15 +4309,19 @@ struct RunSet {
     int R = 0, maxl = 0;
     hipStream_t s = nullptr;           // stream of the batched vector operations
     std::vector<int64_t> n;
+    std::vector<int64_t> gap;          // per run: second half's offset (0: contiguous)
     std::vector<double *> htab;
     DBuf<double *> tab;
-    DBuf<int64_t> dn;
+    DBuf<int64_t> dn;                  // n[0..R), then gap[0..R)
     int nslots() const { return kSlotV0 + maxl + 1; }
     double *ptr(int slot, int r) const { return htab[(size_t)slot * R + r]; }
     void upload_tables()
     {
         tab.upload(htab.data(), htab.size(), s);
-        dn.upload(n.data(), n.size(), s);
+        std::vector<int64_t> ng(n);
+        gap.resize(n.size(), 0);
+        ng.insert(ng.end(), gap.begin(), gap.end());
+        dn.upload(ng.data(), ng.size(), s);
     }
     virtual ~RunSet() = default;
     virtual void matvec(const std::vector<int32_t> &cols, const int32_t *dcols, int sin, int sout) = 0;
@@ -4200,8 +4425,11 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
         } else {
             for (int r = 0; r < R; ++r) {
                 std::vector<double> hx(rs.n[r]);
-                HIPCHK(hipMemcpyAsync(hx.data(), rs.ptr(kSlotX, r), sizeof(double) * hx.size(),
+                const size_t h = hx.size() / 2;
+                HIPCHK(hipMemcpyAsync(hx.data(), rs.ptr(kSlotX, r), sizeof(double) * h,
                                       hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(hx.data() + h, rs.ptr(kSlotX, r) + h + rs.gap[r],
+                                      sizeof(double) * (hx.size() - h), hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
                 double sabs = 0.0;
                 for (double v : hx) sabs += std::fabs(v);
@@ -4519,17 +4747,21 @@ void gmres_block_dev(tabi_ctx *c, int R, double *dB, double *dX, double tol, int
     gmres_runs(rs, c->blk_red, tol, nrmax, x0_zero, iter_out, err_out, ierr_out);
 }
 
+// Right-hand side of the faces [f0, f0 + nf) (default: all of the context's):
+// b[f0 + i] and b[ho + f0 + i], ho = the context's n (a packed context's
+// vectors hold every surface's first halves, then every second half).
 void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double eps_p, double *db,
-             bool dev = false)
+             bool dev = false, int f0 = 0, int nf = -1, bool sync = true)
 {
+    if (nf < 0) nf = c->n;
     DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
     DBuf<double4> qrec;
     qrec.alloc(std::max(nq, 1));
     if (nq > 0) k_charge_rec<<<nblk(nq, 256), 256, 0, c->s>>>(nq, dqp.p, dq.p, qrec.p);
-    k_rhs<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, nq, qrec.p, eps_p, c->tx.p, c->ty.p,
-        c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, db);
+    k_rhs<<<nblk(nf, 256), 256, 0, c->s>>>(nf, nq, qrec.p, eps_p, c->tx.p + f0, c->ty.p + f0,
+        c->tz.p + f0, c->tnx.p + f0, c->tny.p + f0, c->tnz.p + f0, db + f0, c->n);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->s));
+    if (sync) HIPCHK(hipStreamSynchronize(c->s));
 }
 
 // Many surfaces, one context each (own tree, lists and factors), one GMRES run
@@ -4644,6 +4876,68 @@ struct MultiRuns : RunSet {
             }
         }
         k_copyt<<<grid, 256, 0, s>>>(tab.p, R, kSlotT2, sout, dn.p, dcols);
+    }
+};
+
+// One GMRES run per surface of a packed context: every slot is one packed
+// vector (2 * c->n doubles, padding zero), run r's vector is its surface's two
+// segments in it, and one product or preconditioner solve serves every listed
+// surface at once: the far, near and leaf-solve tasks of the other surfaces
+// exit at once (the context's gate flags), leaving their entries stale, which
+// nothing reads again (no interaction crosses surfaces).
+struct PackedRuns : RunSet {
+    tabi_ctx *c;
+    DBuf<double> store;
+    std::vector<double *> base;         // per slot: the packed vector
+    DBuf<int32_t> dgate;
+    std::vector<int32_t> hgate;         // flags last uploaded
+    void set_gate(const std::vector<int32_t> &cols)
+    {
+        std::vector<int32_t> g(R, 0);
+        for (int r : cols) g[r] = 1;
+        if (g != hgate || !dgate.p) {
+            hgate = g;
+            dgate.upload(hgate.data(), (size_t)R, s);
+        }
+        c->gate = dgate.p;
+    }
+    ~PackedRuns() override { c->gate = nullptr; }
+    PackedRuns(tabi_ctx *c_, int maxl_, double *dB, double *dX) : c(c_)
+    {
+        R = c->nsurf();
+        maxl = maxl_;
+        s = c->s;
+        const int64_t L = 2 * (int64_t)c->n;
+        for (int r = 0; r < R; ++r) maxl = (int)std::min<int64_t>(maxl, 2 * (int64_t)c->sf_n[r]);
+        const int nown = nslots() - 2;  // every slot but B and X
+        store.alloc((size_t)L * nown);
+        HIPCHK(hipMemsetAsync(store.p, 0, sizeof(double) * (size_t)L * nown, s));
+        base.assign(nslots(), nullptr);
+        int k = 0;
+        for (int sl = 0; sl < nslots(); ++sl) {
+            if (sl == kSlotB) base[sl] = dB;
+            else if (sl == kSlotX) base[sl] = dX;
+            else base[sl] = store.p + (size_t)L * k++;
+        }
+        n.resize(R);
+        gap.resize(R);
+        htab.assign((size_t)nslots() * R, nullptr);
+        for (int r = 0; r < R; ++r) {
+            n[r] = 2 * (int64_t)c->sf_n[r];
+            gap[r] = (int64_t)c->n - c->sf_n[r];
+            for (int sl = 0; sl < nslots(); ++sl) htab[(size_t)sl * R + r] = base[sl] + c->sf_face0[r];
+        }
+        upload_tables();
+    }
+    void matvec(const std::vector<int32_t> &cols, const int32_t *, int sin, int sout) override
+    {
+        set_gate(cols);
+        matvec_dev(c, base[sin], base[sout]);
+    }
+    void msolve(const std::vector<int32_t> &cols, const int32_t *, int sin, int sout) override
+    {
+        set_gate(cols);
+        msolve_dev(c, base[sin], base[sout]);
     }
 };
 
@@ -4803,16 +5097,21 @@ struct EnergyScratch {
 };
 
 // Solvation and Coulomb energies (src/main.f90:144-157) from device inputs,
-// with the scratch at w (EnergyScratch layout); returns after completion.
+// with the scratch at w (EnergyScratch layout of nf faces); returns after
+// completion.  Faces [f0, f0 + nf) of the context (default all); with
+// weights false the source records already hold the solution's weights (a
+// packed context forms them for every surface at once).
 void energy_core(tabi_ctx *c, int nq, const double *dqp, const double *dq, double eps_p,
-                 const double *dxs, double *w, double *sol, double *cou)
+                 const double *dxs, double *w, double *sol, double *cou, int f0 = 0,
+                 int nf = -1, bool weights = true)
 {
     const double para = 332.0716;  // src/main.f90:64
-    const EnergyScratch L(c->n, nq);
+    if (nf < 0) nf = c->n;
+    const EnergyScratch L(nf, nq);
     double *part = w + L.part, *dptl = w + L.ptl, *cpart = w + L.cpart;
-    k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, dxs, c->src8.p);
-    k_energy<<<L.ngq * L.nch, 64, 0, c->s>>>(c->n, nq, L.nch, dqp, c->src8.p, c->prm.eps,
-                                             c->prm.kappa, part);
+    if (weights) k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, dxs, c->src8.p);
+    k_energy<<<L.ngq * L.nch, 64, 0, c->s>>>(nf, nq, L.nch, dqp, c->src8.p + (size_t)8 * f0,
+                                             c->prm.eps, c->prm.kappa, part);
     k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, L.nch, part, dptl);
     k_coulomb<<<dim3(L.ncb, L.ncb), 256, 0, c->s>>>(nq, dqp, dq, cpart);
     k_sum_part<<<1, 256, 0, c->s>>>(L.ncp, cpart, cpart + L.ncp);
@@ -5139,6 +5438,18 @@ int tabi_ctx_create(const tabi_params *prm, int32_t n, const double *xyz, const 
     });
 }
 
+int tabi_ctx_create_packed(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
+                           const double *const *xyz, const double *const *nrm,
+                           const double *const *area, int32_t *face0_out, tabi_ctx **out)
+{
+    return guarded([&] {
+        auto c = std::make_unique<tabi_ctx>();
+        ctx_setup_packed(c.get(), prm, nsurf, nface, xyz, nrm, area);
+        if (face0_out) std::copy(c->sf_face0.begin(), c->sf_face0.end(), face0_out);
+        *out = c.release();
+    });
+}
+
 void tabi_ctx_destroy(tabi_ctx *c)
 {
     if (!c) return;
@@ -5273,6 +5584,67 @@ int tabi_ctx_solve_block(tabi_ctx *c, int32_t nrhs, int32_t nq, const double *qp
     });
 }
 
+// Many surfaces packed into one context (ctx_setup_packed): one set-up (the
+// trees from host threads, then one list build and one batched leaf LU over
+// every surface), right-hand sides per surface, the GMRES runs of all surfaces
+// in lockstep over the packed operator, energies per surface.
+void solve_many_packed(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
+                       const double *const *xyz, const double *const *nrm,
+                       const double *const *area, const int32_t *nq, const double *const *qpos,
+                       const double *const *q, double eps_p, double tol, int32_t options,
+                       int32_t *iter, double *err, int32_t *ierr, double *soleng, double *couleng)
+{
+    (void)options;
+    auto t0 = std::chrono::steady_clock::now();
+    tabi_ctx c;
+    struct StreamGuard {
+        tabi_ctx *c;
+        ~StreamGuard()
+        {
+            if (c->s) { hipStream_t st = c->s; c->s = nullptr; destroy_stream(st); }
+        }
+    } guard{&c};
+    ctx_setup_packed(&c, prm, nsurf, nface, xyz, nrm, area, g_many + 6);
+    auto tr = std::chrono::steady_clock::now();
+    PoolScope ps_(c.s);
+    const int64_t L = 2 * (int64_t)c.n;
+    DBuf<double> bx;
+    bx.alloc((size_t)2 * L);
+    double *dB = bx.p, *dX = bx.p + L;
+    HIPCHK(hipMemsetAsync(bx.p, 0, sizeof(double) * 2 * L, c.s));
+    for (int k = 0; k < nsurf; ++k)
+        rhs_dev(&c, nq[k], qpos[k], q[k], eps_p, dB, false, c.sf_face0[k], nface[k], false);
+    HIPCHK(hipStreamSynchronize(c.s));
+    auto t1 = std::chrono::steady_clock::now();
+    std::vector<int> it(nsurf), ie(nsurf);
+    {
+        PackedRuns rs(&c, 10, dB, dX);
+        gmres_runs(rs, c.blk_red, tol, 10, true, it.data(), err, ie.data());
+        g_many[0] = 0.0;
+        g_many[1] = 0.0;
+    }
+    HIPCHK(hipStreamSynchronize(c.s));
+    auto t2 = std::chrono::steady_clock::now();
+    k_src_weights<<<nblk(c.n, 256), 256, 0, c.s>>>(c.n, c.tarea.p, dX, c.src8.p);
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < nsurf; ++k) {
+        DevIn dqp(qpos[k], (size_t)3 * nq[k], false, c.s), dq(q[k], nq[k], false, c.s);
+        DBuf<double> w;
+        w.alloc((size_t)EnergyScratch(nface[k], nq[k]).total);
+        energy_core(&c, nq[k], dqp.p, dq.p, eps_p, dX, w.p, soleng + k, couleng + k, c.sf_face0[k],
+                    nface[k], false);
+        iter[k] = it[k];
+        ierr[k] = ie[k];
+    }
+    auto t3 = std::chrono::steady_clock::now();
+    auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    g_many[2] = ms(t0, t1);
+    g_many[3] = ms(t1, t2);
+    g_many[4] = ms(t2, t3);
+    g_many[5] = ms(t0, t3);
+    g_many[9] = ms(tr, t1);
+}
+
 int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
                     const double *const *xyz, const double *const *nrm,
                     const double *const *area, const int32_t *nq, const double *const *qpos,
@@ -5283,6 +5655,11 @@ int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
         if (nsurf < 1) throw Error("need at least one surface");
         for (int k = 0; k < nsurf; ++k)
             if (nq[k] < 1) throw Error("surface " + std::to_string(k) + " has no charges");
+        if (!(options & 2)) {
+            solve_many_packed(prm, nsurf, nface, xyz, nrm, area, nq, qpos, q, eps_p, tol, options,
+                              iter, err, ierr, soleng, couleng);
+            return;
+        }
         auto t0 = std::chrono::steady_clock::now();
         std::vector<std::unique_ptr<tabi_ctx>> ctx(nsurf);
         std::vector<std::unique_ptr<DBuf<double>>> bx(nsurf);
@@ -5370,6 +5747,12 @@ int tabi_solve_many(const tabi_params *prm, int32_t nsurf, const int32_t *nface,
 int tabi_last_solve_many_stats(double *out6)
 {
     for (int i = 0; i < 6; ++i) out6[i] = g_many[i];
+    return 0;
+}
+
+int tabi_last_solve_many_setup(double *out4)
+{
+    for (int i = 0; i < 4; ++i) out4[i] = g_many[6 + i];
     return 0;
 }
 

@@ -25,7 +25,8 @@ EXPORTS = (
     "tabi_ws_param_count", "tabi_ws_create", "tabi_ws_destroy", "tabi_ws_get_params",
     "tabi_ctx_ws_features", "tabi_ws_predict", "tabi_ws_grad", "tabi_ws_train",
     "tabi_ctx_gmres_ws", "tabi_solve_ws", "tabi_ctx_matvec_block", "tabi_ctx_msolve_block",
-    "tabi_ctx_gmres_block", "tabi_ctx_solve_block", "tabi_solve_many",
+    "tabi_ctx_gmres_block", "tabi_ctx_solve_block", "tabi_solve_many", "tabi_ctx_create_packed",
+    "tabi_last_solve_many_setup",
     "tabi_last_solve_many_stats", "tabi_solve_dev", "tabi_vertex_potential",
     "tabi_ctx_vertex_potential", "tabi_write_surface_potential", "tabi_ctx_energy_noalloc",
 )
@@ -494,10 +495,12 @@ def solve(prm: Params, xyz, nrm, area, qpos, q, eps_p, tol=1e-4, model: WsModel 
                 times_ms=dict(zip(SOLVE_PHASES, t)))
 
 
-def solve_many(prm: Params, surfaces, eps_p, tol=1e-4, graph=True):
+def solve_many(prm: Params, surfaces, eps_p, tol=1e-4, graph=True, packed=True):
     """Solves several surfaces together (tabi_solve_many).  surfaces: sequence
-    of (xyz, nrm, area, qpos, q) in original face order.  Returns a dict of
-    per-surface arrays (iter, err, ierr, soleng, couleng) and stats."""
+    of (xyz, nrm, area, qpos, q) in original face order.  packed (default): one
+    context over all surfaces; otherwise one context per surface with a graph
+    branch each (graph=False: launched eagerly).  Returns a dict of per-surface
+    arrays (iter, err, ierr, soleng, couleng) and stats."""
     lib = load()
     k = len(surfaces)
     keep = []
@@ -520,13 +523,59 @@ def solve_many(prm: Params, surfaces, eps_p, tol=1e-4, graph=True):
     err, sol, cou = np.zeros(k), np.zeros(k), np.zeros(k)
     _check(lib.tabi_solve_many(C.byref(prm), C.c_int32(k), _p(nface), ptr["xyz"], ptr["nrm"],
                                ptr["area"], _p(nq), ptr["qpos"], ptr["q"], C.c_double(eps_p),
-                               C.c_double(tol), C.c_int32(0 if graph else 1), _p(it), _p(err),
+                               C.c_double(tol),
+                               C.c_int32((0 if graph else 1) | (0 if packed else 2)), _p(it), _p(err),
                                _p(ierr), _p(sol), _p(cou)))
     st = np.zeros(6)
     lib.tabi_last_solve_many_stats(_p(st))
-    return dict(iter=it, err=err, ierr=ierr, soleng=sol, couleng=cou,
-                stats=dict(zip(("captures", "graph_launches", "setup_ms", "gmres_ms", "energy_ms",
-                                "total_ms"), st)))
+    stats = dict(zip(("captures", "graph_launches", "setup_ms", "gmres_ms", "energy_ms", "total_ms"),
+                     (float(v) for v in st)))
+    if packed:
+        su = np.zeros(4)
+        lib.tabi_last_solve_many_setup(_p(su))
+        stats.update(zip(("tree_ms", "lists_ms", "precond_ms", "rhs_ms"), (float(v) for v in su)))
+    return dict(iter=it, err=err, ierr=ierr, soleng=sol, couleng=cou, stats=stats)
+
+
+class PackedSurfaces(Surface):
+    """Several surfaces in one context (tabi_ctx_create_packed): surface k's
+    faces sit at face0[k] .. face0[k] + n_k of a common tree-order numbering of
+    length n (padding between surfaces); matvec / msolve act on packed vectors
+    [first halves; second halves] of length 2n.  pack() / unpack() convert
+    from / to per-surface tree-order vectors [phi_k; dphi_k/dn]."""
+
+    def __init__(self, prm: Params, surfaces):
+        lib = load()
+        k = len(surfaces)
+        keep = []
+        ptr = {name: (C.c_void_p * k)() for name in ("xyz", "nrm", "area")}
+        nface = np.zeros(k, dtype=np.int32)
+        for i, (xyz, nrm, area) in enumerate(surfaces):
+            arrs = dict(xyz=_d(xyz, (-1, 3)), nrm=_d(nrm, (-1, 3)), area=_d(area))
+            keep.append(arrs)
+            nface[i] = arrs["xyz"].shape[0]
+            for name, a in arrs.items():
+                ptr[name][i] = a.ctypes.data
+        self.face0 = np.zeros(k + 1, dtype=np.int32)
+        self.nface = nface
+        self.prm = prm
+        h = C.c_void_p()
+        _check(lib.tabi_ctx_create_packed(C.byref(prm), C.c_int32(k), _p(nface), ptr["xyz"],
+                                          ptr["nrm"], ptr["area"], _p(self.face0), C.byref(h)))
+        self._h = h
+        self.n = int(self.face0[-1])
+
+    def pack(self, vecs):
+        out = np.zeros(2 * self.n)
+        for k, v in enumerate(vecs):
+            f0, nk = int(self.face0[k]), int(self.nface[k])
+            out[f0:f0 + nk] = v[:nk]
+            out[self.n + f0:self.n + f0 + nk] = v[nk:]
+        return out
+
+    def unpack(self, v):
+        return [np.concatenate([v[f0:f0 + nk], v[self.n + f0:self.n + f0 + nk]])
+                for f0, nk in zip(self.face0[:-1].tolist(), self.nface.tolist())]
 
 
 class DeviceProblem:

@@ -119,13 +119,48 @@ def test_many_surfaces_match_single_solves(precision):
     surfs = [(m["xyz"], m["nrm"], m["area"], m["qpos"], m["q"]) for m in mols]
     print(f"[gpu] many surfaces faces={[m['xyz'].shape[0] for m in mols]}", flush=True)
     prm = WS.params_for(precision)
-    res = hip.solve_many(prm, surfs, WS.EPS_P, tol=1e-4, graph=True)
-    eager = hip.solve_many(prm, surfs, WS.EPS_P, tol=1e-4, graph=False)
+    res = hip.solve_many(prm, surfs, WS.EPS_P, tol=1e-4, graph=True, packed=False)
+    eager = hip.solve_many(prm, surfs, WS.EPS_P, tol=1e-4, graph=False, packed=False)
+    pk = hip.solve_many(prm, surfs, WS.EPS_P, tol=1e-4)
     assert res["stats"]["graph_launches"] > 0 and res["stats"]["captures"] >= 1
     for k, sfc in enumerate(surfs):
         r1 = hip.solve(prm, *sfc, WS.EPS_P, tol=1e-4)
+        # packed: every surface's arithmetic is its own context's, bit for bit
+        assert pk["iter"][k] == r1["iter"] and pk["ierr"][k] == 0
+        assert pk["err"][k] == r1["err"]
+        assert pk["soleng"][k] == r1["soleng"] and pk["couleng"][k] == r1["couleng"]
         assert res["iter"][k] == r1["iter"] and res["ierr"][k] == 0
         assert res["soleng"][k] == pytest.approx(r1["soleng"], rel=1e-10)
         assert res["couleng"][k] == pytest.approx(r1["couleng"], rel=1e-12)
         assert eager["iter"][k] == r1["iter"]
         assert eager["soleng"][k] == pytest.approx(r1["soleng"], rel=1e-10)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_packed_context_matvec_and_msolve_equal_own_contexts(precision):
+    """Four molecules packed into one context (one tree and list set per
+    surface, one launch per kernel for all): every surface's entries of the
+    packed product and preconditioner solve equal its own context's, bit for
+    bit, and the padding between surfaces stays zero."""
+    mols = [WS.molecule(60 + k, na, h=0.6) for k, na in enumerate((90, 40, 170, 60))]
+    print(f"[gpu] packed faces={[m['xyz'].shape[0] for m in mols]} precision={precision}", flush=True)
+    prm = WS.params_for(precision)
+    P = hip.PackedSurfaces(prm, [(m["xyz"], m["nrm"], m["area"]) for m in mols])
+    assert np.all(P.face0 % 64 == 0) and P.face0[0] == 0
+    assert np.all(np.diff(P.face0) >= P.nface)
+    rng = np.random.default_rng(3)
+    xs = [rng.standard_normal(2 * m["xyz"].shape[0]) for m in mols]
+    xp = P.pack(xs)
+    yp, zp = P.matvec(xp), P.msolve(xp)
+    Y, Z = P.unpack(yp), P.unpack(zp)
+    pad = np.ones(2 * P.n, dtype=bool)
+    for k, m in enumerate(mols):
+        f0, nk = int(P.face0[k]), int(P.nface[k])
+        pad[f0:f0 + nk] = False
+        pad[P.n + f0:P.n + f0 + nk] = False
+        S = hip.Surface(prm, m["xyz"], m["nrm"], m["area"])
+        assert np.array_equal(Y[k], S.matvec(xs[k])), k
+        assert np.array_equal(Z[k], S.msolve(xs[k])), k
+        S.close()
+    assert np.all(yp[pad] == 0.0)
+    P.close()
