@@ -312,6 +312,7 @@ struct tabi_ctx {
     std::vector<int32_t> sf_face0, sf_n, sf_node0, sf_group0;
     DBuf<int32_t> groot, gend;           // per group: root node, end of its surface's faces
     DBuf<int32_t> blk_sf;                // per leaf block: its surface
+    DBuf<int32_t> group_sf, chunk_sf, mom_sf;   // per target group / moment chunk / moment node
     // surfaces whose field and preconditioner work is done (device flags, one
     // per surface; null: all): far, near and leaf-solve tasks of a surface
     // whose flag is 0 exit at once, leaving its entries of the outputs stale
@@ -551,8 +552,10 @@ __global__ void __launch_bounds__(256) k_mom_raw(int n, const int32_t *__restric
     const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ tnx, const double *__restrict__ tny,
     const double *__restrict__ tnz, const double *__restrict__ tarea,
-    const double *__restrict__ x, double *__restrict__ part)
+    const double *__restrict__ x, double *__restrict__ part, const int32_t *__restrict__ gate,
+    const int32_t *__restrict__ chunk_sf)
 {
+    if (gate && !gate[chunk_sf[blockIdx.x]]) return;   // surface not being solved
     mom_raw_body<P>(blockIdx.x, n, chunk_node, chunk_beg, chunk_end, ngeo, tx, ty, tz, tnx, tny,
                     tnz, tarea, x, part);
 }
@@ -664,8 +667,9 @@ __device__ __forceinline__ void mom_final_body(int mi, const int32_t *__restrict
 template <int P>
 __global__ void __launch_bounds__(256) k_mom_final(const int32_t *__restrict__ mom_chunk_off,
     const double *__restrict__ part, double eps, double *__restrict__ mom_w,
-    float *__restrict__ mom_w32)
+    float *__restrict__ mom_w32, const int32_t *__restrict__ gate, const int32_t *__restrict__ mom_sf)
 {
+    if (gate && !gate[mom_sf[blockIdx.x]]) return;     // surface not being solved
     mom_final_body<P>(blockIdx.x, mom_chunk_off, part, eps, mom_w, mom_w32);
 }
 
@@ -760,10 +764,13 @@ __device__ __forceinline__ double rsq_nr(double r2)
 
 // per-matvec source weights in the source records: {.., A*phi, A*dphi/dn}
 __global__ void k_src_weights(int n, const double *__restrict__ tarea,
-                              const double *__restrict__ x, double *__restrict__ src8)
+                              const double *__restrict__ x, double *__restrict__ src8,
+                              const int32_t *__restrict__ gate = nullptr,
+                              const int32_t *__restrict__ group_sf = nullptr)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    if (gate && !gate[group_sf[j / kWave]]) return;    // surface not being solved
     const double a = tarea[j];
     src8[(size_t)j * 8 + 6] = a * x[j];
     src8[(size_t)j * 8 + 7] = a * x[n + j];
@@ -902,10 +909,12 @@ __global__ void k_src32_geom(int n, const int32_t *__restrict__ face_leafnode,
 }
 
 __global__ void k_src32_weights(int n, const double *__restrict__ tarea,
-                                const double *__restrict__ x, float *__restrict__ src32)
+                                const double *__restrict__ x, float *__restrict__ src32,
+                                const int32_t *__restrict__ gate, const int32_t *__restrict__ group_sf)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    if (gate && !gate[group_sf[j / kWave]]) return;    // surface not being solved
     const double a = tarea[j];
     src32[(size_t)j * 8 + 6] = (float)(a * x[j]);
     src32[(size_t)j * 8 + 7] = (float)(a * x[n + j]);
@@ -1107,7 +1116,8 @@ __global__ void k_fill_i32(int64_t n, int32_t v, int32_t *__restrict__ a)
 // group (TREE_COMPP_PB's combination, src/treecode3d_pb.f:1154-1155,1177-1178)
 __global__ void k_reduce(int n, NearSlots ns, const int32_t *__restrict__ far_task_off,
     const double2 *__restrict__ part_near, const double2 *__restrict__ part_far,
-    const double *__restrict__ x, double pre1, double pre2, double *__restrict__ y)
+    const double *__restrict__ x, double pre1, double pre2, double *__restrict__ y,
+    const int32_t *__restrict__ gate, const int32_t *__restrict__ group_sf)
 {
     // the group's item records are loaded by the wave's lanes together (one
     // round of dependent loads instead of one per item), then walked in order
@@ -1116,6 +1126,7 @@ __global__ void k_reduce(int n, NearSlots ns, const int32_t *__restrict__ far_ta
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = i / kWave, lane = i % kWave, w = threadIdx.x / kWave;
     if (g * kWave >= n) return;                  // the whole wave is past the end
+    if (gate && !gate[group_sf[g]]) return;      // surface not being solved
     const u64 below = (1ull << lane) - 1;
     double p1 = 0.0, p2 = 0.0;
     const int k0 = ns.off[g], k1 = ns.off[g + 1];
@@ -2894,8 +2905,9 @@ __device__ __forceinline__ int64_t vix(int64_t i, int64_t half, int64_t gap)
 // y += (sgn * coef[r*cs + cofs]) * x
 __global__ void k_axpyt(double *const *__restrict__ tab, int R, int sx, int sy,
     const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int cofs,
-    double sgn, const int32_t *__restrict__ act)
+    double sgn, const int32_t *__restrict__ act, const int32_t *__restrict__ nact = nullptr)
 {
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
     if (i0 >= nv[r]) return;
@@ -2907,20 +2919,26 @@ __global__ void k_axpyt(double *const *__restrict__ tab, int R, int sx, int sy,
 
 // x *= coef[r*cs + cofs]
 __global__ void k_scalt(double *const *__restrict__ tab, int R, int sx, const int64_t *__restrict__ nv,
-    const double *__restrict__ coef, int64_t cs, int cofs, const int32_t *__restrict__ act)
+    const double *__restrict__ coef, int64_t cs, int cofs, const int32_t *__restrict__ act,
+    const int32_t *__restrict__ nact = nullptr)
 {
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
     if (i < nv[r]) tab[(size_t)sx * R + r][vix(i, nv[r] >> 1, nv[R + r])] *= coef[(size_t)r * cs + cofs];
 }
 
 // out = sum_{k < nvv} coef[r*cs + k] * slot(sv0 + k)
+// (with nvr, run r combines nvr[r] vectors instead of nvv)
 __global__ void k_lincombt(double *const *__restrict__ tab, int R, int sv0, int nvv,
     const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int so,
-    const int32_t *__restrict__ act)
+    const int32_t *__restrict__ act, const int32_t *__restrict__ nact = nullptr,
+    const int32_t *__restrict__ nvr = nullptr)
 {
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
+    if (nvr) nvv = nvr[r];
     if (i0 >= nv[r]) return;
     const int64_t i = vix(i0, nv[r] >> 1, nv[R + r]);
     const double *cr = coef + (size_t)r * cs;
@@ -2961,18 +2979,30 @@ __device__ __forceinline__ double sum_partials(const double *p, int np, double *
 // where h is the dot of slot hs, summed from its partials.  Then the partials of
 // slot(sa) . y into slot o1 when o1 >= 0, and of y . y into slot o2 when
 // o2 >= 0.  Grid (kRedBlocks, columns).
+// With hcond (the re-orthogonalisation pass of DORTH, src/dgmres_dep.f:1386-1411),
+// the axpy is skipped when h + 0.001*c == h for h = hcond[r * hstride]: c is
+// then DORTH's TEM, and h the Hessenberg entry it would change.
+__device__ __forceinline__ bool reo_negligible(double h, double tem)
+{
+#pragma clang fp contract(off)
+    return h + 0.001 * tem == h;
+}
+
 __global__ void __launch_bounds__(256) k_axpy_dott(double *const *__restrict__ tab, int R, int sx,
     int sy, int sa, const int64_t *__restrict__ nv, int hs, const int32_t *__restrict__ act,
-    double *__restrict__ part, int64_t dvs, int o1, int o2)
+    double *__restrict__ part, int64_t dvs, int o1, int o2, const int32_t *__restrict__ nact = nullptr,
+    const double *__restrict__ hcond = nullptr, int64_t hstride = 0)
 {
     __shared__ double red[2][256];
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int col = blockIdx.y, r = act[col], t = threadIdx.x;
     double *y = tab[(size_t)sy * R + r];
     const double *x = tab[(size_t)(sx >= 0 ? sx : sy) * R + r];
     const double *a = tab[(size_t)(o1 >= 0 ? sa : sy) * R + r];
     const int64_t n = nv[r], hn = n >> 1, gp = nv[R + r];
     double *pr = part + (size_t)r * dvs * gridDim.x;
-    const double c = sx >= 0 ? -1.0 * sum_partials(pr + (size_t)hs * gridDim.x, gridDim.x, red[0]) : 0.0;
+    double c = sx >= 0 ? -1.0 * sum_partials(pr + (size_t)hs * gridDim.x, gridDim.x, red[0]) : 0.0;
+    if (hcond && sx >= 0 && reo_negligible(hcond[(size_t)r * hstride], c)) sx = -1;
     double s1 = 0.0, s2 = 0.0;
     // a thread's elements i0 + u*stride in order, kFuseU at a time: all loads
     // of a group are issued before its stores, so they are in flight together
@@ -3028,9 +3058,10 @@ struct CoefArg {
 __global__ void __launch_bounds__(256) k_lincomb_dott(double *const *__restrict__ tab, int R, int sv0,
     int nvv, const int64_t *__restrict__ nv, const double *__restrict__ coef, int64_t cs, int so,
     int csc, const int32_t *__restrict__ act, double *__restrict__ part, int64_t dvs, int o,
-    bool by_value, CoefArg ca)
+    bool by_value, CoefArg ca, const int32_t *__restrict__ nact = nullptr)
 {
     __shared__ double red[256];
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int col = blockIdx.y, r = act[col], t = threadIdx.x;
     const double *cr = by_value ? ca.v : coef + (size_t)r * cs;
     double *out = tab[(size_t)so * R + r];
@@ -3109,8 +3140,10 @@ __global__ void k_subt(double *const *__restrict__ tab, int R, int sa, int sb, i
 }
 
 __global__ void k_copyt(double *const *__restrict__ tab, int R, int sa, int so,
-    const int64_t *__restrict__ nv, const int32_t *__restrict__ act)
+    const int64_t *__restrict__ nv, const int32_t *__restrict__ act,
+    const int32_t *__restrict__ nact = nullptr)
 {
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
     if (i < nv[r]) {
@@ -3120,8 +3153,9 @@ __global__ void k_copyt(double *const *__restrict__ tab, int R, int sa, int so,
 }
 
 __global__ void k_fillt(double *const *__restrict__ tab, int R, int so, const int64_t *__restrict__ nv,
-    double v, const int32_t *__restrict__ act)
+    double v, const int32_t *__restrict__ act, const int32_t *__restrict__ nact = nullptr)
 {
+    if (nact && (int)blockIdx.y >= *nact) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = act[blockIdx.y];
     if (i < nv[r]) tab[(size_t)so * R + r][vix(i, nv[r] >> 1, nv[R + r])] = v;
@@ -3137,9 +3171,9 @@ void launch_moments(tabi_ctx *c, const double *x)
     if (c->nmom == 0) return;
     k_mom_raw<P><<<c->nchunk, 256, 0, c->s>>>(c->n, c->chunk_node.p, c->chunk_beg.p,
         c->chunk_end.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
-        c->tarea.p, x, c->mom_part.p);
+        c->tarea.p, x, c->mom_part.p, c->gate, c->chunk_sf.p);
     k_mom_final<P><<<c->nmom, 256, 0, c->s>>>(c->mom_chunk_off.p, c->mom_part.p, c->prm.eps,
-        c->mom_w.p, c->prm.precision == 1 ? c->mom_w32.p : nullptr);
+        c->mom_w.p, c->prm.precision == 1 ? c->mom_w32.p : nullptr, c->gate, c->mom_sf.p);
 }
 
 template <int P>
@@ -3204,12 +3238,14 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     if (c->n_near_task) {
         KTimer kt(c, 1);
         if (c->prm.precision == 1) {
-            k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src32.p);
+            k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src32.p, c->gate,
+                                                               c->group_sf.p);
             k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src32.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
                 (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p, c->gate);
         } else {
-            k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src8.p);
+            k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src8.p, c->gate,
+                                                             c->group_sf.p);
             k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src8.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
                 c->prm.eps, c->part_near.p, c->gate);
@@ -3220,7 +3256,7 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     {
         KTimer kt(c, 3);
         k_reduce<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, near_slots(c), c->far_task_off.p,
-            c->part_near.p, c->part_far.p, x, pre1, pre2, dy);
+            c->part_near.p, c->part_far.p, x, pre1, pre2, dy, c->gate, c->group_sf.p);
     }
     HIPCHK(hipGetLastError());
 }
@@ -3561,12 +3597,19 @@ void build_lists(tabi_ctx *c)
     c->n_far_task = (int32_t)ft.size();
 
     // moment nodes and their particle chunks
-    std::vector<int32_t> n2m(nn, -1), cnode, cbeg, cend, mco(1, 0);
+    std::vector<int32_t> n2m(nn, -1), cnode, cbeg, cend, mco(1, 0), msf;
     int32_t nm = 0;
     c->p_mom = 0;
+    // surface of each node and group (all 0 for a context of one surface)
+    std::vector<int32_t> ndsf(nn, 0), gsf(G, 0);
+    for (int sf = 0; sf < nsf; ++sf) {
+        for (int nd = node0[sf]; nd < node0[sf + 1]; ++nd) ndsf[nd] = sf;
+        for (int g = group0[sf]; g < group0[sf + 1]; ++g) gsf[g] = sf;
+    }
     for (int i = 0; i < nn; ++i) {
         if (!need[i]) continue;
         n2m[i] = nm++;
+        msf.push_back(ndsf[i]);
         c->p_mom += T.numpar[i];
         for (int b = T.ibeg[i]; b <= T.iend[i]; b += kMomChunk) {
             cnode.push_back(i);
@@ -3577,6 +3620,8 @@ void build_lists(tabi_ctx *c)
     }
     c->nmom = nm;
     c->nchunk = (int32_t)cnode.size();
+    std::vector<int32_t> csf(cnode.size());
+    for (size_t k = 0; k < cnode.size(); ++k) csf[k] = ndsf[cnode[k]];
 
     // every table goes up through one pinned arena, asynchronously
     const bool geom32 = c->prm.precision == 1;
@@ -3584,7 +3629,9 @@ void build_lists(tabi_ctx *c)
                             arena_bytes(sizeof(NearTask) * nt.size()) + arena_bytes(sizeof(FarTask) * ft.size()) +
                             arena_bytes(sizeof(int32_t) * fto.size()) + arena_bytes(sizeof(int32_t) * nn) +
                             3 * arena_bytes(sizeof(int32_t) * cnode.size()) + arena_bytes(sizeof(int32_t) * mco.size()) +
-                            arena_bytes(sizeof(int32_t) * T.leaves.size()) + 256;
+                            arena_bytes(sizeof(int32_t) * T.leaves.size()) + arena_bytes(sizeof(int32_t) * G) +
+                            arena_bytes(sizeof(int32_t) * csf.size()) + arena_bytes(sizeof(int32_t) * msf.size()) +
+                            256;
     PinnedArena up{pinned_up().get(up_bytes)};
     auto put = [&](auto &d, const auto &v) {
         using E = typename std::decay_t<decltype(v)>::value_type;
@@ -3611,6 +3658,9 @@ void build_lists(tabi_ctx *c)
     c->part_near.alloc((size_t)std::max(c->n_near_task, 1) * kWave);
     c->part_far.alloc((size_t)std::max(c->n_far_task, 1) * kWave);
     put(c->node2mom, n2m);
+    put(c->group_sf, gsf);
+    put(c->chunk_sf, csf);
+    put(c->mom_sf, msf);
     if (nm) {
         put(c->chunk_node, cnode);
         put(c->chunk_beg, cbeg);
@@ -4331,6 +4381,11 @@ struct RunSet {
         matvec(cols, dcols, sin, kSlotWK);
         msolve(cols, dcols, kSlotWK, sout);
     }
+    // Operators that can take the runs to work on from device flags (one per
+    // run, set by device code while a captured graph runs) instead of the host
+    // list: returns false where unsupported.  null restores host lists.
+    virtual bool device_gate_ok() const { return false; }
+    virtual bool use_device_gate(const int32_t *) { return false; }
 };
 
 // GMRES with the reference's settings for each run (gmres_dev), the runs in
@@ -4696,6 +4751,521 @@ void gmres_runs(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_ze
     }
 }
 
+// ---- GMRES with its control on the device (one captured graph per cycle) ----
+// The scalar work of gmres_runs -- the Hessenberg column, the conditional
+// re-orthogonalisation, the Givens rotations, DRLCAL's coefficients, the
+// stopping test, DHELS and the restart decision -- done by small device
+// kernels on per-run state in device memory, with exactly the host's
+// operations (no contraction, the same summation of the dot partials), so a
+// whole restart cycle (MAXL Arnoldi steps and the solution update) is one
+// captured graph that is launched once per cycle for every run at once.  The
+// runs each kernel works on come from device-built lists with device counts;
+// the operator skips the work of runs no longer iterating through device gate
+// flags; the host only waits for the count of runs still active after each
+// cycle.  Results are bit-identical to gmres_runs.
+constexpr int kGmMaxl = 32;    // largest MAXL of the device form
+enum : int { kLstC = 0, kLstA, kLstR, kLstG, kLstY, kLstN };
+enum : int { kFlA = 1, kFlReo = 2, kFlConv = 4, kFlStall = 8, kFlEarly = 16 };
+
+struct GmDev {
+    int R, maxl, ldh, cst, dvs, np, nrmax;
+    double tol;
+    double *bnrm, *r0nrm, *err, *prod, *rho, *snormw, *vnrm, *hes, *q, *coef;
+    const double *part;
+    int32_t *iter, *ierr, *lgmr, *iflag, *fl, *act, *gate, *nvv, *lst, *cnt;
+    double *hval;          // mapped host memory: number of runs still active
+    int64_t *hflag;        // mapped host memory: raised after each cycle
+    __device__ double *H(int r) const { return hes + (size_t)r * ldh * maxl; }
+    __device__ double *Q(int r) const { return q + (size_t)r * 2 * maxl; }
+    __device__ double *CF(int r) const { return coef + (size_t)r * cst; }
+    __device__ const double *P(int r, int slot) const { return part + ((size_t)r * dvs + slot) * np; }
+};
+
+// lists[kind] = the runs of lists[from] (in order) whose flags have bit `bit`,
+// or (from < 0) the runs r = 0..R-1 with act[r]; optionally gate[r] = membership
+__global__ void k_gm_list(GmDev g, int kind, int from, int bit, int set_gate)
+{
+    if (threadIdx.x != 0) return;
+    int *out = g.lst + (size_t)kind * g.R, n = 0;
+    if (set_gate)
+        for (int r = 0; r < g.R; ++r) g.gate[r] = 0;
+    const int cnt = from < 0 ? g.R : g.cnt[from];
+    for (int k = 0; k < cnt; ++k) {
+        const int r = from < 0 ? k : g.lst[(size_t)from * g.R + k];
+        const bool in = from < 0 ? g.act[r] != 0 : (g.fl[r] & bit) != 0;
+        if (!in) continue;
+        out[n++] = r;
+        if (set_gate) g.gate[r] = 1;
+    }
+    g.cnt[kind] = n;
+}
+
+// cycle start, per run of C (gm.f:1652-1700): r0nrm = |M^-1 r0|, early exit,
+// the first basis vector's scale, cleared Hessenberg matrix
+__global__ void __launch_bounds__(256) k_gm_start(GmDev g, int sT)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    if ((int)blockIdx.x >= g.cnt[kLstC]) return;
+    const int r = g.lst[(size_t)kLstC * g.R + blockIdx.x];
+    const double s = sum_partials(g.P(r, sT), g.np, red);
+    const double rn = sqrt(s);
+    const bool early = rn / g.bnrm[r] <= g.tol;
+    if (!early)
+        for (int k = threadIdx.x; k < g.ldh * g.maxl; k += blockDim.x) g.H(r)[k] = 0.0;
+    if (threadIdx.x != 0) return;
+    g.r0nrm[r] = rn;
+    g.err[r] = rn / g.bnrm[r];
+    g.lgmr[r] = 0;
+    g.iflag[r] = 0;
+    g.fl[r] = early ? kFlEarly : kFlA;
+    if (!early) {
+        g.CF(r)[0] = 1.0 / rn;
+        g.prod[r] = 1.0;
+        g.rho[r] = 0.0;
+        g.snormw[r] = 0.0;
+    }
+}
+
+// Arnoldi step ll, per run of A: the Hessenberg column from the modified
+// Gram-Schmidt dots, |w| before and after, and DORTH's test for the
+// re-orthogonalisation pass
+__global__ void __launch_bounds__(256) k_gm_hcol(GmDev g, int ll, int sN, int sS)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    if ((int)blockIdx.x >= g.cnt[kLstA]) return;
+    const int r = g.lst[(size_t)kLstA * g.R + blockIdx.x];
+    double *hcol = g.H(r) + (size_t)(ll - 1) * g.ldh;
+    for (int i = 0; i < ll; ++i) {
+        const double v = sum_partials(g.P(r, i), g.np, red);
+        if (threadIdx.x == 0) hcol[i] = v;
+    }
+    const double vn = sqrt(sum_partials(g.P(r, sN), g.np, red));
+    const double sw = sqrt(sum_partials(g.P(r, sS), g.np, red));
+    if (threadIdx.x != 0) return;
+    g.lgmr[r] = ll;
+    g.vnrm[r] = vn;
+    g.snormw[r] = sw;
+    if (!(vn + 0.001 * sw != vn)) g.fl[r] |= kFlReo;
+}
+
+// after the re-orthogonalisation chain, per run of the list R: DORTH's updates
+// of the column (TEM of each basis vector from its dot) and of |w|
+__global__ void __launch_bounds__(256) k_gm_reo_end(GmDev g, int ll)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    if ((int)blockIdx.x >= g.cnt[kLstR]) return;
+    const int r = g.lst[(size_t)kLstR * g.R + blockIdx.x];
+    double *hcol = g.H(r) + (size_t)(ll - 1) * g.ldh;
+    double sumdsq = 0.0;
+    for (int i = 0; i < ll; ++i) {
+        const double tem = -sum_partials(g.P(r, i), g.np, red);
+        if (threadIdx.x == 0 && !(hcol[i] + 0.001 * tem == hcol[i])) {
+            hcol[i] = hcol[i] - tem;
+            sumdsq += tem * tem;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (sumdsq != 0.0) {
+        const double sw = g.snormw[r];
+        g.snormw[r] = sqrt(fmax(0.0, sw * sw - sumdsq));
+    }
+    g.fl[r] &= ~kFlReo;
+}
+
+// Givens rotations of the new column (DHEQR, gm.f:1093-1125), the stall test,
+// the residual estimate and DRLCAL's coefficients (gm.f:1948-1977), one thread
+// per run of A; then the list G of runs that form DRLCAL's residual
+__global__ void __launch_bounds__(256) k_gm_givens(GmDev g, int ll)
+{
+#pragma clang fp contract(off)
+    const int na = g.cnt[kLstA];
+    for (int b = threadIdx.x; b < na; b += blockDim.x) {
+        const int r = g.lst[(size_t)kLstA * g.R + b];
+        double *hcol = g.H(r) + (size_t)(ll - 1) * g.ldh;
+        double *qq = g.Q(r);
+        hcol[ll] = g.snormw[r];
+        for (int k = 0; k < ll - 1; ++k) {
+            const double cc = qq[2 * k], ss = qq[2 * k + 1];
+            const double t1 = hcol[k], t2 = hcol[k + 1];
+            hcol[k] = cc * t1 - ss * t2;
+            hcol[k + 1] = ss * t1 + cc * t2;
+        }
+        const double t1 = hcol[ll - 1], t2 = hcol[ll];
+        double cc, ss;
+        if (t2 == 0.0) { cc = 1.0; ss = 0.0; }
+        else if (fabs(t2) >= fabs(t1)) {
+            const double t = t1 / t2;
+            ss = -1.0 / sqrt(1.0 + t * t);
+            cc = -ss * t;
+        } else {
+            const double t = t2 / t1;
+            cc = 1.0 / sqrt(1.0 + t * t);
+            ss = -cc * t;
+        }
+        qq[2 * (ll - 1)] = cc;
+        qq[2 * (ll - 1) + 1] = ss;
+        hcol[ll - 1] = cc * t1 - ss * t2;
+        if (hcol[ll - 1] == 0.0) { g.fl[r] = (g.fl[r] & ~kFlA) | kFlStall; continue; }
+        g.prod[r] = g.prod[r] * qq[2 * ll - 1];
+        g.rho[r] = fabs(g.prod[r] * g.r0nrm[r]);
+        double *cf = g.CF(r);
+        for (int k = 0; k <= ll; ++k) cf[k] = 0.0;
+        cf[0] = 1.0;
+        for (int i = 1; i <= ll - 1; ++i) {
+            const double s2 = qq[2 * i - 1], c2 = qq[2 * i - 2];
+            for (int k = 0; k < i; ++k) cf[k] *= s2;
+            cf[i] = c2;
+        }
+        const double s2 = qq[2 * ll - 1], c2 = qq[2 * ll - 2] / g.snormw[r];
+        for (int k = 0; k < ll; ++k) cf[k] *= s2;
+        cf[ll] = c2;
+        const double tem = g.r0nrm[r] * g.prod[r];
+        for (int k = 0; k <= ll; ++k) cf[k] *= tem;
+        cf[g.maxl + 1] = 1.0 / g.snormw[r];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int n = 0;
+    int *out = g.lst + (size_t)kLstG * g.R;
+    for (int b = 0; b < na; ++b) {
+        const int r = g.lst[(size_t)kLstA * g.R + b];
+        if (!(g.fl[r] & kFlStall)) out[n++] = r;
+    }
+    g.cnt[kLstG] = n;
+}
+
+// DRLCAL's residual norm and the stopping test (ITOL=2), per run of G
+__global__ void __launch_bounds__(256) k_gm_conv(GmDev g, int sT)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    if ((int)blockIdx.x >= g.cnt[kLstG]) return;
+    const int r = g.lst[(size_t)kLstG * g.R + blockIdx.x];
+    const double s = sum_partials(g.P(r, sT), g.np, red);
+    if (threadIdx.x != 0) return;
+    g.err[r] = sqrt(s) / g.bnrm[r];
+    if (g.err[r] <= g.tol) g.fl[r] = (g.fl[r] & ~kFlA) | kFlConv;
+}
+
+// end of a cycle, per run of C: the outcome (DPIGMR's IFLAG), DHELS's least
+// squares solution (gm.f:925-947) as the coefficients of the update z =
+// V y; then the list Y of runs to update
+__global__ void __launch_bounds__(256) k_gm_end(GmDev g)
+{
+#pragma clang fp contract(off)
+    const int nc = g.cnt[kLstC];
+    for (int b = threadIdx.x; b < nc; b += blockDim.x) {
+        const int r = g.lst[(size_t)kLstC * g.R + b];
+        const int f = g.fl[r];
+        int iflag;
+        if (f & kFlEarly) iflag = 0;
+        else if (f & kFlStall) iflag = 2;
+        else if (f & kFlConv) iflag = 0;
+        else iflag = (g.rho[r] < g.r0nrm[r]) ? 1 : 2;
+        g.iflag[r] = iflag;
+        g.nvv[r] = 0;
+        if ((f & kFlEarly) || iflag == 2) continue;
+        const int nn = g.lgmr[r];
+        double yls[kGmMaxl + 1];
+        for (int k = 0; k <= g.maxl; ++k) yls[k] = 0.0;
+        yls[0] = g.r0nrm[r];
+        const double *qq = g.Q(r);
+        for (int k = 0; k < nn; ++k) {
+            const double cc = qq[2 * k], ss = qq[2 * k + 1];
+            const double t1 = yls[k], t2 = yls[k + 1];
+            yls[k] = cc * t1 - ss * t2;
+            yls[k + 1] = ss * t1 + cc * t2;
+        }
+        const double *Hm = g.H(r);
+        for (int kb = 0; kb < nn; ++kb) {
+            const int k = nn - 1 - kb;
+            yls[k] = yls[k] / Hm[(size_t)k * g.ldh + k];
+            const double t = -yls[k];
+            for (int i = 0; i < k; ++i) yls[i] += t * Hm[(size_t)k * g.ldh + i];
+        }
+        double *cf = g.CF(r);
+        for (int k = 0; k < nn; ++k) cf[k] = yls[k];
+        cf[g.maxl + 1] = 1.0;
+        g.nvv[r] = nn;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int n = 0;
+    int *out = g.lst + (size_t)kLstY * g.R;
+    for (int b = 0; b < nc; ++b) {
+        const int r = g.lst[(size_t)kLstC * g.R + b];
+        if (g.nvv[r] > 0) out[n++] = r;
+    }
+    g.cnt[kLstY] = n;
+}
+
+// after the update x += z: iteration counts, IERR, the runs that restart, the
+// NRMAX cap (IERR=1, gm.f:821-850); the count of active runs goes to the host
+__global__ void k_gm_fin(GmDev g, int64_t seq)
+{
+    if (threadIdx.x != 0) return;
+    const int nc = g.cnt[kLstC];
+    int *nrsts = g.cnt + kLstN;
+    for (int b = 0; b < nc; ++b) {
+        const int r = g.lst[(size_t)kLstC * g.R + b];
+        g.iter[r] += g.lgmr[r];
+        if (g.iflag[r] == 1) continue;
+        g.ierr[r] = g.iflag[r] == 0 ? 0 : 2;
+        g.act[r] = 0;
+    }
+    *nrsts += 1;
+    int na = 0;
+    for (int r = 0; r < g.R; ++r) {
+        if (!g.act[r]) continue;
+        if (*nrsts > g.nrmax) { g.ierr[r] = 1; g.act[r] = 0; continue; }
+        ++na;
+    }
+    g.hval[0] = (double)na;
+    __threadfence_system();
+    __hip_atomic_store(g.hflag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The runs of rs from their initial residual state (DL = M^-1 r0, X = x0,
+// bnrm = |M^-1 b|, iter/ierr zero) to the end, cycle by cycle through one
+// captured graph; returns false (nothing done) when rs cannot take device gates.
+bool gmres_graph_cycles(RunSet &rs, DBuf<double> &red, double tol, int nrmax,
+                        const std::vector<double> &bnrm, int *iter_out, double *err_out,
+                        int *ierr_out, int64_t *launches)
+{
+    const int R = rs.R, maxl = rs.maxl, ldh = maxl + 1;
+    if (maxl > kGmMaxl || maxl < 1) return false;
+    hipStream_t st = rs.s;
+    const int64_t cst = maxl + 2, dvs = ldh + 4;
+    const int sN = ldh, sS = ldh + 1, sT = ldh + 2;
+    int64_t nmax = 0;
+    for (int64_t v : rs.n) nmax = std::max(nmax, v);
+    const unsigned nb = nblk(nmax, 256);
+    DBuf<double> dd;
+    DBuf<int32_t> di;
+    const size_t nd = (size_t)R * (7 + ldh * maxl + 2 * maxl + cst);
+    const size_t ni = (size_t)R * (8 + kLstN) + kLstN + 1;
+    dd.alloc(nd);
+    di.alloc(ni);
+    red.alloc((size_t)R * dvs * kRedBlocks);
+    GmDev g{};
+    g.R = R; g.maxl = maxl; g.ldh = ldh; g.cst = (int)cst; g.dvs = (int)dvs; g.np = kRedBlocks;
+    g.nrmax = nrmax; g.tol = tol;
+    double *p = dd.p;
+    g.bnrm = p; p += R; g.r0nrm = p; p += R; g.err = p; p += R; g.prod = p; p += R;
+    g.rho = p; p += R; g.snormw = p; p += R; g.vnrm = p; p += R;
+    g.hes = p; p += (size_t)R * ldh * maxl; g.q = p; p += (size_t)R * 2 * maxl; g.coef = p;
+    int32_t *ip = di.p;
+    g.iter = ip; ip += R; g.ierr = ip; ip += R; g.lgmr = ip; ip += R; g.iflag = ip; ip += R;
+    g.fl = ip; ip += R; g.act = ip; ip += R; g.gate = ip; ip += R; g.nvv = ip; ip += R;
+    g.lst = ip; ip += (size_t)R * kLstN; g.cnt = ip;
+    g.part = red.p;
+    // initial state: every run active, counters zero
+    {
+        std::vector<double> hb(bnrm);
+        std::vector<int32_t> hi(ni, 0);
+        for (int r = 0; r < R; ++r) hi[(size_t)5 * R + r] = 1;   // act
+        HIPCHK(hipMemcpyAsync(g.bnrm, hb.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(di.p, hi.data(), sizeof(int32_t) * ni, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    PinnedScalars &ps = pinned_scalars(std::max<size_t>(1, (size_t)R * dvs), std::max(R, 1), (size_t)R * cst * kCoefRing);
+    g.hval = ps.dval;
+    g.hflag = ps.dflag;
+    if (!rs.use_device_gate(g.gate)) return false;
+    double *const *T = rs.tab.p;
+    const int64_t *NV = rs.dn.p;
+    auto V = [&](int k) { return kSlotV0 + k; };
+    const int32_t *LC = g.lst + (size_t)kLstC * R, *LA = g.lst + (size_t)kLstA * R,
+                  *LR = g.lst + (size_t)kLstR * R, *LG = g.lst + (size_t)kLstG * R,
+                  *LY = g.lst + (size_t)kLstY * R;
+    const int32_t *nC = g.cnt + kLstC, *nA = g.cnt + kLstA, *nR = g.cnt + kLstR,
+                  *nG = g.cnt + kLstG, *nY = g.cnt + kLstY;
+    const dim3 gv(nb, (unsigned)R), gr(kRedBlocks, (unsigned)R);
+    const std::vector<int32_t> none;
+    int64_t seq0 = ps.seq;
+    // one cycle of launches (captured once, replayed per cycle); the flag
+    // value it raises is read from a device counter-free sequence: each replay
+    // raises seq0 + 1 and the host resets the flag before the next
+    auto cycle = [&]() {
+        k_gm_list<<<1, 64, 0, st>>>(g, kLstC, -1, 0, 0);
+        k_copyt<<<gv, 256, 0, st>>>(T, R, kSlotDL, kSlotR, NV, LC, nC);
+        k_fillt<<<gv, 256, 0, st>>>(T, R, kSlotZ, NV, 0.0, LC, nC);
+        k_copyt<<<gv, 256, 0, st>>>(T, R, kSlotR, V(0), NV, LC, nC);
+        k_axpy_dott<<<gr, 256, 0, st>>>(T, R, -1, V(0), V(0), NV, 0, LC, red.p, dvs, sT, -1, nC);
+        k_gm_start<<<R, 256, 0, st>>>(g, sT);
+        k_gm_list<<<1, 64, 0, st>>>(g, kLstA, kLstC, kFlA, 1);
+        k_scalt<<<gv, 256, 0, st>>>(T, R, V(0), NV, g.coef, cst, 0, LA, nA);
+        for (int ll = 1; ll <= maxl; ++ll) {
+            rs.apply(none, LA, V(ll - 1), V(ll));
+            k_axpy_dott<<<gr, 256, 0, st>>>(T, R, -1, V(ll), V(0), NV, 0, LA, red.p, dvs, 0, sN, nA);
+            for (int i = 1; i <= ll; ++i)
+                k_axpy_dott<<<gr, 256, 0, st>>>(T, R, V(i - 1), V(ll), V(i), NV, i - 1, LA, red.p, dvs,
+                                                i < ll ? i : -1, i < ll ? -1 : sS, nA);
+            k_gm_hcol<<<R, 256, 0, st>>>(g, ll, sN, sS);
+            // DORTH's second pass for the runs that need it (rare): dot i,
+            // then the conditional axpy of i fused with dot i + 1
+            k_gm_list<<<1, 64, 0, st>>>(g, kLstR, kLstA, kFlReo, 0);
+            k_axpy_dott<<<gr, 256, 0, st>>>(T, R, -1, V(ll), V(0), NV, 0, LR, red.p, dvs, 0, -1, nR);
+            for (int i = 1; i <= ll; ++i)
+                k_axpy_dott<<<gr, 256, 0, st>>>(T, R, V(i - 1), V(ll), V(i), NV, i - 1, LR, red.p, dvs,
+                                                i < ll ? i : -1, -1, nR,
+                                                g.hes + (size_t)(ll - 1) * ldh + (i - 1), (int64_t)ldh * maxl);
+            k_gm_reo_end<<<R, 256, 0, st>>>(g, ll);
+            k_gm_givens<<<1, 256, 0, st>>>(g, ll);
+            k_lincomb_dott<<<gr, 256, 0, st>>>(T, R, V(0), ll + 1, NV, g.coef, cst, kSlotDL,
+                                               ll < maxl ? maxl + 1 : -1, LG, red.p, dvs, sT, false,
+                                               CoefArg{}, nG);
+            k_gm_conv<<<R, 256, 0, st>>>(g, sT);
+            k_gm_list<<<1, 64, 0, st>>>(g, kLstA, kLstA, kFlA, 1);
+        }
+        k_gm_end<<<1, 256, 0, st>>>(g);
+        k_lincombt<<<gv, 256, 0, st>>>(T, R, V(0), 0, NV, g.coef, cst, kSlotZ, LY, nY, g.nvv);
+        k_axpyt<<<gv, 256, 0, st>>>(T, R, kSlotZ, kSlotX, NV, g.coef, cst, maxl + 1, 1.0, LY, nY);
+        k_gm_fin<<<1, 64, 0, st>>>(g, seq0 + 1);
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        cycle();
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        rs.use_device_gate(nullptr);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        rs.use_device_gate(nullptr);
+        HIPCHK(ie);
+    }
+    int64_t nl = 0;
+    try {
+        for (;;) {
+            __atomic_store_n(ps.flag, seq0, __ATOMIC_RELEASE);
+            HIPCHK(hipGraphLaunch(exec, st));
+            ++nl;
+            wait_flags(ps.flag, std::vector<int32_t>{0}, seq0 + 1, st);
+            if (ps.val[0] == 0.0) break;
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        ps.seq = seq0 + 1;
+        (void)hipGraphExecDestroy(exec);
+        rs.use_device_gate(nullptr);
+        throw;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    ps.seq = seq0 + 1;                   // the flag now holds this value
+    (void)hipGraphExecDestroy(exec);
+    rs.use_device_gate(nullptr);
+    std::vector<int32_t> it(R), ier(R);
+    HIPCHK(hipMemcpy(it.data(), g.iter, sizeof(int32_t) * R, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ier.data(), g.ierr, sizeof(int32_t) * R, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(err_out, g.err, sizeof(double) * R, hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; ++r) { iter_out[r] = it[r]; ierr_out[r] = ier[r]; }
+    if (launches) *launches = nl;
+    return true;
+}
+
+// gmres_runs with the cycles on the device (gmres_graph_cycles): the host
+// forms each run's initial state as gmres_runs does -- M^-1 b and its norm,
+// the initial guess's preconditioned residual M^-1 (b - A x0), the fallback
+// to x0 = 0 when that residual is the larger -- and leaves M^-1 r0 in DL.
+// Returns false (nothing done) for run sets without device gates or MAXL
+// above kGmMaxl.
+bool gmres_graph(RunSet &rs, DBuf<double> &red, double tol, int nrmax, bool x0_zero, int *iter_out,
+                 double *err_out, int *ierr_out, bool fallback = false, int *x0_used = nullptr,
+                 int64_t *launches = nullptr)
+{
+    const int R = rs.R, maxl = rs.maxl, ldh = maxl + 1;
+    if (!rs.device_gate_ok() || maxl > kGmMaxl || maxl < 1) return false;
+    hipStream_t st = rs.s;
+    PoolScope ps_(st);
+    const int64_t dvs = ldh + 4;
+    const int sT = ldh + 2;
+    int64_t nmax = 0;
+    for (int64_t v : rs.n) nmax = std::max(nmax, v);
+    const unsigned nb = nblk(nmax, 256);
+    red.alloc((size_t)R * dvs * kRedBlocks);
+    double *const *T = rs.tab.p;
+    const int64_t *NV = rs.dn.p;
+    PinnedScalars &ps = pinned_scalars((size_t)R * dvs, R, (size_t)kCoefRing * R * (maxl + 2));
+    auto gy = [&](size_t cnt) { return dim3(nb, (unsigned)cnt); };
+    auto dots = [&](const std::vector<int32_t> &cols, const int32_t *dcols, int sa, int sb) {
+        k_axpy_dott<<<dim3(kRedBlocks, (unsigned)cols.size()), 256, 0, st>>>(T, R, -1, sb, sa, NV, 0,
+                                                                             dcols, red.p, dvs, sT, -1);
+        ++ps.seq;
+        k_sum_slots<<<(unsigned)cols.size(), 256, 0, st>>>(kRedBlocks, red.p, ps.dval, ps.dflag, ps.seq,
+                                                            dvs, sT, sT + 1, -1, -1, dcols);
+        HIPCHK(hipGetLastError());
+        wait_flags(ps.flag, cols, ps.seq, st);
+    };
+    std::vector<int32_t> all(R);
+    for (int r = 0; r < R; ++r) all[r] = r;
+    DBuf<int32_t> dall, dsub;
+    dall.upload(all.data(), R, st);
+    rs.msolve(all, dall.p, kSlotB, kSlotR);
+    dots(all, dall.p, kSlotR, kSlotR);
+    std::vector<double> bnrm(R);
+    for (int r = 0; r < R; ++r) bnrm[r] = std::sqrt(ps.val[(size_t)r * dvs + sT]);
+    k_copyt<<<gy(R), 256, 0, st>>>(T, R, kSlotR, kSlotDL, NV, dall.p);
+    std::vector<int32_t> guessed;
+    if (!x0_zero) {
+        for (int r = 0; r < R; ++r) {
+            std::vector<double> hx(rs.n[r]);
+            const size_t h = hx.size() / 2;
+            HIPCHK(hipMemcpyAsync(hx.data(), rs.ptr(kSlotX, r), sizeof(double) * h, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(hx.data() + h, rs.ptr(kSlotX, r) + h + rs.gap[r],
+                                  sizeof(double) * (hx.size() - h), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            double sabs = 0.0;
+            for (double v : hx) sabs += std::fabs(v);
+            if (!(sabs < 1.0e-10)) guessed.push_back(r);
+        }
+    }
+    std::vector<int> used(R, 0);
+    if (!guessed.empty()) {
+        // M^-1 (b - A x0) into Z, kept in DL unless it is worse than M^-1 b
+        dsub.upload(guessed.data(), guessed.size(), st);
+        rs.matvec(guessed, dsub.p, kSlotX, kSlotR);
+        k_subt<<<gy(guessed.size()), 256, 0, st>>>(T, R, kSlotB, kSlotR, kSlotR, NV, dsub.p);
+        k_copyt<<<gy(guessed.size()), 256, 0, st>>>(T, R, kSlotR, kSlotWK, NV, dsub.p);
+        rs.msolve(guessed, dsub.p, kSlotWK, kSlotZ);
+        dots(guessed, dsub.p, kSlotZ, kSlotZ);
+        std::vector<int32_t> keep, drop;
+        for (int r : guessed) {
+            const double r0 = std::sqrt(ps.val[(size_t)r * dvs + sT]);
+            (fallback && !(r0 <= bnrm[r]) ? drop : keep).push_back(r);
+        }
+        if (!keep.empty()) {
+            DBuf<int32_t> dk;
+            dk.upload(keep.data(), keep.size(), st);
+            k_copyt<<<gy(keep.size()), 256, 0, st>>>(T, R, kSlotZ, kSlotDL, NV, dk.p);
+            for (int r : keep) used[r] = 1;
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        if (!drop.empty()) {
+            DBuf<int32_t> dd;
+            dd.upload(drop.data(), drop.size(), st);
+            k_fillt<<<gy(drop.size()), 256, 0, st>>>(T, R, kSlotX, NV, 0.0, dd.p);
+            HIPCHK(hipStreamSynchronize(st));
+        }
+    }
+    HIPCHK(hipGetLastError());
+    if (!gmres_graph_cycles(rs, red, tol, nrmax, bnrm, iter_out, err_out, ierr_out, launches))
+        throw Error("device GMRES unavailable for this run set");
+    if (x0_used)
+        for (int r = 0; r < R; ++r) x0_used[r] = used[r];
+    return true;
+}
+
 // R right-hand sides on one surface: vectors stored with a common stride so the
 // block field and preconditioner kernels see them as block columns
 struct BlockRuns : RunSet {
@@ -4891,8 +5461,18 @@ struct PackedRuns : RunSet {
     std::vector<double *> base;         // per slot: the packed vector
     DBuf<int32_t> dgate;
     std::vector<int32_t> hgate;         // flags last uploaded
+    const int32_t *devgate = nullptr;   // device flags in use (graph form)
+    bool device_gate_ok() const override { return true; }
+    bool use_device_gate(const int32_t *g) override
+    {
+        devgate = g;
+        c->gate = g;
+        hgate.clear();
+        return true;
+    }
     void set_gate(const std::vector<int32_t> &cols)
     {
+        if (devgate) { c->gate = devgate; return; }
         std::vector<int32_t> g(R, 0);
         for (int r : cols) g[r] = 1;
         if (g != hgate || !dgate.p) {
@@ -4976,6 +5556,13 @@ struct SingleRun : RunSet {
     {
         msolve_dev(c, ptr(sin, 0), ptr(sout, 0));
     }
+    bool device_gate_ok() const override { return true; }
+    bool use_device_gate(const int32_t *g) override
+    {
+        c->gate = g;
+        return true;
+    }
+    ~SingleRun() override { c->gate = nullptr; }
 };
 
 // One GMRES run on the context's stream (gmres_runs with R = 1); with fallback,
@@ -4988,6 +5575,10 @@ void gmres_dev(tabi_ctx *c, const double *db, double *dxs, double tol, int maxl,
     const int64_t N = 2 * (int64_t)c->n;
     if (maxl > N) maxl = (int)N;
     SingleRun rs(c, maxl, db, dxs);
+    // flags bit 4: the cycles as captured graphs with the control on the device
+    if ((c->prm.flags & 16) &&
+        gmres_graph(rs, c->blk_red, tol, nrmax, x0_zero, iter_out, err_out, ierr_out, fallback, x0_used))
+        return;
     gmres_runs(rs, c->blk_red, tol, nrmax, x0_zero, iter_out, err_out, ierr_out, fallback, x0_used);
 }
 
@@ -5619,9 +6210,16 @@ void solve_many_packed(const tabi_params *prm, int32_t nsurf, const int32_t *nfa
     std::vector<int> it(nsurf), ie(nsurf);
     {
         PackedRuns rs(&c, 10, dB, dX);
-        gmres_runs(rs, c.blk_red, tol, 10, true, it.data(), err, ie.data());
-        g_many[0] = 0.0;
-        g_many[1] = 0.0;
+        int64_t launches = 0;
+        if (!(options & 1) &&
+            gmres_graph(rs, c.blk_red, tol, 10, true, it.data(), err, ie.data(), false, nullptr, &launches)) {
+            g_many[0] = 1.0;                 // one cycle graph, captured once
+            g_many[1] = (double)launches;    // launched once per restart cycle
+        } else {
+            gmres_runs(rs, c.blk_red, tol, 10, true, it.data(), err, ie.data());
+            g_many[0] = 0.0;
+            g_many[1] = 0.0;
+        }
     }
     HIPCHK(hipStreamSynchronize(c.s));
     auto t2 = std::chrono::steady_clock::now();

@@ -35,6 +35,7 @@ EXPORTS = (
 FLAG_MATVEC_L23 = 1       # preconditioner with matvec-consistent L2/L3 placement
 FLAG_HOST_TREE = 2        # sequential host tree build
 FLAG_DIAG_PREC = 4        # diagonal scaling MSOLVEprec (usrdata.in iprcd != 1)
+FLAG_GRAPH_GMRES = 16     # GMRES cycles as captured graphs with the control on the device
 
 SOLVE_PHASES = ("tree", "lists", "precond", "rhs", "gmres", "energy", "total", "warmstart")
 

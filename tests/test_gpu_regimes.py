@@ -147,6 +147,32 @@ def test_vertex_potential_of_reference_solution_is_the_reference_file(name):
         assert len(faces) == int(g["pot_nface"])
 
 
+@pytest.mark.parametrize("name", ["mol150_restart", "sph3_cap", "sph3_diag", "sph3_degen"])
+def test_device_controlled_gmres_equals_host_controlled(name):
+    """GMRES with its control on the device, each restart cycle one captured
+    graph (flags bit 4), repeats the host-controlled run bit for bit: the
+    restart (mol150_restart), the IERR=1 cap (sph3_cap), the diagonal
+    preconditioner and culled faces; from zero and from an initial guess."""
+    P, S, g = case(name)
+    tol, nrmax = float(g["tol"]), int(g["nrmax"])
+    b = S.rhs(P.qpos, P.q, P.usr.epsp)
+    S2 = hip.Surface(P.params(flags=hip.FLAG_GRAPH_GMRES), P.xyz, P.nrm, P.area)
+    # a guess: half the reference's (tree-order) solution, where it is stored whole
+    x0 = 0.5 * np.asarray(g["xvct"]) if int(g["stride"]) == 1 else None
+    for guess in (None, x0):
+        x, it, err, ierr = S.gmres(b, guess, tol=tol, nrmax=nrmax)
+        x2, it2, err2, ierr2 = S2.gmres(b, guess, tol=tol, nrmax=nrmax)
+        print(f"[gpu] {name} guess={guess is not None}: host {it} {ierr} {err:.3e}, "
+              f"device {it2} {ierr2} {err2:.3e}", flush=True)
+        assert (it2, ierr2) == (it, ierr) and err2 == err
+        assert np.array_equal(x2, x)
+        if guess is None:
+            assert (it, ierr) == (int(g["iter"]), int(g["ierr"]))
+        if x0 is None:
+            break
+    S2.close()
+
+
 def test_restart_and_cap_through_block_and_many_surface_forms():
     """The lockstep forms take the same restart and IERR=1 paths: a block of
     three right-hand sides (b, -2b, b/3) and a one-surface tabi_solve_many give
@@ -159,10 +185,13 @@ def test_restart_and_cap_through_block_and_many_surface_forms():
     for k, f in enumerate((1.0, -2.0, 1.0 / 3.0)):
         assert rel_l2(X[k][::1], f * g["xvct"]) <= 1e-9
     P2, S2, g2 = case("mol150_restart")
-    r = hip.solve_many(P2.params(), [(P2.xyz, P2.nrm, P2.area, P2.qpos, P2.q)], P2.usr.epsp,
-                       tol=float(g2["tol"]))
-    assert (int(r["iter"][0]), int(r["ierr"][0])) == (int(g2["iter"]), int(g2["ierr"]))
-    assert r["soleng"][0] == pytest.approx(float(g2["soleng"]), rel=1e-10)
+    for graph in (True, False):
+        r = hip.solve_many(P2.params(), [(P2.xyz, P2.nrm, P2.area, P2.qpos, P2.q)], P2.usr.epsp,
+                           tol=float(g2["tol"]), graph=graph)
+        assert (int(r["iter"][0]), int(r["ierr"][0])) == (int(g2["iter"]), int(g2["ierr"]))
+        assert r["soleng"][0] == pytest.approx(float(g2["soleng"]), rel=1e-10)
+        if graph:   # 14 steps: two restart cycles, one captured graph
+            assert r["stats"]["captures"] == 1 and r["stats"]["graph_launches"] == 2
 
 
 def test_mixed_precision_on_the_molecule():
