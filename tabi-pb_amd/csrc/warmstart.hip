@@ -18,6 +18,7 @@
 // deltas pass through LDS only where every wave needs every tile (the next
 // layer, and the weight gradients, whose contraction runs over faces).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdint>
@@ -133,18 +134,92 @@ __device__ __forceinline__ float gelu_d(float z)
 //   11 sum w2 (how buried the face is: a smooth count of charges within R2)
 constexpr float kCurvH = 1.0f;      // curvature kernel width (Angstrom)
 constexpr float kR1 = 5.0f, kR2 = 10.0f;
-constexpr int kAtomTile = 256;
 
+// The charges are visited in tiles of kQTile that are compact in space (sorted
+// by a Morton code of their position on a 1 A grid), each with a bounding
+// sphere.  A wave skips a tile when, for every one of its faces, the sphere
+// lies beyond the widest window (3 R2) and beyond the face's second-nearest
+// charge so far: no charge of the tile could then change any sum or the
+// nearest-charge features.  Each wave starts at the tile of its own position,
+// so the nearest charges come first.
+constexpr int kQTile = 32;
+constexpr float kQSkip = 9.0f * 100.0f;     // (3 R2)^2, R2 = 10 A
+constexpr float kMortonOff = 512.0f;        // grid origin -512 A, 1 A cells, 10 bits per axis
+
+__device__ __forceinline__ uint32_t morton10(float x, float y, float z)
+{
+    auto spread = [](uint32_t v) {
+        v &= 0x3ffu;
+        v = (v | (v << 16)) & 0x030000ffu;
+        v = (v | (v << 8)) & 0x0300f00fu;
+        v = (v | (v << 4)) & 0x030c30c3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    auto cell = [](float c) {
+        const float g = floorf(c + kMortonOff);
+        return (uint32_t)fminf(fmaxf(g, 0.0f), 1023.0f);
+    };
+    return spread(cell(x)) | (spread(cell(y)) << 1) | (spread(cell(z)) << 2);
+}
+
+__global__ void k_ws_qkey(int nq, const double *__restrict__ qpos, uint32_t *__restrict__ key,
+                          int32_t *__restrict__ idx)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nq) return;
+    key[a] = morton10((float)qpos[3 * a], (float)qpos[3 * a + 1], (float)qpos[3 * a + 2]);
+    idx[a] = a;
+}
+
+// charge records {x, y, z, q} in FP32, in Morton order
+__global__ void k_ws_qrec(int nq, const double *__restrict__ qpos, const double *__restrict__ qch,
+                          const int32_t *__restrict__ idx, float4 *__restrict__ rec)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const int a = idx[t];
+    rec[t] = make_float4((float)qpos[3 * a], (float)qpos[3 * a + 1], (float)qpos[3 * a + 2], (float)qch[a]);
+}
+
+// bounding sphere of each tile {centre, radius}: the centre of the tile's box,
+// the radius its largest distance to a charge (plus a margin for rounding)
+__global__ void k_ws_qtile(int nq, const float4 *__restrict__ rec, float4 *__restrict__ tile)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a0 = t * kQTile, a1 = min(nq, a0 + kQTile);
+    if (a0 >= nq) return;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int a = a0; a < a1; ++a) {
+        const float4 c = rec[a];
+        lo[0] = fminf(lo[0], c.x); hi[0] = fmaxf(hi[0], c.x);
+        lo[1] = fminf(lo[1], c.y); hi[1] = fmaxf(hi[1], c.y);
+        lo[2] = fminf(lo[2], c.z); hi[2] = fmaxf(hi[2], c.z);
+    }
+    const float mx = 0.5f * (lo[0] + hi[0]), my = 0.5f * (lo[1] + hi[1]), mz = 0.5f * (lo[2] + hi[2]);
+    float r2 = 0.0f;
+    for (int a = a0; a < a1; ++a) {
+        const float4 c = rec[a];
+        const float dx = c.x - mx, dy = c.y - my, dz = c.z - mz;
+        r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
+    }
+    tile[t] = make_float4(mx, my, mz, sqrtf(r2) * 1.0001f + 1.0e-3f);
+}
+
+// Each wave stages the records of the tile it visits in its own LDS slots
+// (one coalesced load), then every lane reads them as broadcasts.
 __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz, const double *__restrict__ tnx,
     const double *__restrict__ tny, const double *__restrict__ tnz,
     const double *__restrict__ tarea, const int32_t *__restrict__ leaf_of,
     const int32_t *__restrict__ leaf_beg, const int32_t *__restrict__ leaf_m2, int nq,
-    const double *__restrict__ qpos, const double *__restrict__ qch, double eps_p,
+    const float4 *__restrict__ qrec, const float4 *__restrict__ qtile,
+    const uint32_t *__restrict__ qkey, double eps_p,
     const double *__restrict__ b, double s, float *__restrict__ feat)
 {
-    __shared__ float sx[kAtomTile], sy[kAtomTile], sz[kAtomTile], sq[kAtomTile];
+    __shared__ float4 s_rec[256 / 64][kQTile];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const bool valid = i < n;
     const int ii = valid ? i : n - 1;
     const float cx = (float)tx[ii], cy = (float)ty[ii], cz = (float)tz[ii];
@@ -154,32 +229,49 @@ __global__ void __launch_bounds__(256) k_ws_feat(int n, const double *__restrict
     float d1 = INFINITY, d2 = INFINITY;          // squared distances, ascending
     float ax = 0.f, ay = 0.f, az = 0.f;          // nearest charge
     float p1 = 0.f, f1 = 0.f, p2 = 0.f, f2 = 0.f, bur = 0.f;
-    for (int a0 = 0; a0 < nq; a0 += kAtomTile) {
-        __syncthreads();
-        const int a = a0 + threadIdx.x;
-        if (a < nq) {
-            sx[threadIdx.x] = (float)qpos[3 * a];
-            sy[threadIdx.x] = (float)qpos[3 * a + 1];
-            sz[threadIdx.x] = (float)qpos[3 * a + 2];
-            sq[threadIdx.x] = (float)qch[a];
+    static_assert(kQSkip == 9.0f * kR2 * kR2, "the skip bound is the wide window's cutoff");
+    const int ntile = (nq + kQTile - 1) / kQTile;
+    // the wave's first tile: the first whose charges' Morton codes reach the
+    // code of the wave's first face (a binary search in the sorted codes)
+    int t0 = 0;
+    if (ntile > 0) {
+        const uint32_t key = morton10(__shfl(cx, 0), __shfl(cy, 0), __shfl(cz, 0));
+        int lo = 0, hi = nq;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (qkey[mid] < key) lo = mid + 1; else hi = mid;
         }
-        __syncthreads();
-        const int na = min(kAtomTile, nq - a0);
+        t0 = min(lo / kQTile, ntile - 1);
+    }
+    for (int tt = 0; tt < ntile; ++tt) {
+        int t = t0 + tt;
+        if (t >= ntile) t -= ntile;
+        const float4 sp = qtile[t];
+        const float ex = sp.x - cx, ey = sp.y - cy, ez = sp.z - cz;
+        const float dt = sqrtf(ex * ex + ey * ey + ez * ez) - sp.w;
+        const bool far = valid && dt > 0.0f && dt * dt > fmaxf(kQSkip, d2);
+        if (__all(far || !valid)) continue;
+        const int a0 = t * kQTile, na = min(nq - a0, kQTile);
+        __builtin_amdgcn_wave_barrier();          // the previous tile's reads are done
+        if (ln < na) s_rec[wv][ln] = qrec[a0 + ln];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
         for (int k = 0; k < na; ++k) {
-            const float dx = sx[k] - cx, dy = sy[k] - cy, dz = sz[k] - cz;
+            const float4 c = s_rec[wv][k];
+            const float dx = c.x - cx, dy = c.y - cy, dz = c.z - cz;
             const float r2 = dx * dx + dy * dy + dz * dz;
             // the two nearest (squared) distances, first nearest charge kept
             const bool closer = r2 < d1;
             d2 = fminf(d2, fmaxf(d1, r2));
             d1 = fminf(d1, r2);
-            ax = closer ? sx[k] : ax;
-            ay = closer ? sy[k] : ay;
-            az = closer ? sz[k] : az;
+            ax = closer ? c.x : ax;
+            ay = closer ? c.y : ay;
+            az = closer ? c.z : az;
             if (r2 < 9.0f * kR2 * kR2 && r2 > 0.f) {
                 const float ir = __builtin_amdgcn_rsqf(r2);
                 // kR1 = kR2 / 2: exp(-r2/kR1^2) = exp(-r2/kR2^2)^4
                 const float w2 = __expf(-r2 * i2), w2s = w2 * w2, w1 = w2s * w2s;
-                const float pot = sq[k] * ir;
+                const float pot = c.w * ir;
                 const float fld = pot * (nx * dx + ny * dy + nz * dz) * ir * ir;
                 p1 += pot * w1; f1 += fld * w1;
                 p2 += pot * w2; f2 += fld * w2;
@@ -639,8 +731,24 @@ void ws_features(const WsFeatIn &in, float *d_feat, void *stream)
 {
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (in.n <= 0) return;
+    // charges sorted into compact tiles (Morton order), records and spheres
+    const int nq = std::max(in.nq, 0), ntile = (nq + kQTile - 1) / kQTile;
+    Dev<float4> qrec(std::max(nq, 1)), qtile(std::max(ntile, 1));
+    Dev<uint32_t> key(std::max(nq, 1)), key2(std::max(nq, 1));
+    Dev<int32_t> idx(std::max(nq, 1)), idx2(std::max(nq, 1));
+    if (nq > 0) {
+        k_ws_qkey<<<nb(nq, 256), 256, 0, s>>>(nq, in.qpos, key.p, idx.p);
+        size_t tmp_bytes = 0;
+        WCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key2.p, idx.p, idx2.p, nq, 0,
+                                                30, s));
+        Dev<char> tmp(tmp_bytes);
+        WCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, key.p, key2.p, idx.p, idx2.p, nq, 0,
+                                                30, s));
+        k_ws_qrec<<<nb(nq, 256), 256, 0, s>>>(nq, in.qpos, in.q, idx2.p, qrec.p);
+        k_ws_qtile<<<nb(ntile, 64), 64, 0, s>>>(nq, qrec.p, qtile.p);
+    }
     k_ws_feat<<<nb(in.n, 256), 256, 0, s>>>(in.n, in.tx, in.ty, in.tz, in.tnx, in.tny, in.tnz,
-        in.tarea, in.leaf_of, in.leaf_beg, in.leaf_m2, in.nq, in.qpos, in.q, in.eps_p, in.b,
+        in.tarea, in.leaf_of, in.leaf_beg, in.leaf_m2, nq, qrec.p, qtile.p, key2.p, in.eps_p, in.b,
         in.scale, d_feat);
     WCHK(hipGetLastError());
 }
