@@ -95,17 +95,25 @@ __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1) with the hardware exponential and
+// reciprocal (absolute error ~1e-7; exp overflow gives 1, underflow -1)
+__device__ __forceinline__ float tanh_fast(float u)
+{
+    const float e = __expf(2.0f * u);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
 // GELU, tanh form, and its derivative
 __device__ __forceinline__ float gelu(float z)
 {
     const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    return 0.5f * z * (1.0f + tanhf(u));
+    return 0.5f * z * (1.0f + tanh_fast(u));
 }
 
 __device__ __forceinline__ float gelu_d(float z)
 {
     const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    const float t = tanhf(u);
+    const float t = tanh_fast(u);
     return 0.5f * (1.0f + t) +
            0.5f * z * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 0.134145f * z * z);
 }
