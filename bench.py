@@ -226,8 +226,10 @@ def many_molecules(nmol=48, lo=200, hi=4000, workers=12):
 
 
 def packed_surfaces(mols, precision=1):
-    """The molecules solved together (tabi_solve_many: threaded set-up, lockstep
-    GMRES with one captured graph per step) and one after another."""
+    """The molecules solved together (tabi_solve_many: one packed context for
+    all surfaces -- per-surface trees, one list build, one batched leaf LU --
+    and GMRES with its control on the device, one captured graph per restart
+    cycle of every surface) and one after another."""
     prm = warmstart.params_for(precision)
     surfs = [(m["xyz"], m["nrm"], m["area"], m["qpos"], m["q"]) for m in mols]
     hip.solve_many(prm, surfs[:4], warmstart.EPS_P)             # warm-up
