@@ -245,6 +245,29 @@ struct DevIn {
     }
 };
 
+// Events that queued work may still wait on; destroyed by clear() once that
+// work is known to have finished (or with the owner).
+struct EventSet {
+    std::vector<hipEvent_t> e;
+    EventSet() = default;
+    explicit EventSet(int count) { make(count); }
+    EventSet(const EventSet &) = delete;
+    EventSet &operator=(const EventSet &) = delete;
+    ~EventSet() { clear(); }
+    void make(int k)
+    {
+        clear();
+        e.assign(k, nullptr);
+        for (auto &x : e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    }
+    void clear()
+    {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+        e.clear();
+    }
+};
+
 struct NearTask {
     int32_t s0, s1, leaf, sf;   // source range [s0,s1) of leaf node `leaf`; surface
 };
@@ -295,6 +318,12 @@ struct tabi_ctx {
     DBuf<double> lu_linv;
     DBuf<double> lu;
     DBuf<float> lu32;                    // mixed precision: the factors stored in FP32
+    // factorisation in flight: its tile lists and events (freed by precond_finish)
+    DBuf<int4> lu_tl, lu_tlo;
+    DBuf<int2> lu_sw, lu_ts;
+    DBuf<int32_t> lu_tab;
+    DBuf<int64_t> lu_toff;
+    EventSet pre_ev;
     // vectors
     DBuf<double> vx, vy, red;
     // blocks of vectors
@@ -1299,13 +1328,6 @@ __device__ __forceinline__ int lu_swap_record(int lane, int k0, int nbe, int pr,
     return nr;
 }
 
-// Panel factorisation with partial pivoting, for a panel in which the
-// optimistic pass met a column whose diagonal entry is not its first maximum
-// in magnitude: the panel is reloaded and factored with the first-maximum rule
-// (IDAMAX): per column a block-wide arg-max (wave shuffles, a ballot for the
-// first maximum, the waves' results combined by every thread), a row exchange
-// through LDS and the rank-1 update; then stored.  Out of line, so the common
-// pass keeps its registers.
 // The double held by lane `lane` (a compile-time or wave-uniform index) of
 // the wave, read into every lane through two 32-bit lane reads.
 __device__ __forceinline__ double readlane_d(double v, int lane)
@@ -1316,11 +1338,65 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// A double / an int from another lane of the same row of 16 lanes (DPP
+// control Ctrl: quad permutations and row rotations only, so every source
+// lane exists).  Every lane of the wave must be active.
+template <int Ctrl>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    const unsigned lo2 = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, Ctrl, 0xF, 0xF, false);
+    const unsigned hi2 = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, Ctrl, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi2 << 32) | lo2));
+}
+template <int Ctrl>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, Ctrl, 0xF, 0xF, false);
+}
+
+// Maximum (minimum) over the wave, in every lane: quad_perm [1,0,3,2] and
+// [2,3,0,1], row_ror 4 and 8 leave each row of 16 lanes holding its row's
+// extreme, then the four rows' values are read from lanes 0, 16, 32, 48.  No
+// LDS round trips: the arg-max of a pivoting panel column is on its critical
+// path.  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_max_d(double v)
+{
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x124>(v));
+    v = fmax(v, dpp_d<0x128>(v));
+    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+}
+__device__ __forceinline__ int wave_min_i(int v)
+{
+    v = min(v, dpp_i<0xB1>(v));
+    v = min(v, dpp_i<0x4E>(v));
+    v = min(v, dpp_i<0x124>(v));
+    v = min(v, dpp_i<0x128>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// Panel factorisation with partial pivoting, for a panel in which the
+// optimistic pass met a column whose diagonal entry is not its first maximum
+// in magnitude: the panel is reloaded and factored with the first-maximum rule
+// (IDAMAX), one barrier per column.  Per column every wave finds its
+// candidate (largest magnitude, first row among equals) with DPP reductions
+// and the candidate's lane leaves the value, the row index and the whole row in
+// LDS, beside the row at the diagonal; after the barrier every thread combines
+// the waves' candidates in the same order, the two rows trade places through
+// those copies and the rank-1 update reads the pivot row from its LDS copy.
+// The per-column buffers alternate by column parity: a thread writes column
+// c + 2's only after the barrier of column c + 1, which every thread passes
+// only after its reads of column c.  s_cr holds 2 * NW * kLuNB doubles.  Out
+// of line, so the common pass keeps its registers.
 template <int RPT, int kPanelThreads>
 __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nbe,
     double *__restrict__ A, int32_t *__restrict__ pv, int32_t *__restrict__ info_l,
     double (*s_best)[kPanelThreads / kWave], int (*s_bi)[kPanelThreads / kWave],
-    double (*s_row)[2][kLuNB], int32_t *s_piv)
+    double (*s_row)[2][kLuNB], double *__restrict__ s_cr, int32_t *s_piv)
 {
     constexpr int NW = kPanelThreads / kWave;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -1337,62 +1413,70 @@ __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nb
     for (int c = 0; c < kLuNB; ++c) {
         if (c < nbe) {  // nbe is uniform over the workgroup
         const int col = k0 + c, pb = c & 1;
-        // arg-max of |a[row][col]| over rows col..m-1, first maximum (IDAMAX)
+        // this thread's candidate among its rows col..m-1 (rows grow with j)
         double best = -1.0;
-        int bi = 0x7fffffff;
+        int bi = 0x7fffffff, jb = 0;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int row = k0 + t + j * kPanelThreads;
             if (row >= col && row < m) {
                 const double v = fabs(p[j][c]);
-                if (v > best) { best = v; bi = row; }
+                if (v > best) { best = v; bi = row; jb = j; }
             }
         }
-        double wmax = best;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-        int wbi;
+        // the wave's: largest magnitude, then the first row holding it (with
+        // one row per thread rows grow with the lane)
+        const double wmax = wave_max_d(best);
+        int wrow;
         if (RPT == 1) {
-            // rows grow with the lane: the first lane holding the maximum
             const u64 hit = __ballot(best == wmax);
-            wbi = __shfl(bi, __ffsll((long long)hit) - 1);
+            wrow = __builtin_amdgcn_readlane(bi, __ffsll((long long)hit) - 1);
         } else {
-            wbi = best == wmax ? bi : 0x7fffffff;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) wbi = min(wbi, __shfl_xor(wbi, off));
+            wrow = wave_min_i(best == wmax ? bi : 0x7fffffff);
         }
-        if (lane == 0) { s_best[pb][w] = wmax; s_bi[pb][w] = wbi; }
+        double *cr = s_cr + (size_t)(pb * NW + w) * kLuNB;
+        if (wmax < 0.0) {                        // no row of this wave is a candidate
+            if (lane == 0) { s_best[pb][w] = -1.0; s_bi[pb][w] = 0x7fffffff; }
+        } else if (best == wmax && bi == wrow) { // exactly one lane: rows are distinct
+            s_best[pb][w] = wmax;
+            s_bi[pb][w] = wrow;
+#pragma unroll
+            for (int j = 0; j < RPT; ++j)
+                if (j == jb) {
+#pragma unroll
+                    for (int cc = 0; cc < kLuNB; ++cc) cr[cc] = p[j][cc];
+                }
+        }
+        if (t == c) {                            // row col is row 0 of thread c
+#pragma unroll
+            for (int cc = 0; cc < kLuNB; ++cc) s_row[pb][0][cc] = p[0][cc];
+        }
         __syncthreads();
-        // every thread combines the waves' results (same order everywhere)
+        // every thread combines the waves' candidates (same order everywhere)
         double gmax = s_best[pb][0];
-        int prow = s_bi[pb][0];
+        int prow = s_bi[pb][0], qw = 0;
 #pragma unroll
         for (int q = 1; q < NW; ++q) {
             const double v = s_best[pb][q];
             const int r = s_bi[pb][q];
-            if (v > gmax || (v == gmax && r < prow)) { gmax = v; prow = r; }
+            if (v > gmax || (v == gmax && r < prow)) { gmax = v; prow = r; qw = q; }
         }
+        const double *pr = s_cr + (size_t)(pb * NW + qw) * kLuNB;   // the pivot row
         if (t == 0) { s_piv[c] = prow; pv[col] = prow; }
-        // rows col and prow publish themselves, then trade
+        if (prow != col) {
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int row = k0 + t + j * kPanelThreads;
-            if (row == col || row == prow) {
+            for (int j = 0; j < RPT; ++j) {
+                const int row = k0 + t + j * kPanelThreads;
+                if (row == prow) {
 #pragma unroll
-                for (int cc = 0; cc < kLuNB; ++cc) s_row[pb][row == col ? 0 : 1][cc] = p[j][cc];
+                    for (int cc = 0; cc < kLuNB; ++cc) p[j][cc] = s_row[pb][0][cc];
+                } else if (row == col) {
+#pragma unroll
+                    for (int cc = 0; cc < kLuNB; ++cc) p[j][cc] = pr[cc];
+                }
             }
         }
-        __syncthreads();
-        const int pr = prow == col ? 0 : 1;   // s_row[pb][pr] is the pivot row
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int row = k0 + t + j * kPanelThreads;
-            if (prow != col && (row == col || row == prow)) {
-#pragma unroll
-                for (int cc = 0; cc < kLuNB; ++cc) p[j][cc] = s_row[pb][row == col ? 1 : 0][cc];
-            }
-        }
-        const double pivot = s_row[pb][pr][c];
+        const double pivot = pr[c];
         if (pivot == 0.0) {
             bad = col + 1;
         } else {
@@ -1404,8 +1488,7 @@ __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nb
                     p[j][c] *= rinv;
                     const double lv = p[j][c];
 #pragma unroll
-                    for (int cc = c + 1; cc < kLuNB; ++cc)
-                        p[j][cc] = fma(-lv, s_row[pb][pr][cc], p[j][cc]);
+                    for (int cc = c + 1; cc < kLuNB; ++cc) p[j][cc] = fma(-lv, pr[cc], p[j][cc]);
                 }
             }
         }
@@ -1438,6 +1521,40 @@ __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nb
 // interchanges: each tile's touched rows are read into registers, then, after
 // a barrier, written to their destinations.  Out of line: only panels that
 // pivot reach it.
+// The same interchanges on the strip's earlier columns and the block's other
+// columns together (every column but the panel's and those k_lu_swap_trsm
+// takes), when at most kSwapCols rows are touched: one thread per column reads
+// the touched rows' entries of its column, then writes them to their
+// destinations (a permutation of the same rows), so no barrier is needed.
+constexpr int kSwapCols = 16;
+template <int kPanelThreads>
+__device__ __attribute__((noinline)) void lu_swap_columns(int k0, int m, int nbe, double *A, int nr,
+                                                          const int *rows, const int *src)
+{
+    if (nr <= 0) return;                      // a zero pivot without interchanges
+    const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
+    const bool has_right = send > k0 + kLuNB;
+    const int r0 = (k0 + kLuNB) / 64, r1 = (send - 1) / 64;
+    int sr[kSwapCols], dr[kSwapCols];
+#pragma unroll
+    for (int q = 0; q < kSwapCols; ++q) {
+        sr[q] = src[min(q, nr - 1)];
+        dr[q] = rows[min(q, nr - 1)];
+    }
+    for (int j = threadIdx.x; j < m; j += kPanelThreads) {
+        if (j >= k0 && j < k0 + nbe) continue;                          // the panel
+        if (has_right && j / 64 >= r0 && j / 64 <= r1) continue;         // k_lu_swap_trsm's
+        double *col = A + (size_t)j * m;
+        double v[kSwapCols];
+#pragma unroll
+        for (int q = 0; q < kSwapCols; ++q)
+            if (q < nr) v[q] = col[sr[q]];
+#pragma unroll
+        for (int q = 0; q < kSwapCols; ++q)
+            if (q < nr) col[dr[q]] = v[q];
+    }
+}
+
 template <int kPanelThreads>
 __device__ __attribute__((noinline)) void lu_swap_rest(int k0, int m, int nbe, double *A, int nr,
                                                        const int *rows, const int *src)
@@ -1572,8 +1689,9 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
         }
     }
     } else {
+        static_assert(2 * NW <= kLuNB + 1, "candidate rows exceed s_u");
         lu_panel_pivoted<RPT, kPanelThreads>(k0, m, nbe, A, pv, info + l, s_best, s_bi, s_row,
-                                             s_piv);
+                                             &s_u[0][0], s_piv);
     }
     __syncthreads();
     // interchange record for the other columns (read by k_lu_swap_trsm)
@@ -1589,6 +1707,11 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     if (fast) return;
     __syncthreads();
     const int nr = s_prow;
+    if (nr <= kSwapCols) {
+        // few touched rows: every column at once, one thread per column
+        lu_swap_columns<kPanelThreads>(k0, m, nbe, A, nr, s_dst, s_src);
+        return;
+    }
     if (ncl > 0) {
         // element e = t + kPanelThreads i: row slot e % 64, column e / 64
         constexpr int kEs = (2 * kLuNB * (kLuW - kLuNB) + kPanelThreads - 1) / kPanelThreads;
@@ -2041,14 +2164,19 @@ __device__ __forceinline__ double gemv_row(const FT *__restrict__ Ac, int m, int
     return s + s1;
 }
 
-// NW waves per leaf: wave 0 solves each 64x64 diagonal tile (the solved
-// entry of each step reaches the other lanes through a scalar-register lane
-// read, not an LDS shuffle: this chain of 64 dependent steps per tile is the
-// critical path of a large leaf), and the row
-// updates outside the tile (the bulk of the factor stream) are shared by all NW
-// waves, so a large leaf is streamed NW times faster than by one wave.  Leaves
-// are visited in the order lorder (largest first) so the longest solves start
-// first.
+// NW waves per leaf, one barrier per 64-row tile, with one tile of
+// lookahead.  Forward step bi starts with z of tile bi-1 final: wave 0 applies
+// it to tile bi's rows and solves the 64x64 diagonal tile bi (the solved entry
+// of each step reaches the other lanes through a scalar-register lane read,
+// not an LDS shuffle: this chain of 64 dependent steps per tile is the
+// critical path of a large leaf), while the other waves apply z of tile bi-1
+// to the rows below tile bi, 64 rows per chunk taken from an LDS counter;
+// wave 0 joins them once its tile is solved.  So the bulk of the factor stream
+// keeps flowing while the diagonal tile is solved, instead of waiting for it.
+// The back substitution runs the same way upwards.  Each row still takes the
+// tiles' contributions in the same order with the same arithmetic, so z is
+// bit-identical to the tile-by-tile schedule.  Leaves are visited in the order
+// lorder (largest first) so the longest solves start first.
 //
 // FT is the stored factor type: double, or float for the mixed-precision form
 // (factors computed in FP64 and rounded once for storage; the substitutions
@@ -2061,22 +2189,41 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
     double *__restrict__ z, const int32_t *__restrict__ gate, const int32_t *__restrict__ blk_sf)
 {
     extern __shared__ double zs[];
+    __shared__ int s_next[2];                  // chunk counters, alternating by step
     const int l = lorder[blockIdx.x], t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (gate && !gate[blk_sf[l]]) return;      // surface not being solved
     const int m = blk_m[l], nl = m / 2, ib = blk_ibeg[l];
     const FT *A = lu + blk_off[l];
     const int32_t *pm = perm + piv_off[l];
+    if (t == 0) { s_next[0] = 0; s_next[1] = 0; }
     for (int k = t; k < m; k += 64 * NW) {
         const int src = pm[k];
         zs[k] = src < nl ? r[ib + src] : r[n + ib + src - nl];
     }
     __syncthreads();
     const int nb = (m + kSolveB - 1) / kSolveB;
-    for (int bi = 0; bi < nb; ++bi) {
+    // rows [rbeg, rend) take the contribution of the tile at column ks (width
+    // bs), 64 rows per chunk, chunks handed out by the counter
+    auto bulk = [&](int *next, int rbeg, int rend, int ks, int bs) {
+        const int nch = rend > rbeg ? (rend - rbeg + 63) / 64 : 0;
+        for (;;) {
+            int q = 0;
+            if (lane == 0) q = atomicAdd(next, 1);
+            q = __builtin_amdgcn_readlane(q, 0);
+            if (q >= nch) break;
+            const int row = rbeg + 64 * q + lane;
+            if (row < rend) zs[row] = gemv_row(A + (size_t)ks * m + row, m, bs, zs + ks, zs[row]);
+        }
+    };
+    int step = 0;
+    for (int bi = 0; bi < nb; ++bi, ++step) {
         const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
+        if (t == 0) s_next[(step + 1) & 1] = 0;
         if (w == 0) {
             const bool in = lane < bs;
             double zl = in ? zs[kb + lane] : 0.0;
+            if (bi > 0 && in)                  // tile bi-1's contribution to tile bi
+                zl = gemv_row(A + (size_t)(kb - kSolveB) * m + kb + lane, m, kSolveB, zs + kb - kSolveB, zl);
 #pragma unroll 1
             for (int h = 0; h < kSolveB; h += kHalf) {
                 FT tf[kHalf];
@@ -2093,18 +2240,18 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
             }
             if (in) zs[kb + lane] = zl;
         }
-        __syncthreads();
-        for (int r0 = kb + bs + 64 * w; r0 < m; r0 += 64 * NW) {
-            const int row = r0 + lane;
-            if (row < m) zs[row] = gemv_row(A + (size_t)kb * m + row, m, bs, zs + kb, zs[row]);
-        }
+        if (bi > 0) bulk(&s_next[step & 1], kb + kSolveB, m, kb - kSolveB, kSolveB);
         __syncthreads();
     }
-    for (int bi = nb - 1; bi >= 0; --bi) {
+    for (int bi = nb - 1; bi >= 0; --bi, ++step) {
         const int kb = bi * kSolveB, bs = min(kSolveB, m - kb);
+        const int kb1 = kb + kSolveB, bs1 = min(kSolveB, m - kb1);   // tile bi+1
+        if (t == 0) s_next[(step + 1) & 1] = 0;
         if (w == 0) {
             const bool in = lane < bs;
             double zl = in ? zs[kb + lane] : 0.0;
+            if (bi < nb - 1 && in)             // tile bi+1's contribution to tile bi
+                zl = gemv_row(A + (size_t)kb1 * m + kb + lane, m, bs1, zs + kb1, zl);
 #pragma unroll 1
             for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
                 FT tf[kHalf];
@@ -2125,9 +2272,7 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
             }
             if (in) zs[kb + lane] = zl;
         }
-        __syncthreads();
-        for (int r0 = 64 * w; r0 < kb; r0 += 64 * NW)
-            zs[r0 + lane] = gemv_row(A + (size_t)kb * m + r0 + lane, m, bs, zs + kb, zs[r0 + lane]);
+        if (bi < nb - 1) bulk(&s_next[step & 1], 0, kb, kb1, bs1);
         __syncthreads();
     }
     for (int q = t; q < nl; q += 64 * NW) {
@@ -2214,6 +2359,76 @@ __global__ void __launch_bounds__(64) k_energy(int n, int nq, int nchunk,
         s = fma(h1, rb.z, fma(h2, rb.w, s));
     }
     part[(size_t)task * kWave + lane] = s;
+}
+
+// Mixed precision: the energy's source records in FP32, two float4 per face:
+// {x, y, z} relative to the first face of the face's chunk (the difference
+// taken in FP64, then rounded) and the normal {nx, ny, nz}; the .w entries
+// hold the weights w1 = A*phi, w2 = A*dphi/dn of the solution when src8 holds
+// them (weights true), else 0 (the block form reads its weights apart).
+__global__ void k_energy_rec32(int n, const double *__restrict__ src8, bool weights,
+                               float4 *__restrict__ rec)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double *a = src8 + (size_t)j * 8;
+    const double *o = src8 + (size_t)(j / kEnergyChunk * kEnergyChunk) * 8;
+    rec[2 * (size_t)j] = make_float4((float)(a[0] - o[0]), (float)(a[1] - o[1]), (float)(a[2] - o[2]),
+                                     weights ? (float)a[6] : 0.f);
+    rec[2 * (size_t)j + 1] = make_float4((float)a[3], (float)a[4], (float)a[5],
+                                         weights ? (float)a[7] : 0.f);
+}
+
+// The FP32 factors of one (charge, face) pair of the energy, d = face - charge,
+// r = |d|: P = (n.d)(1 - eps e^{-kr}(1 + kr))/r^3 and Q = (1 - e^{-kr})/r, so
+// that w1 P + w2 Q is 4 pi (w1 (G1 - eps G2) + w2 (G0 - Gk)) of k_energy.  The
+// charge (qx, qy, qz) is relative to the chunk's first face, like the record.
+__device__ __forceinline__ void energy_pq32(const float4 ra, const float4 rb, float qx, float qy,
+                                            float qz, float kappa, float eps, float &P, float &Q)
+{
+    const float dx = ra.x - qx, dy = ra.y - qy, dz = ra.z - qz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float krs = kappa * (r2 * inv);
+    const float ek = __expf(-krs);
+    const float nd = fmaf(rb.x, dx, fmaf(rb.y, dy, rb.z * dz));
+    const float a = fmaf(-eps, fmaf(krs, ek, ek), 1.f);
+    P = nd * (inv * inv) * a * inv;
+    Q = (1.f - ek) * inv;
+}
+
+// The potential at each charge as k_energy computes it, with the pair terms in
+// FP32 (energy_pq32): the terms of 16 faces are summed in FP32, those sums in
+// FP64.  Tasks and partials as k_energy's.
+constexpr int kEnergySub = 16;
+__global__ void __launch_bounds__(64) k_energy32(int n, int nq, int nchunk,
+    const double *__restrict__ qpos, const double *__restrict__ src8,
+    const float4 *__restrict__ rec, float eps, float kappa, double *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    const int g = task / nchunk, ch = task % nchunk;
+    const int lane = lane_id();
+    const int k = g * kWave + lane;
+    const bool valid = k < nq;
+    const int f0 = ch * kEnergyChunk, f1 = min(n, f0 + kEnergyChunk);
+    const double *o = src8 + (size_t)f0 * 8;
+    const float qx = valid ? (float)(qpos[3 * k] - o[0]) : 0.f;
+    const float qy = valid ? (float)(qpos[3 * k + 1] - o[1]) : 0.f;
+    const float qz = valid ? (float)(qpos[3 * k + 2] - o[2]) : 0.f;
+    double sd = 0.0;
+    for (int j0 = f0; j0 < f1; j0 += kEnergySub) {
+        const int j1 = min(f1, j0 + kEnergySub);
+        float s = 0.f;
+#pragma unroll 4
+        for (int j = j0; j < j1; ++j) {
+            const float4 ra = rec[2 * (size_t)j], rb = rec[2 * (size_t)j + 1];
+            float P, Q;
+            energy_pq32(ra, rb, qx, qy, qz, kappa, eps, P, Q);
+            s = fmaf(ra.w, P, fmaf(rb.w, Q, s));
+        }
+        sd += (double)s;
+    }
+    part[(size_t)task * kWave + lane] = sd * kOneOver4Pi;
 }
 
 __global__ void k_energy_reduce(int nq, int nchunk, const double *__restrict__ part,
@@ -2875,6 +3090,51 @@ __global__ void __launch_bounds__(64) k_energyb(int n, int nq, int nchunk, int R
 #pragma unroll
     for (int c = 0; c < RC; ++c)
         if (r0 + c < R) part[((size_t)task * R + r0 + c) * kWave + lane] = s[c];
+}
+
+// k_energyb with the pair terms in FP32 (energy_pq32, summed as k_energy32
+// sums them, so a column's potentials equal those of a single solve's);
+// records from k_energy_rec32 without weights, weights wbf[j][2*Rp] in FP32.
+template <int RC>
+__global__ void __launch_bounds__(64) k_energyb32(int n, int nq, int nchunk, int R, int Rp,
+    const double *__restrict__ qpos, const double *__restrict__ src8,
+    const float4 *__restrict__ rec, const float *__restrict__ wb, float eps, float kappa,
+    double *__restrict__ part)
+{
+    const int task = blockIdx.x;
+    const int r0 = blockIdx.y * RC;
+    const int g = task / nchunk, ch = task % nchunk;
+    const int lane = lane_id();
+    const int k = g * kWave + lane;
+    const bool valid = k < nq;
+    const int f0 = ch * kEnergyChunk, f1 = min(n, f0 + kEnergyChunk);
+    const double *o = src8 + (size_t)f0 * 8;
+    const float qx = valid ? (float)(qpos[3 * k] - o[0]) : 0.f;
+    const float qy = valid ? (float)(qpos[3 * k + 1] - o[1]) : 0.f;
+    const float qz = valid ? (float)(qpos[3 * k + 2] - o[2]) : 0.f;
+    double sd[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) sd[c] = 0.0;
+    for (int j0 = f0; j0 < f1; j0 += kEnergySub) {
+        const int j1 = min(f1, j0 + kEnergySub);
+        float s[RC];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) s[c] = 0.f;
+#pragma unroll 2
+        for (int j = j0; j < j1; ++j) {
+            const float4 ra = rec[2 * (size_t)j], rb = rec[2 * (size_t)j + 1];
+            float P, Q;
+            energy_pq32(ra, rb, qx, qy, qz, kappa, eps, P, Q);
+            const float *w = wb + (size_t)j * 2 * Rp + 2 * r0;
+#pragma unroll
+            for (int c = 0; c < RC; ++c) s[c] = fmaf(w[2 * c], P, fmaf(w[2 * c + 1], Q, s[c]));
+        }
+#pragma unroll
+        for (int c = 0; c < RC; ++c) sd[c] += (double)s[c];
+    }
+#pragma unroll
+    for (int c = 0; c < RC; ++c)
+        if (r0 + c < R) part[((size_t)task * R + r0 + c) * kWave + lane] = sd[c] * kOneOver4Pi;
 }
 
 __global__ void k_energy_reduceb(int nq, int nchunk, int R, const double *__restrict__ part,
@@ -3794,7 +4054,10 @@ void xcd_group(std::vector<T> &v, size_t beg, T pad)
     v.insert(v.end(), out.begin(), out.end());
 }
 
-void build_precond(tabi_ctx *c)
+// Leaf-block tables of the preconditioner (block sizes and offsets, the face
+// -> block map), uploaded on the context's stream; the learned-guess features
+// read them too.
+void precond_tables(tabi_ctx *c)
 {
     const HostTree &T = c->tree;
     const int nl = (int)T.leaves.size();
@@ -3835,12 +4098,21 @@ void build_precond(tabi_ctx *c)
     c->lu_swrec.alloc((size_t)nl * kSwRec);
     c->lu_linv.alloc((size_t)nl * (kLuW / kLuNB) * kLuNB * kLuNB);
     HIPCHK(hipMemsetAsync(c->lu_info.p, 0, sizeof(int32_t) * nl, c->s));
+}
+
+// Queues the assembly and factorisation of every leaf block (and, for mixed
+// precision, the rounding of the factors to FP32) on the context's stream and
+// its side streams, without waiting: work on other streams can run beside it
+// until precond_finish.
+void precond_factor(tabi_ctx *c)
+{
+    const int nl = c->nleaf;
+    const std::vector<int32_t> &bm = c->h_blk_m;
+    const int64_t off = (int64_t)c->lu.n;
     // the tile lists below are uploaded on a side stream that waits for the
     // work queued so far but not for the assembly, so the copies overlap it
-    hipEvent_t ev_fork = nullptr, ev_up = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
-    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
-    for (auto &e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->pre_ev.make(5);
+    hipEvent_t ev_fork = c->pre_ev.e[0], ev_up = c->pre_ev.e[1], *ev_join = &c->pre_ev.e[2];
     HIPCHK(hipEventRecord(ev_up, c->s));
     k_lu_assemble<<<nblk(c->n, 4 * kAsmPerWave), 256, 0, c->s>>>(c->n, c->leaf_of.p, c->blk_ibeg.p, c->blk_m.p,
         c->blk_off.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->tarea.p,
@@ -3881,10 +4153,11 @@ void build_precond(tabi_ctx *c)
     }
     const int64_t *sw_off = toff.data(), *tl_off = sw_off + (nst + 1);
     const int64_t *ts_off = tl_off + (nst + 1), *tlo_off = ts_off + (nst + 1);
-    DBuf<int4> dtl, dtlo;
-    DBuf<int2> dsw, dts;
-    DBuf<int32_t> dtab;
-    DBuf<int64_t> dtoff;
+    // the tile lists outlive this call (the queued kernels read them)
+    DBuf<int4> &dtl = c->lu_tl, &dtlo = c->lu_tlo;
+    DBuf<int2> &dsw = c->lu_sw, &dts = c->lu_ts;
+    DBuf<int32_t> &dtab = c->lu_tab;
+    DBuf<int64_t> &dtoff = c->lu_toff;
     {
         const hipStream_t up = side_stream(0);
         HIPCHK(hipStreamWaitEvent(up, ev_up, 0));
@@ -3971,27 +4244,40 @@ void build_precond(tabi_ctx *c)
                                                              c->blk_m.p, c->blk_off.p, c->lu.p);
     }
     HIPCHK(hipGetLastError());
-    c->perm.alloc(pof);
+    c->perm.alloc(c->piv.n);
     k_lu_perm<<<nl, kWave, 0, c->s>>>(c->blk_m.p, c->piv_off.p, c->piv.p, c->perm.p);
     HIPCHK(hipGetLastError());
-    std::vector<int32_t> info(nl);
-    HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    (void)hipEventDestroy(ev_fork);
-    (void)hipEventDestroy(ev_up);
-    for (auto &e : ev_join) (void)hipEventDestroy(e);
-    for (int l = 0; l < nl; ++l)
-        if (info[l]) throw Error("singular leaf block " + std::to_string(l + 1) + " (column " +
-                                 std::to_string(info[l]) + ")");
     if (c->prm.precision == 1) {
         // mixed precision: the FP64 factors are rounded once to FP32 for the
         // solves, which then stream half the bytes
         c->lu32.alloc(off);
         k_to_float<<<to_float_grid(off), 256, 0, c->s>>>(off, c->lu.p, c->lu32.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->s));
-        c->lu.release();
     }
+}
+
+// Waits for the factorisation, reports a singular leaf block (its first zero
+// pivot) and frees what only the factorisation used.
+void precond_finish(tabi_ctx *c)
+{
+    const int nl = c->nleaf;
+    std::vector<int32_t> info(nl);
+    HIPCHK(hipMemcpyAsync(info.data(), c->lu_info.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->pre_ev.clear();
+    c->lu_tl.release(); c->lu_tlo.release(); c->lu_sw.release(); c->lu_ts.release();
+    c->lu_tab.release(); c->lu_toff.release();
+    if (c->prm.precision == 1) c->lu.release();
+    for (int l = 0; l < nl; ++l)
+        if (info[l]) throw Error("singular leaf block " + std::to_string(l + 1) + " (column " +
+                                 std::to_string(info[l]) + ")");
+}
+
+void build_precond(tabi_ctx *c)
+{
+    precond_tables(c);
+    precond_factor(c);
+    precond_finish(c);
 }
 
 // Centroids from interleaved {x,y,z} to three arrays.
@@ -4024,7 +4310,7 @@ __global__ void k_gather_geom(int n, const int32_t *__restrict__ order,
 
 void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz,
                const double *nrm, const double *area, int32_t *orderarr_out, double *t_phase,
-               bool dev = false)
+               bool dev = false, bool factor = true)
 {
     if (!prm) throw Error("null parameters");
     if (n < 1) throw Error("need at least one face");
@@ -4093,10 +4379,14 @@ void ctx_setup(tabi_ctx *c, const tabi_params *prm, int32_t n, const double *xyz
     auto t1 = std::chrono::steady_clock::now();
     build_lists(c);
     auto t2 = std::chrono::steady_clock::now();
+    // without factor only the leaf-block tables: the caller queues the
+    // factorisation (precond_factor) and waits for it (precond_finish)
     if (prm->flags & 4)
         c->nleaf = (int32_t)c->tree.leaves.size();   // diagonal scaling: no leaf blocks
-    else
+    else if (factor)
         build_precond(c);
+    else
+        precond_tables(c);
     auto t3 = std::chrono::steady_clock::now();
     if (t_phase) {
         t_phase[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -4269,20 +4559,6 @@ void ctx_stats(tabi_ctx *c, int64_t *o)
     o[9] = c->lu.n * (int64_t)sizeof(double) + c->lu32.n * (int64_t)sizeof(float);
 }
 
-
-// Events that are destroyed with their owner.
-struct EventSet {
-    std::vector<hipEvent_t> e;
-    explicit EventSet(int count) : e(count, nullptr)
-    {
-        for (auto &x : e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
-    }
-    ~EventSet()
-    {
-        for (auto x : e)
-            if (x) (void)hipEventDestroy(x);
-    }
-};
 
 // Waits for an event by polling: the wake-up latency of a blocking wait is
 // tens of microseconds, several times the work between the GMRES round trips.
@@ -5619,13 +5895,28 @@ void energy_block_dev(tabi_ctx *c, int R, int nq, const double *qpos, const doub
     dq.upload(q, (size_t)R * nq, c->s);
     const int ngq = (nq + kWave - 1) / kWave;
     const int nch = (c->n + kEnergyChunk - 1) / kEnergyChunk;
-    wb.alloc((size_t)c->n * 2 * Rp);
     part.alloc((size_t)ngq * nch * R * kWave);
     dptl.alloc((size_t)R * nq);
-    k_src_weightsb<double><<<dim3(nblk(c->n, 256), Rp), 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, dX, N,
-                                                                      dall.p, wb.p);
-    k_energyb<RC><<<dim3(ngq * nch, Rp / RC), 64, 0, c->s>>>(c->n, nq, nch, R, Rp, dqp.p, c->src8.p,
-                                                            wb.p, c->prm.eps, c->prm.kappa, part.p);
+    if (c->prm.precision == 1) {
+        // mixed precision: FP32 pair terms, as the single-solution energy
+        DBuf<float> wbf;
+        DBuf<float4> rec;
+        wbf.alloc((size_t)c->n * 2 * Rp);
+        rec.alloc((size_t)c->n * 2);
+        k_src_weightsb<float><<<dim3(nblk(c->n, 256), Rp), 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, dX,
+                                                                         N, dall.p, wbf.p);
+        k_energy_rec32<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->src8.p, false, rec.p);
+        k_energyb32<RC><<<dim3(ngq * nch, Rp / RC), 64, 0, c->s>>>(c->n, nq, nch, R, Rp, dqp.p,
+            c->src8.p, rec.p, wbf.p, (float)c->prm.eps, (float)c->prm.kappa, part.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->s));
+    } else {
+        wb.alloc((size_t)c->n * 2 * Rp);
+        k_src_weightsb<double><<<dim3(nblk(c->n, 256), Rp), 256, 0, c->s>>>(c->n, R, Rp, c->tarea.p, dX,
+                                                                          N, dall.p, wb.p);
+        k_energyb<RC><<<dim3(ngq * nch, Rp / RC), 64, 0, c->s>>>(c->n, nq, nch, R, Rp, dqp.p,
+            c->src8.p, wb.p, c->prm.eps, c->prm.kappa, part.p);
+    }
     k_energy_reduceb<<<dim3(nblk(nq, 256), R), 256, 0, c->s>>>(nq, nch, R, part.p, dptl.p);
     const int ncb = (nq + 255) / 256, ncp = ncb * ncb;
     cpart.alloc((size_t)R * ncp);
@@ -5671,10 +5962,11 @@ double ws_features_dev(tabi_ctx *c, int nq, const double *qpos, const double *q,
 
 // Scratch of the energy evaluation for nq charges on n faces: per (charge
 // group, face chunk) partials, per-charge potentials, Coulomb partials.
+// Mixed precision adds the FP32 source records (8 floats per face).
 struct EnergyScratch {
     int ngq, nch, ncb, ncp;
-    int64_t part, ptl, cpart, total;   // offsets and size in doubles
-    EnergyScratch(int n, int nq)
+    int64_t part, ptl, cpart, rec, total;   // offsets and size in doubles
+    EnergyScratch(int n, int nq, bool mixed)
     {
         ngq = (nq + kWave - 1) / kWave;
         nch = (n + kEnergyChunk - 1) / kEnergyChunk;
@@ -5683,7 +5975,8 @@ struct EnergyScratch {
         part = 0;
         ptl = (int64_t)ngq * nch * kWave;
         cpart = ptl + nq;
-        total = cpart + ncp + 1;
+        rec = (cpart + ncp + 1 + 1) / 2 * 2;   // 16-byte aligned
+        total = mixed ? rec + 4 * (int64_t)n : cpart + ncp + 1;
     }
 };
 
@@ -5698,11 +5991,19 @@ void energy_core(tabi_ctx *c, int nq, const double *dqp, const double *dq, doubl
 {
     const double para = 332.0716;  // src/main.f90:64
     if (nf < 0) nf = c->n;
-    const EnergyScratch L(nf, nq);
+    const bool mixed = c->prm.precision == 1;
+    const EnergyScratch L(nf, nq, mixed);
     double *part = w + L.part, *dptl = w + L.ptl, *cpart = w + L.cpart;
     if (weights) k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, dxs, c->src8.p);
-    k_energy<<<L.ngq * L.nch, 64, 0, c->s>>>(nf, nq, L.nch, dqp, c->src8.p + (size_t)8 * f0,
-                                             c->prm.eps, c->prm.kappa, part);
+    if (mixed) {
+        float4 *rec = reinterpret_cast<float4 *>(w + L.rec);
+        k_energy_rec32<<<nblk(nf, 256), 256, 0, c->s>>>(nf, c->src8.p + (size_t)8 * f0, true, rec);
+        k_energy32<<<L.ngq * L.nch, 64, 0, c->s>>>(nf, nq, L.nch, dqp, c->src8.p + (size_t)8 * f0,
+                                                   rec, (float)c->prm.eps, (float)c->prm.kappa, part);
+    } else {
+        k_energy<<<L.ngq * L.nch, 64, 0, c->s>>>(nf, nq, L.nch, dqp, c->src8.p + (size_t)8 * f0,
+                                                 c->prm.eps, c->prm.kappa, part);
+    }
     k_energy_reduce<<<nblk(nq, 256), 256, 0, c->s>>>(nq, L.nch, part, dptl);
     k_coulomb<<<dim3(L.ncb, L.ncb), 256, 0, c->s>>>(nq, dqp, dq, cpart);
     k_sum_part<<<1, 256, 0, c->s>>>(L.ncp, cpart, cpart + L.ncp);
@@ -5723,7 +6024,7 @@ void energy_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double
     if (nq <= 0) { *sol = 0.0; *cou = 0.0; return; }
     DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
     DBuf<double> w;
-    w.alloc((size_t)EnergyScratch(c->n, nq).total);
+    w.alloc((size_t)EnergyScratch(c->n, nq, c->prm.precision == 1).total);
     energy_core(c, nq, dqp.p, dq.p, eps_p, dxs, w.p, sol, cou);
 }
 
@@ -6228,7 +6529,7 @@ void solve_many_packed(const tabi_params *prm, int32_t nsurf, const int32_t *nfa
     for (int k = 0; k < nsurf; ++k) {
         DevIn dqp(qpos[k], (size_t)3 * nq[k], false, c.s), dq(q[k], nq[k], false, c.s);
         DBuf<double> w;
-        w.alloc((size_t)EnergyScratch(nface[k], nq[k]).total);
+        w.alloc((size_t)EnergyScratch(nface[k], nq[k], c.prm.precision == 1).total);
         energy_core(&c, nq[k], dqp.p, dq.p, eps_p, dX, w.p, soleng + k, couleng + k, c.sf_face0[k],
                     nface[k], false);
         iter[k] = it[k];
@@ -6487,7 +6788,7 @@ int tabi_ctx_energy_noalloc(tabi_ctx *c, int32_t nq, const double *d_qpos, const
         if (!soleng || !couleng) throw Error("null output");
         if (nq <= 0) { *soleng = 0.0; *couleng = 0.0; return; }
         if (!d_qpos || !d_q || !d_x) throw Error("null device array");
-        const EnergyScratch L(c->n, nq);
+        const EnergyScratch L(c->n, nq, c->prm.precision == 1);
         if (L.total > kScratchDoubles)
             throw Error("tabi_ctx_energy_noalloc: " + std::to_string(nq) + " charges on " +
                         std::to_string(c->n) + " faces need " + std::to_string(L.total * 8) +
@@ -6536,26 +6837,57 @@ void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const doub
     if (nq < 1) throw Error("need at least one charge");
     auto t0 = std::chrono::steady_clock::now();
     tabi_ctx c;
+    hipStream_t s2 = nullptr;
     try {
         double ph[3] = {0, 0, 0};
-        ctx_setup(&c, prm, n, xyz, nrm, area, orderarr_out, ph, dev);
+        const bool split = !(prm && (prm->flags & 4));
+        ctx_setup(&c, prm, n, xyz, nrm, area, orderarr_out, ph, dev, !split);
         PoolScope ps_(c.s);
-        auto t1 = std::chrono::steady_clock::now();
         const int64_t N = 2 * (int64_t)n;
         DBuf<double> db, dxs_own;
         db.alloc(N);
         double *dxs = (dev && x_out) ? x_out : (dxs_own.alloc(N), dxs_own.p);
-        rhs_dev(&c, nq, qpos, q, eps_p, db.p, dev);
         HIPCHK(hipMemsetAsync(dxs, 0, sizeof(double) * N, c.s));
-        auto t2 = std::chrono::steady_clock::now();
-        if (model) {
-            DBuf<float> feat;
-            feat.alloc((size_t)n * tabi::ws::kF);
-            const double scale = ws_features_dev(&c, nq, qpos, q, eps_p, db.p, feat.p, dev);
-            tabi::ws_predict_dev(model->m, n, feat.p, scale, dxs, c.s);
-            HIPCHK(hipStreamSynchronize(c.s));
+        // The right-hand side and the learned guess need the tree and the
+        // leaf tables but not the factors: they run on a second stream beside
+        // the factorisation (whose panel steps leave most of the GPU idle).
+        HIPCHK(hipStreamCreate(&s2));
+        EventSet ev(2);
+        HIPCHK(hipEventRecord(ev.e[0], c.s));
+        HIPCHK(hipStreamWaitEvent(s2, ev.e[0], 0));
+        auto t1 = std::chrono::steady_clock::now();
+        if (split) precond_factor(&c);
+        const hipStream_t s_main = c.s;
+        double t_rhs = 0.0, t_ws = 0.0;
+        {
+            // the helpers below queue on the context's stream: point it at s2
+            c.s = s2;
+            PoolScope ps2(s2);
+            try {
+                auto a = std::chrono::steady_clock::now();
+                rhs_dev(&c, nq, qpos, q, eps_p, db.p, dev);
+                auto b = std::chrono::steady_clock::now();
+                if (model) {
+                    DBuf<float> feat;
+                    feat.alloc((size_t)n * tabi::ws::kF);
+                    const double scale = ws_features_dev(&c, nq, qpos, q, eps_p, db.p, feat.p, dev);
+                    tabi::ws_predict_dev(model->m, n, feat.p, scale, dxs, s2);
+                    HIPCHK(hipStreamSynchronize(s2));
+                }
+                auto d = std::chrono::steady_clock::now();
+                t_rhs = std::chrono::duration<double, std::milli>(b - a).count();
+                t_ws = std::chrono::duration<double, std::milli>(d - b).count();
+            } catch (...) {
+                c.s = s_main;
+                throw;
+            }
+            c.s = s_main;
         }
+        HIPCHK(hipEventRecord(ev.e[1], s2));
+        HIPCHK(hipStreamWaitEvent(c.s, ev.e[1], 0));
+        if (split) precond_finish(&c);
         auto t2b = std::chrono::steady_clock::now();
+        ph[2] += std::chrono::duration<double, std::milli>(t2b - t1).count();
         int it = 0, ie = 0, used = 0;
         // without a model dxs was just zeroed: no host-side check of the guess
         gmres_dev(&c, db.p, dxs, tol, 10, 10, &it, err, &ie, model != nullptr, &used,
@@ -6572,14 +6904,16 @@ void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const doub
         if (ws_used) *ws_used = used;
         auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
         g_times[0] = ph[0]; g_times[1] = ph[1]; g_times[2] = ph[2];
-        g_times[3] = ms(t1, t2); g_times[4] = ms(t2b, t3); g_times[5] = ms(t3, t4);
-        g_times[6] = ms(t0, t4); g_times[7] = ms(t2, t2b);
+        g_times[3] = t_rhs; g_times[4] = ms(t2b, t3); g_times[5] = ms(t3, t4);
+        g_times[6] = ms(t0, t4); g_times[7] = t_ws;
     } catch (...) {
+        destroy_stream(s2);
         hipStream_t s = c.s;
         c.s = nullptr;
         destroy_stream(s);
         throw;
     }
+    destroy_stream(s2);
     hipStream_t s = c.s;
     c.s = nullptr;
     destroy_stream(s);

@@ -196,7 +196,9 @@ def test_restart_and_cap_through_block_and_many_surface_forms():
 
 def test_mixed_precision_on_the_molecule():
     """Mixed precision on the SES molecule: matvec within 1e-5 of the
-    reference's FP64 product, the solve converged with the energy within 1e-4."""
+    reference's FP64 product, the solve converged with the energy within 1e-4,
+    and the mixed context's energy (FP32 pair terms) within 2e-6 of the FP64
+    evaluation of the same solution."""
     P, _, g = case("mol150_restart")
     S = hip.Surface(P.params(precision=1), P.xyz, P.nrm, P.area)
     s = int(g["stride"])
@@ -207,4 +209,13 @@ def test_mixed_precision_on_the_molecule():
     assert ierr == 0 and err <= 1e-6
     sol, _ = S.energy(P.qpos, P.q, P.usr.epsp, x)
     assert sol == pytest.approx(float(g["soleng"]), rel=1e-4)
+    # the energy of a mixed context takes its pair terms in FP32: against the
+    # FP64 evaluation of the same solution vector
+    S64 = hip.Surface(P.params(), P.xyz, P.nrm, P.area)
+    sol64, cou64 = S64.energy(P.qpos, P.q, P.usr.epsp, x)
+    _, cou = S.energy(P.qpos, P.q, P.usr.epsp, x)
+    print(f"[gpu] FP32-term energy vs FP64: rel {abs(sol - sol64) / abs(sol64):.2e}", flush=True)
+    assert abs(sol - sol64) <= 2e-6 * abs(sol64)
+    assert cou == cou64
+    S64.close()
     S.close()
