@@ -2119,7 +2119,9 @@ __global__ void __launch_bounds__(kWave) k_lu_perm(const int32_t *__restrict__ b
 // 64 column loads back to back so the factor stream keeps many loads in
 // flight.  z lives in LDS.
 constexpr int kSolveB = 64;
+constexpr int kSolveNW = 3;      // waves per leaf block in k_lu_solve
 constexpr int kHalf = 32;
+constexpr int kDiagH = 32;       // diagonal-tile factors in flight per pass
 
 // s - sum_{c < bs} A[c*m] * zt[c] for one row, loads issued 32 at a time
 // A factor entry that is used only when `use` holds.  Float factors are
@@ -2225,18 +2227,18 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
             if (bi > 0 && in)                  // tile bi-1's contribution to tile bi
                 zl = gemv_row(A + (size_t)(kb - kSolveB) * m + kb + lane, m, kSolveB, zs + kb - kSolveB, zl);
 #pragma unroll 1
-            for (int h = 0; h < kSolveB; h += kHalf) {
-                FT tf[kHalf];
+            for (int h = 0; h < kSolveB; h += kDiagH) {
+                FT tf[kDiagH];
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c)
+                for (int c = 0; c < kDiagH; ++c)
                     tf[c] = factor_ld(A, (size_t)(kb + h + c) * m + kb + lane,
                                       (size_t)(kb + min(h + c, bs - 1)) * m + kb + min(lane, bs - 1),
                                       in && h + c < lane);
-                double tr[kHalf];
+                double tr[kDiagH];
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
+                for (int c = 0; c < kDiagH; ++c) tr[c] = (double)tf[c];
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c) zl -= tr[c] * readlane_d(zl, h + c);
+                for (int c = 0; c < kDiagH; ++c) zl -= tr[c] * readlane_d(zl, h + c);
             }
             if (in) zs[kb + lane] = zl;
         }
@@ -2253,18 +2255,18 @@ __global__ void __launch_bounds__(64 * NW) k_lu_solve(int n, const int32_t *__re
             if (bi < nb - 1 && in)             // tile bi+1's contribution to tile bi
                 zl = gemv_row(A + (size_t)kb1 * m + kb + lane, m, bs1, zs + kb1, zl);
 #pragma unroll 1
-            for (int h = kSolveB - kHalf; h >= 0; h -= kHalf) {
-                FT tf[kHalf];
+            for (int h = kSolveB - kDiagH; h >= 0; h -= kDiagH) {
+                FT tf[kDiagH];
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c)
+                for (int c = 0; c < kDiagH; ++c)
                     tf[c] = factor_ld(A, (size_t)(kb + h + c) * m + kb + lane,
                                       (size_t)(kb + min(h + c, bs - 1)) * m + kb + min(lane, bs - 1),
                                       in && h + c >= lane && h + c < bs);
-                double tr[kHalf];
+                double tr[kDiagH];
 #pragma unroll
-                for (int c = 0; c < kHalf; ++c) tr[c] = (double)tf[c];
+                for (int c = 0; c < kDiagH; ++c) tr[c] = (double)tf[c];
 #pragma unroll
-                for (int c = kHalf - 1; c >= 0; --c) {
+                for (int c = kDiagH - 1; c >= 0; --c) {
                     if (lane == h + c && in) zl = zl / tr[c];
                     const double zc = readlane_d(zl, h + c);
                     if (lane < h + c) zl -= tr[c] * zc;
@@ -3546,12 +3548,18 @@ void msolve_dev(tabi_ctx *c, const double *dr, double *dz)
     }
     const size_t shm = sizeof(double) * (size_t)c->mmax;
     KTimer kt(c, 4);
+    // three waves per leaf: with the kernel's registers a CU holds four such
+    // workgroups (three of four waves), and more leaves in flight beat more
+    // waves per leaf (measured on the 484k-face surface: 0.467 ms against
+    // 0.497 ms with four waves, 1.01 ms with eight)
     if (c->lu32.p)
-        k_lu_solve<4, float><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
-            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu32.p, c->perm.p, dr, dz, c->gate, c->blk_sf.p);
+        k_lu_solve<kSolveNW, float><<<c->nleaf, kSolveNW * kWave, shm, c->s>>>(c->n, c->lorder.p,
+            c->blk_ibeg.p, c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu32.p, c->perm.p, dr, dz, c->gate,
+            c->blk_sf.p);
     else
-        k_lu_solve<4, double><<<c->nleaf, 4 * kWave, shm, c->s>>>(c->n, c->lorder.p, c->blk_ibeg.p,
-            c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz, c->gate, c->blk_sf.p);
+        k_lu_solve<kSolveNW, double><<<c->nleaf, kSolveNW * kWave, shm, c->s>>>(c->n, c->lorder.p,
+            c->blk_ibeg.p, c->blk_m.p, c->blk_off.p, c->piv_off.p, c->lu.p, c->perm.p, dr, dz, c->gate,
+            c->blk_sf.p);
     HIPCHK(hipGetLastError());
 }
 
