@@ -2325,6 +2325,58 @@ __global__ void __launch_bounds__(256) k_rhs(int n, int nq, const double4 *__res
     }
 }
 
+// Mixed precision: the right-hand side with FP32 pair terms.  A charge record
+// keeps its position in FP64 (the differences are taken in FP64, then
+// rounded) and its charge in FP32; per face the terms of 16 charges are
+// summed in FP32, those sums in FP64.  b1 += q/r, b2 += q (n.d)/r^3 as k_rhs
+// forms them (d = charge - face).
+struct QRec32 {
+    double x, y, z;
+    float q, pad;
+};
+__global__ void k_charge_rec32(int nq, const double *__restrict__ qpos, const double *__restrict__ q,
+                               QRec32 *__restrict__ rec)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nq) rec[k] = QRec32{qpos[3 * k], qpos[3 * k + 1], qpos[3 * k + 2], (float)q[k], 0.f};
+}
+
+constexpr int kRhsSub = 16;      // charges per FP32 partial sum
+__global__ void __launch_bounds__(256) k_rhs32(int n, int nq, const QRec32 *__restrict__ qrec,
+    double eps_p, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double *__restrict__ b, int64_t ho)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    const float vx = valid ? (float)tnx[i] : 0.f, vy = valid ? (float)tny[i] : 0.f,
+                vz = valid ? (float)tnz[i] : 0.f;
+    double b1 = 0.0, b2 = 0.0;
+    for (int k0 = 0; k0 < nq; k0 += kRhsSub) {
+        const int k1 = min(nq, k0 + kRhsSub);
+        float s1 = 0.f, s2 = 0.f;
+#pragma unroll 4
+        for (int k = k0; k < k1; ++k) {
+            const QRec32 c = qrec[k];
+            const float dx = (float)(c.x - px), dy = (float)(c.y - py), dz = (float)(c.z - pz);
+            const float inv = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+            const float qi = c.q * inv;
+            const float nd = fmaf(vx, dx, fmaf(vy, dy, vz * dz));
+            s1 += qi;
+            s2 = fmaf(qi, nd * (inv * inv), s2);
+        }
+        b1 += (double)s1;
+        b2 += (double)s2;
+    }
+    if (valid) {
+        const double s = 1.0 / (4.0 * kPi * eps_p);
+        b[i] = b1 * s;
+        b[ho + i] = b2 * s;
+    }
+}
+
 // Potential at each charge from the surface solution (potential_molecule,
 // src/main.f90:627-655): tasks of (64 charges, a run of kEnergyChunk faces), one
 // wave each, faces read wave-uniformly from the source records (which hold
@@ -3038,6 +3090,73 @@ __global__ void __launch_bounds__(256) k_rhsb(int n, int nq, int R, const double
                 b1[k] += sq[3 + k][c] * inv;
                 b2[k] += sq[3 + k][c] * ct * inv * inv;
             }
+        }
+    }
+    if (valid) {
+        const double s = 1.0 / (4.0 * kPi * eps_p);
+#pragma unroll
+        for (int k = 0; k < RC; ++k) {
+            if (k >= rc) break;
+            double *br = b + (size_t)(r0 + k) * 2 * n;
+            br[i] = b1[k] * s;
+            br[n + i] = b2[k] * s;
+        }
+    }
+}
+
+// k_rhsb with FP32 pair terms (mixed precision), summed as k_rhs32 sums them,
+// so each set's right-hand side equals its single-set one
+template <int RC>
+__global__ void __launch_bounds__(256) k_rhsb32(int n, int nq, int R, const double *__restrict__ qpos,
+    const double *__restrict__ q, double eps_p, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ tnx, const double *__restrict__ tny,
+    const double *__restrict__ tnz, double *__restrict__ b)
+{
+    static_assert(256 % kRhsSub == 0, "charge batches must hold whole FP32 partials");
+    __shared__ double sq[3][256];
+    __shared__ float sqq[RC][256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r0 = blockIdx.y * RC, rc = min(RC, R - r0);
+    const bool valid = i < n;
+    const double px = valid ? tx[i] : 0.0, py = valid ? ty[i] : 0.0, pz = valid ? tz[i] : 0.0;
+    const float vx = valid ? (float)tnx[i] : 0.f, vy = valid ? (float)tny[i] : 0.f,
+                vz = valid ? (float)tnz[i] : 0.f;
+    double b1[RC], b2[RC];
+#pragma unroll
+    for (int k = 0; k < RC; ++k) { b1[k] = 0.0; b2[k] = 0.0; }
+    for (int k0 = 0; k0 < nq; k0 += 256) {
+        const int kk = k0 + threadIdx.x;
+        __syncthreads();
+        if (kk < nq) {
+            sq[0][threadIdx.x] = qpos[3 * kk];
+            sq[1][threadIdx.x] = qpos[3 * kk + 1];
+            sq[2][threadIdx.x] = qpos[3 * kk + 2];
+#pragma unroll
+            for (int k = 0; k < RC; ++k)
+                sqq[k][threadIdx.x] = k < rc ? (float)q[(size_t)(r0 + k) * nq + kk] : 0.f;
+        }
+        __syncthreads();
+        const int cnt = min(256, nq - k0);
+        for (int c0 = 0; c0 < cnt; c0 += kRhsSub) {
+            const int c1 = min(cnt, c0 + kRhsSub);
+            float s1[RC], s2[RC];
+#pragma unroll
+            for (int k = 0; k < RC; ++k) { s1[k] = 0.f; s2[k] = 0.f; }
+            for (int c = c0; c < c1; ++c) {
+                const float dx = (float)(sq[0][c] - px), dy = (float)(sq[1][c] - py),
+                            dz = (float)(sq[2][c] - pz);
+                const float inv = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+                const float g = fmaf(vx, dx, fmaf(vy, dy, vz * dz)) * (inv * inv);
+#pragma unroll
+                for (int k = 0; k < RC; ++k) {
+                    const float qi = sqq[k][c] * inv;
+                    s1[k] += qi;
+                    s2[k] = fmaf(qi, g, s2[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < RC; ++k) { b1[k] += (double)s1[k]; b2[k] += (double)s2[k]; }
         }
     }
     if (valid) {
@@ -5609,6 +5728,16 @@ void rhs_dev(tabi_ctx *c, int nq, const double *qpos, const double *q, double ep
 {
     if (nf < 0) nf = c->n;
     DevIn dqp(qpos, (size_t)3 * nq, dev, c->s), dq(q, nq, dev, c->s);
+    if (c->prm.precision == 1) {
+        DBuf<QRec32> qrec;
+        qrec.alloc(std::max(nq, 1));
+        if (nq > 0) k_charge_rec32<<<nblk(nq, 256), 256, 0, c->s>>>(nq, dqp.p, dq.p, qrec.p);
+        k_rhs32<<<nblk(nf, 256), 256, 0, c->s>>>(nf, nq, qrec.p, eps_p, c->tx.p + f0, c->ty.p + f0,
+            c->tz.p + f0, c->tnx.p + f0, c->tny.p + f0, c->tnz.p + f0, db + f0, c->n);
+        HIPCHK(hipGetLastError());
+        if (sync) HIPCHK(hipStreamSynchronize(c->s));
+        return;
+    }
     DBuf<double4> qrec;
     qrec.alloc(std::max(nq, 1));
     if (nq > 0) k_charge_rec<<<nblk(nq, 256), 256, 0, c->s>>>(nq, dqp.p, dq.p, qrec.p);
@@ -5875,8 +6004,12 @@ void rhs_block_dev(tabi_ctx *c, int R, int nq, const double *qpos, const double 
     DBuf<double> dqp, dq;
     dqp.upload(qpos, (size_t)3 * std::max(nq, 1), c->s);
     dq.upload(q, (size_t)R * std::max(nq, 1), c->s);
-    k_rhsb<RC><<<dim3(nblk(c->n, 256), (R + RC - 1) / RC), 256, 0, c->s>>>(c->n, nq, R, dqp.p, dq.p,
-        eps_p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, dB);
+    if (c->prm.precision == 1)
+        k_rhsb32<RC><<<dim3(nblk(c->n, 256), (R + RC - 1) / RC), 256, 0, c->s>>>(c->n, nq, R, dqp.p,
+            dq.p, eps_p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, dB);
+    else
+        k_rhsb<RC><<<dim3(nblk(c->n, 256), (R + RC - 1) / RC), 256, 0, c->s>>>(c->n, nq, R, dqp.p, dq.p,
+            eps_p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, dB);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
 }

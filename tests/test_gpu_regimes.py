@@ -197,8 +197,8 @@ def test_restart_and_cap_through_block_and_many_surface_forms():
 def test_mixed_precision_on_the_molecule():
     """Mixed precision on the SES molecule: matvec within 1e-5 of the
     reference's FP64 product, the solve converged with the energy within 1e-4,
-    and the mixed context's energy (FP32 pair terms) within 2e-6 of the FP64
-    evaluation of the same solution."""
+    and the mixed context's energy and right-hand side (FP32 pair terms)
+    within 2e-6 and 1e-6 of the FP64 evaluations."""
     P, _, g = case("mol150_restart")
     S = hip.Surface(P.params(precision=1), P.xyz, P.nrm, P.area)
     s = int(g["stride"])
@@ -212,6 +212,10 @@ def test_mixed_precision_on_the_molecule():
     # the energy of a mixed context takes its pair terms in FP32: against the
     # FP64 evaluation of the same solution vector
     S64 = hip.Surface(P.params(), P.xyz, P.nrm, P.area)
+    # and so does its right-hand side
+    b64 = S64.rhs(P.qpos, P.q, P.usr.epsp)
+    print(f"[gpu] FP32-term right-hand side vs FP64: rel L2 {rel_l2(b, b64):.2e}", flush=True)
+    assert rel_l2(b, b64) <= 1e-6
     sol64, cou64 = S64.energy(P.qpos, P.q, P.usr.epsp, x)
     _, cou = S.energy(P.qpos, P.q, P.usr.epsp, x)
     print(f"[gpu] FP32-term energy vs FP64: rel {abs(sol - sol64) / abs(sol64):.2e}", flush=True)
