@@ -40,6 +40,11 @@ typedef struct {
                             bit 2: diagonal-scaling preconditioner MSOLVEprec
                             (src/main.f90:532-542, usrdata.in iprcd != 1)
                             instead of the leaf-block LU (no factorisation);
+                            bit 4: GMRES with its control on the device, one
+                            captured graph per restart cycle (same results);
+                            bit 5 (mixed precision): round the finished FP64
+                            leaf factors to FP32 in one pass instead of as each
+                            entry is finished (same values);
                             bits 8-11 (t): field kernels cut the work into about
                             2^(10+t) tasks per launch (t = 0: 2^16) */
 } tabi_params;

@@ -1394,7 +1394,8 @@ __device__ __forceinline__ int wave_min_i(int v)
 // of line, so the common pass keeps its registers.
 template <int RPT, int kPanelThreads>
 __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nbe,
-    double *__restrict__ A, int32_t *__restrict__ pv, int32_t *__restrict__ info_l,
+    double *__restrict__ A, float *__restrict__ A32, int32_t *__restrict__ pv,
+    int32_t *__restrict__ info_l,
     double (*s_best)[kPanelThreads / kWave], int (*s_bi)[kPanelThreads / kWave],
     double (*s_row)[2][kLuNB], double *__restrict__ s_cr, int32_t *s_piv)
 {
@@ -1502,7 +1503,10 @@ __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nb
         if (row < m) {
 #pragma unroll
             for (int c = 0; c < kLuNB; ++c)
-                if (c < nbe) A[(size_t)(k0 + c) * m + row] = p[j][c];
+                if (c < nbe) {
+                    A[(size_t)(k0 + c) * m + row] = p[j][c];
+                    if (A32) A32[(size_t)(k0 + c) * m + row] = (float)p[j][c];
+                }
         }
     }
 }
@@ -1528,8 +1532,8 @@ __device__ __attribute__((noinline)) void lu_panel_pivoted(int k0, int m, int nb
 // destinations (a permutation of the same rows), so no barrier is needed.
 constexpr int kSwapCols = 16;
 template <int kPanelThreads>
-__device__ __attribute__((noinline)) void lu_swap_columns(int k0, int m, int nbe, double *A, int nr,
-                                                          const int *rows, const int *src)
+__device__ __attribute__((noinline)) void lu_swap_columns(int k0, int m, int nbe, double *A, float *A32,
+                                                          int nr, const int *rows, const int *src)
 {
     if (nr <= 0) return;                      // a zero pivot without interchanges
     const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
@@ -1552,12 +1556,18 @@ __device__ __attribute__((noinline)) void lu_swap_columns(int k0, int m, int nbe
 #pragma unroll
         for (int q = 0; q < kSwapCols; ++q)
             if (q < nr) col[dr[q]] = v[q];
+        if (A32) {
+            float *col32 = A32 + (size_t)j * m;
+#pragma unroll
+            for (int q = 0; q < kSwapCols; ++q)
+                if (q < nr) col32[dr[q]] = (float)v[q];
+        }
     }
 }
 
 template <int kPanelThreads>
-__device__ __attribute__((noinline)) void lu_swap_rest(int k0, int m, int nbe, double *A, int nr,
-                                                       const int *rows, const int *src)
+__device__ __attribute__((noinline)) void lu_swap_rest(int k0, int m, int nbe, double *A, float *A32,
+                                                       int nr, const int *rows, const int *src)
 {
     const int t = threadIdx.x;
     const int K0 = k0 - k0 % kLuW, send = min(m, K0 + kLuW);
@@ -1579,7 +1589,10 @@ __device__ __attribute__((noinline)) void lu_swap_rest(int k0, int m, int nbe, d
 #pragma unroll
         for (int i = 0; i < kTs; ++i) {
             const int e = t + kPanelThreads * i, q = e % 64, j = j0 + e / 64;
-            if (q < nr && j < m && (j < K0 || j >= k0 + nbe)) A[(size_t)j * m + rows[q]] = v[i];
+            if (q < nr && j < m && (j < K0 || j >= k0 + nbe)) {
+                A[(size_t)j * m + rows[q]] = v[i];
+                if (A32) A32[(size_t)j * m + rows[q]] = (float)v[i];
+            }
         }
         __syncthreads();
     }
@@ -1590,7 +1603,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
     double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info,
-    int32_t *__restrict__ swrec)
+    int32_t *__restrict__ swrec, float *__restrict__ lu32)
 {
     constexpr int NW = kPanelThreads / kWave;
     // per column: the waves' maxima and rows, and the two exchanged rows;
@@ -1605,6 +1618,9 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     const int m = blk_m[l];
     const int nbe = min(kLuNB, m - k0);
     double *A = lu + blk_off[l];
+    // the FP32 copy of the factors (mixed precision), written as entries are
+    // finished: the panel's columns here, their later interchanges below
+    float *A32 = lu32 ? lu32 + blk_off[l] : nullptr;
     int32_t *pv = piv + piv_off[l];
     double p[RPT][kLuNB];
 #pragma unroll
@@ -1685,12 +1701,15 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
         if (row < m) {
 #pragma unroll
             for (int c = 0; c < kLuNB; ++c)
-                if (c < nbe) A[(size_t)(k0 + c) * m + row] = p[j][c];
+                if (c < nbe) {
+                    A[(size_t)(k0 + c) * m + row] = p[j][c];
+                    if (A32) A32[(size_t)(k0 + c) * m + row] = (float)p[j][c];
+                }
         }
     }
     } else {
         static_assert(2 * NW <= kLuNB + 1, "candidate rows exceed s_u");
-        lu_panel_pivoted<RPT, kPanelThreads>(k0, m, nbe, A, pv, info + l, s_best, s_bi, s_row,
+        lu_panel_pivoted<RPT, kPanelThreads>(k0, m, nbe, A, A32, pv, info + l, s_best, s_bi, s_row,
                                              &s_u[0][0], s_piv);
     }
     __syncthreads();
@@ -1709,7 +1728,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
     const int nr = s_prow;
     if (nr <= kSwapCols) {
         // few touched rows: every column at once, one thread per column
-        lu_swap_columns<kPanelThreads>(k0, m, nbe, A, nr, s_dst, s_src);
+        lu_swap_columns<kPanelThreads>(k0, m, nbe, A, A32, nr, s_dst, s_src);
         return;
     }
     if (ncl > 0) {
@@ -1725,13 +1744,16 @@ __global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
 #pragma unroll
         for (int i = 0; i < kEs; ++i) {
             const int e = t + kPanelThreads * i, q = e % 64, jj = e / 64;
-            if (q < nr && jj < ncl) A[(size_t)(K0 + jj) * m + s_dst[q]] = v[i];
+            if (q < nr && jj < ncl) {
+                A[(size_t)(K0 + jj) * m + s_dst[q]] = v[i];
+                if (A32) A32[(size_t)(K0 + jj) * m + s_dst[q]] = (float)v[i];
+            }
         }
         __syncthreads();
     }
     // and on the block's column tiles outside the strip that k_lu_swap_trsm
     // does not take
-    lu_swap_rest<kPanelThreads>(k0, m, nbe, A, nr, s_dst, s_src);
+    lu_swap_rest<kPanelThreads>(k0, m, nbe, A, A32, nr, s_dst, s_src);
 }
 
 // Inverse of the unit lower triangle L11 of panel k0 (rows and columns
@@ -1799,7 +1821,8 @@ __global__ void __launch_bounds__(64) k_lu_linv(int k0, int first,
 // interchanges only.
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    double *__restrict__ lu, const int32_t *__restrict__ swrec, const double *__restrict__ linv)
+    double *__restrict__ lu, const int32_t *__restrict__ swrec, const double *__restrict__ linv,
+    float *__restrict__ lu32)
 {
     __shared__ double Li[kLuNB][kLuNB + 1];          // L11^-1 (row-major)
     __shared__ double T[2 * kLuNB][64 + 1];
@@ -1876,10 +1899,16 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(int k0, const int2 *__rest
         }
     }
     __syncthreads();
+    // the panel rows of the strip's columns are final here (U12), so is every
+    // interchanged entry left of the strip; the FP32 copy takes them all
+    float *A32 = lu32 ? lu32 + blk_off[l] : nullptr;
 #pragma unroll
     for (int i = 0; i < kTs; ++i) {
         const int e = t + 256 * i, q = e % 64, j = j0 + e / 64;
-        if (q < nr && col_ok(j)) A[(size_t)j * m + rows[q]] = T[q][e / 64];
+        if (q < nr && col_ok(j)) {
+            A[(size_t)j * m + rows[q]] = T[q][e / 64];
+            if (A32) A32[(size_t)j * m + rows[q]] = (float)T[q][e / 64];
+        }
     }
 }
 
@@ -1898,7 +1927,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int kStS = kLuW + 2;   // LDS column stride in doubles: conflict-free operand reads
 __global__ void __launch_bounds__(256, 2) k_lu_strip_trsm(int K0, const int2 *__restrict__ tiles,
     const int32_t *__restrict__ blk_m, const int64_t *__restrict__ blk_off,
-    double *__restrict__ lu, const double *__restrict__ linv)
+    double *__restrict__ lu, const double *__restrict__ linv, float *__restrict__ lu32)
 {
     __shared__ double T[64 * kStS];
     const int2 tl = tiles[blockIdx.x];           // {block, column tile beyond the strip}
@@ -1975,10 +2004,14 @@ __global__ void __launch_bounds__(256, 2) k_lu_strip_trsm(int K0, const int2 *__
             for (int v = 0; v < 4; ++v) X[kLuNB * p + 16 * h + lk + 4 * v] = x[h][v];
     }
     __syncthreads();
+    float *A32 = lu32 ? lu32 + blk_off[l] : nullptr;   // U12 beyond the strip is final
 #pragma unroll
     for (int i = 0; i < kTs; ++i) {
         const int e = t + 256 * i, c = e / kLuW;
-        if (j0 + c < m) A[(size_t)(j0 + c) * m + K0 + e % kLuW] = T[c * kStS + e % kLuW];
+        if (j0 + c < m) {
+            A[(size_t)(j0 + c) * m + K0 + e % kLuW] = T[c * kStS + e % kLuW];
+            if (A32) A32[(size_t)(j0 + c) * m + K0 + e % kLuW] = (float)T[c * kStS + e % kLuW];
+        }
     }
 }
 
@@ -4236,6 +4269,16 @@ void precond_factor(tabi_ctx *c)
     const int nl = c->nleaf;
     const std::vector<int32_t> &bm = c->h_blk_m;
     const int64_t off = (int64_t)c->lu.n;
+    // mixed precision: the solves read FP32 factors.  The kernels that finish
+    // an entry (the panels, the in-strip and strip-end solves, the
+    // interchanges) also write its FP32 rounding; flags bit 5 instead rounds
+    // the finished FP64 factors in one pass (the same values)
+    const bool mixed = c->prm.precision == 1, one_pass = (c->prm.flags & 32) != 0;
+    float *l32 = nullptr;
+    if (mixed) {
+        c->lu32.alloc(off);
+        if (!one_pass) l32 = c->lu32.p;
+    }
     // the tile lists below are uploaded on a side stream that waits for the
     // work queued so far but not for the assembly, so the copies overlap it
     c->pre_ev.make(5);
@@ -4334,16 +4377,16 @@ void precond_factor(tabi_ctx *c)
             }
             if (cls == 2048)
                 k_lu_panel<2, 1024><<<cnt, 1024, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p, l32);
             else if (cls == 1024)
                 k_lu_panel<1, 1024><<<cnt, 1024, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p, l32);
             else if (cls == 512)
                 k_lu_panel<1, 512><<<cnt, 512, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p, l32);
             else
                 k_lu_panel<1, 256><<<cnt, 256, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p,
-                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p);
+                    c->blk_off.p, c->piv_off.p, c->lu.p, c->piv.p, c->lu_info.p, c->lu_swrec.p, l32);
             // the class's panel inverses follow its panels on the same stream,
             // beside the other classes' panels
             k_lu_linv<<<cnt, 64, 0, ps>>>(k0, q, dlorder_p, c->blk_m.p, c->blk_off.p, c->lu.p,
@@ -4356,7 +4399,7 @@ void precond_factor(tabi_ctx *c)
         const int64_t nsw = sw_off[step + 1] - sw_off[step];
         if (nsw > 0)
             k_lu_swap_trsm<<<(unsigned)nsw, 256, 0, c->s>>>(k0, dsw.p + sw_off[step], c->blk_m.p,
-                c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p);
+                c->blk_off.p, c->lu.p, c->lu_swrec.p, c->lu_linv.p, l32);
         const int64_t nt = tl_off[step + 1] - tl_off[step];
         if (nt > 0)
             k_lu_gemm<kLuNB><<<(unsigned)nt, 256, 0, c->s>>>(k0, dtl.p + tl_off[step], c->blk_m.p,
@@ -4364,7 +4407,7 @@ void precond_factor(tabi_ctx *c)
         const int64_t nts = ts_off[step + 1] - ts_off[step];
         if (nts > 0)
             k_lu_strip_trsm<<<(unsigned)nts, 256, 0, c->s>>>(k0 + kLuNB - kLuW, dts.p + ts_off[step],
-                c->blk_m.p, c->blk_off.p, c->lu.p, c->lu_linv.p);
+                c->blk_m.p, c->blk_off.p, c->lu.p, c->lu_linv.p, l32);
         const int64_t nto = tlo_off[step + 1] - tlo_off[step];
         if (nto > 0)
             k_lu_gemm<kLuW><<<(unsigned)nto, 256, 0, c->s>>>(k0 + kLuNB - kLuW, dtlo.p + tlo_off[step],
@@ -4374,10 +4417,7 @@ void precond_factor(tabi_ctx *c)
     c->perm.alloc(c->piv.n);
     k_lu_perm<<<nl, kWave, 0, c->s>>>(c->blk_m.p, c->piv_off.p, c->piv.p, c->perm.p);
     HIPCHK(hipGetLastError());
-    if (c->prm.precision == 1) {
-        // mixed precision: the FP64 factors are rounded once to FP32 for the
-        // solves, which then stream half the bytes
-        c->lu32.alloc(off);
+    if (mixed && one_pass) {
         k_to_float<<<to_float_grid(off), 256, 0, c->s>>>(off, c->lu.p, c->lu32.p);
         HIPCHK(hipGetLastError());
     }

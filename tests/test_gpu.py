@@ -399,6 +399,37 @@ def test_mixed_precision_full_solve_on_molecule():
     assert abs(r32["soleng"] - r64["soleng"]) / abs(r64["soleng"]) < 1e-4
 
 
+@pytest.mark.parametrize("n,box,amax,leaf", [(3000, 12.0, 2.0, 500), (1500, 6.0, 6.0, 400)])
+def test_fp32_factors_written_as_finished_equal_one_pass_rounding(n, box, amax, leaf):
+    """Mixed precision: the FP32 factors that the LU kernels write as each
+    entry is finished (panels, in-strip and strip-end solves, the interchanges
+    of later panels) are the rounding of the finished FP64 factors -- the
+    preconditioner solve equals, bit for bit, the one whose factors are rounded
+    in one pass after the factorisation (flags bit 5), on blocks that pivot on
+    many columns and on an SES molecule's blocks."""
+    print(f"[gpu] FP32 factors n={n} leaf={leaf}", flush=True)
+    xyz, nrm, area = _pivoting_faces(n, box, amax, 7)
+    eps, kappa = 80.0, 0.1257
+    r = np.random.default_rng(3).standard_normal(2 * n)
+    S = hip.Surface(hip.make_params(eps, kappa, 0.8, 3, leaf, precision=1), xyz, nrm, area)
+    S1 = hip.Surface(hip.make_params(eps, kappa, 0.8, 3, leaf, precision=1, flags=32), xyz, nrm,
+                     area)
+    assert np.array_equal(S.msolve(r), S1.msolve(r))
+    S.close()
+    S1.close()
+    from tabipb import surface
+    mx, mq, mr = surface.synth_molecule(120, seed=5)
+    v, nv, f = surface.ses_mesh(mx, mr, probe=1.4, h=0.5)
+    f, _ = driver.cull_faces(v, f)
+    fx, fn, fa = driver.face_geometry(v, nv, f)
+    S = hip.Surface(hip.make_params(eps, kappa, 0.8, 3, 300, precision=1), fx, fn, fa)
+    S1 = hip.Surface(hip.make_params(eps, kappa, 0.8, 3, 300, precision=1, flags=32), fx, fn, fa)
+    r = np.random.default_rng(4).standard_normal(2 * S.n)
+    assert np.array_equal(S.msolve(r), S1.msolve(r))
+    S.close()
+    S1.close()
+
+
 def test_singular_leaf_block_is_reported():
     """The reference ignores DGETRF's INFO (src/treecode3d_pb.f:1530); the
     library reports a singular leaf block as an error from set-up.  With
