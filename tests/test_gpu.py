@@ -430,6 +430,54 @@ def test_fp32_factors_written_as_finished_equal_one_pass_rounding(n, box, amax, 
     S1.close()
 
 
+def test_concurrent_solves_from_two_threads_equal_sequential():
+    """Two host threads each run full solves (their own contexts, streams from
+    the shared pool, device blocks from the shared cache) and GMRES on two
+    contexts at the same time: every result equals the same call made alone,
+    bit for bit.  The threads' calls overlap on the GPU; blocks a thread frees
+    while its kernels are pending must not reach the other thread's stream."""
+    import threading
+    names = ["sph5_born", "sph4_off_p6"]
+    Ps = [problem(nm) for nm in names]
+    prms = [P.params(precision=k % 2) for k, P in enumerate(Ps)]
+
+    def solve(k):
+        P = Ps[k]
+        return hip.solve(prms[k], P.xyz, P.nrm, P.area, P.qpos, P.q, P.usr.epsp, 1e-6)
+
+    ref = [solve(k) for k in range(2)]
+    Ss = [hip.Surface(prms[k], P.xyz, P.nrm, P.area) for k, P in enumerate(Ps)]
+    bs = [S.rhs(P.qpos, P.q, P.usr.epsp) for S, P in zip(Ss, Ps)]
+    gref = [S.gmres(b, tol=1e-8) for S, b in zip(Ss, bs)]
+    out = [[None] * 3 for _ in range(2)]
+    errs = []
+
+    def work(k):
+        try:
+            for rep in range(3):
+                r = solve(k)
+                g = Ss[k].gmres(bs[k], tol=1e-8)
+                out[k][rep] = (r, g)
+        except Exception as e:      # reported by the main thread
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=120)
+    assert not any(x.is_alive() for x in th), "a solve thread did not finish"
+    assert not errs, errs
+    for k in range(2):
+        for r, g in out[k]:
+            assert (r["iter"], r["ierr"]) == (ref[k]["iter"], ref[k]["ierr"])
+            assert np.array_equal(r["x"], ref[k]["x"])
+            assert r["soleng"] == ref[k]["soleng"] and r["couleng"] == ref[k]["couleng"]
+            assert g[1:] == gref[k][1:] and np.array_equal(g[0], gref[k][0])
+    for S in Ss:
+        S.close()
+
+
 def test_singular_leaf_block_is_reported():
     """The reference ignores DGETRF's INFO (src/treecode3d_pb.f:1530); the
     library reports a singular leaf block as an error from set-up.  With
