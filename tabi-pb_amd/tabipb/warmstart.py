@@ -203,6 +203,8 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=8192)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--workers", type=int, default=8)
+    t.add_argument("--min-atoms", type=int, default=300)
+    t.add_argument("--max-atoms", type=int, default=3000)
     t.add_argument("--out", default=DEFAULT_MODEL)
     e = sub.add_parser("evaluate")
     e.add_argument("--model", default=DEFAULT_MODEL)
@@ -214,12 +216,12 @@ def main(argv=None):
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
         seeds = list(range(a.first_seed, a.first_seed + a.nmol))
-        sizes = molecule_sizes(seeds)
+        sizes = molecule_sizes(seeds, a.min_atoms, a.max_atoms)
         feat, tgt = build_dataset(seeds, sizes, a.nsample, a.workers, log)
         P, losses = fit(feat, tgt, a.epochs, a.batch, a.lr, log=log)
         save(a.out, P, dict(nmol=a.nmol, first_seed=a.first_seed, nsample=a.nsample,
-                            epochs=a.epochs, batch=a.batch, lr=a.lr,
-                            losses=[float(x) for x in losses]))
+                            epochs=a.epochs, batch=a.batch, lr=a.lr, min_atoms=a.min_atoms,
+                            max_atoms=a.max_atoms, losses=[float(x) for x in losses]))
         log(f"[warmstart] model written to {a.out}")
     else:
         seeds = list(range(a.first_seed, a.first_seed + a.nmol))
