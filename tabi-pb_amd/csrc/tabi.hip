@@ -404,6 +404,28 @@ struct tabi_ctx {
     }
     // stats
     int64_t p_near = 0, p_far = 0, p_mom = 0, n_far_items = 0, n_near_items = 0;
+    // a second stream for the near field, which runs beside the moments and
+    // the far field of a product (taken from the stream pool on first use;
+    // declared last, so it is synchronised before any buffer is freed)
+    struct AuxStream {
+        hipStream_t s = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+        AuxStream() = default;
+        AuxStream(const AuxStream &) = delete;
+        AuxStream &operator=(const AuxStream &) = delete;
+        void ensure()
+        {
+            if (!fork) HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            if (!join) HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+            if (!s) s = StreamPool::acquire();
+        }
+        ~AuxStream()
+        {
+            StreamPool::release(s);
+            if (fork) (void)hipEventDestroy(fork);
+            if (join) (void)hipEventDestroy(join);
+        }
+    } aux;
 };
 
 namespace {
@@ -1646,8 +1668,12 @@ __device__ __attribute__((noinline)) void lu_swap_rest(int k0, int m, int nbe, d
     }
 }
 
+// At least three waves per SIMD: left to itself the compiler gives the 256-
+// and 512-thread forms 248 registers (two waves per SIMD); capped at 168,
+// more panels are in flight (measured on the 484k-face surface: set-up
+// 19.9 -> 19.6 ms; four waves per SIMD, 128 registers, was slower).
 template <int RPT, int kPanelThreads>
-__global__ void __launch_bounds__(kPanelThreads) k_lu_panel(int k0, int first,
+__global__ void __launch_bounds__(kPanelThreads, 3) k_lu_panel(int k0, int first,
     const int32_t *__restrict__ lorder, const int32_t *__restrict__ blk_m,
     const int64_t *__restrict__ blk_off, const int32_t *__restrict__ piv_off,
     double *__restrict__ lu, int32_t *__restrict__ piv, int32_t *__restrict__ info,
@@ -3696,24 +3722,50 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
         HIPCHK(hipMemcpyAsync(c->vx.p, dx, sizeof(double) * 2 * c->n, hipMemcpyDeviceToDevice, c->s));
         x = c->vx.p;
     }
-    { KTimer kt(c, 0); dispatch_moments(c, x); }
+    // The near field reads x and the geometry only, so it can run beside the
+    // moments (short, latency-bound launches) and the far field on a second
+    // stream, which fills the CUs those leave idle; the two write separate
+    // partial arrays, so the product is the same either way.  Serial on one
+    // stream when kernels are timed one by one or the stream is being
+    // captured into a graph.
+    bool side = c->n_near_task > 0 && c->nmom > 0 && c->n_far_task > 0 && !c->prof;
+    if (side) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(c->s, &cs));
+        side = cs == hipStreamCaptureStatusNone;
+    }
+    hipStream_t ns = c->s;
+    if (side) {
+        c->aux.ensure();
+        ns = c->aux.s;
+        HIPCHK(hipEventRecord(c->aux.fork, c->s));
+        HIPCHK(hipStreamWaitEvent(ns, c->aux.fork, 0));
+    } else {
+        KTimer kt(c, 0);
+        dispatch_moments(c, x);
+    }
     if (c->n_near_task) {
         KTimer kt(c, 1);
         if (c->prm.precision == 1) {
-            k_src32_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src32.p, c->gate,
-                                                               c->group_sf.p);
-            k_near32<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
+            k_src32_weights<<<nblk(c->n, 256), 256, 0, ns>>>(c->n, c->tarea.p, x, c->src32.p, c->gate,
+                                                             c->group_sf.p);
+            k_near32<<<c->n_near_task, 64, 0, ns>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src32.p, c->ngeo.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p,
                 (float)c->prm.kappa, (float)c->prm.eps, c->part_near.p, c->gate);
         } else {
-            k_src_weights<<<nblk(c->n, 256), 256, 0, c->s>>>(c->n, c->tarea.p, x, c->src8.p, c->gate,
-                                                             c->group_sf.p);
-            k_near<<<c->n_near_task, 64, 0, c->s>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
+            k_src_weights<<<nblk(c->n, 256), 256, 0, ns>>>(c->n, c->tarea.p, x, c->src8.p, c->gate,
+                                                           c->group_sf.p);
+            k_near<<<c->n_near_task, 64, 0, ns>>>(c->near_tasks.p, c->near_tgt.p, c->n_near_task, c->n,
                 c->src8.p, c->tx.p, c->ty.p, c->tz.p, c->tnx.p, c->tny.p, c->tnz.p, c->prm.kappa,
                 c->prm.eps, c->part_near.p, c->gate);
         }
     }
+    if (side) {
+        HIPCHK(hipEventRecord(c->aux.join, ns));
+        dispatch_moments(c, x);
+    }
     if (c->n_far_task) { KTimer kt(c, 2); dispatch_far(c); }
+    if (side) HIPCHK(hipStreamWaitEvent(c->s, c->aux.join, 0));
     const double pre1 = 0.5 * (1.0 + c->prm.eps), pre2 = 0.5 * (1.0 + 1.0 / c->prm.eps);
     {
         KTimer kt(c, 3);
@@ -7137,12 +7189,18 @@ void solve_core(const tabi_params *prm, int32_t n, const double *xyz, const doub
         g_times[3] = t_rhs; g_times[4] = ms(t2b, t3); g_times[5] = ms(t3, t4);
         g_times[6] = ms(t0, t4); g_times[7] = t_ws;
     } catch (...) {
+        StreamPool::release(c.aux.s);
+        c.aux.s = nullptr;
         StreamPool::release(s2);
         hipStream_t s = c.s;
         c.s = nullptr;
         StreamPool::release(s);
         throw;
     }
+    // back to the pool in the reverse order of their taking, so the next
+    // solve gets each stream in the same role
+    StreamPool::release(c.aux.s);
+    c.aux.s = nullptr;
     StreamPool::release(s2);
     hipStream_t s = c.s;
     c.s = nullptr;

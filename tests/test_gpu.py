@@ -478,6 +478,37 @@ def test_concurrent_solves_from_two_threads_equal_sequential():
         S.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_near_field_beside_far_field_equals_serial_order(precision):
+    """A product runs its near field on a second stream beside the moments and
+    the far field; with per-kernel timing on, the same kernels run one after
+    another on one stream.  Both orders give the same product, bit for bit,
+    product after product, on the host and the device entry points."""
+    P = problem("sph5_born")
+    S = hip.Surface(P.params(precision=precision), P.xyz, P.nrm, P.area)
+    st = S.stats()
+    assert st["p_near"] > 0 and st["p_far"] > 0 and st["p_mom"] > 0
+    rng = np.random.default_rng(7)
+    xs = [rng.standard_normal(2 * S.n) for _ in range(3)]
+    side = [S.matvec(x) for x in xs]
+    S.profile(True)
+    serial = [S.matvec(x) for x in xs]
+    S.profile(False)
+    again = [S.matvec(x) for x in xs]
+    dx = hip.DeviceArray.from_host(xs[1])
+    dy = hip.DeviceArray(2 * S.n)
+    S.matvec_dev(dx.ptr, dy.ptr)
+    S.sync()
+    dev = dy.get()
+    S.matvec_dev(dx.ptr, dx.ptr)             # in place: reads a copy of x
+    S.sync()
+    inplace = dx.get()
+    for a, b, c in zip(side, serial, again):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert np.array_equal(dev, serial[1]) and np.array_equal(inplace, serial[1])
+    S.close()
+
+
 def test_singular_leaf_block_is_reported():
     """The reference ignores DGETRF's INFO (src/treecode3d_pb.f:1530); the
     library reports a singular leaf block as an error from set-up.  With
