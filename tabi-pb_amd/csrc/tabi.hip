@@ -3725,15 +3725,10 @@ void matvec_dev(tabi_ctx *c, const double *dx, double *dy)
     // The near field reads x and the geometry only, so it can run beside the
     // moments (short, latency-bound launches) and the far field on a second
     // stream, which fills the CUs those leave idle; the two write separate
-    // partial arrays, so the product is the same either way.  Serial on one
-    // stream when kernels are timed one by one or the stream is being
-    // captured into a graph.
-    bool side = c->n_near_task > 0 && c->nmom > 0 && c->n_far_task > 0 && !c->prof;
-    if (side) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        HIPCHK(hipStreamIsCapturing(c->s, &cs));
-        side = cs == hipStreamCaptureStatusNone;
-    }
+    // partial arrays, so the product is the same either way.  Under graph
+    // capture the fork and the join become the graph's edges.  Serial on one
+    // stream when kernels are timed one by one.
+    const bool side = c->n_near_task > 0 && c->nmom > 0 && c->n_far_task > 0 && !c->prof;
     hipStream_t ns = c->s;
     if (side) {
         c->aux.ensure();
